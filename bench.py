@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""bench.py -- GPop/s of linear-GP fitness evaluation on one MI355X.
+
+One step = one generation's fitness evaluation of the whole population with
+inputs already resident in HBM (programs, offsets, dataset).  The default
+workload is the plain wave-per-individual evaluation of 1,048,576 random
+EXT8 programs (lengths U{16..256}, 8 registers) on 1,024 cases.  1 GP-op is
+one executed instruction on one fitness case (the reference's convention,
+tests/cuda_evo_bench.cu:5-7).
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+  python bench.py [--steps K] [--warmup W] [--workload plain|uneven|casebound]
+                  [--kernel plain|batched] [--programs P] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X published dense FP32 (vector) peak
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl, gpu_fit, target_s, threads):
+    """The oracle (C restatement of CpuEvaluator) timed on a bounded prefix of
+    the same population on this host; also checks the GPU fitness of that
+    prefix against the oracle's FP64-SSE fitness."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_py as O
+    isa = O.isa_from(wl.isa)
+    x = wl.dataset.inputs[: wl.dataset.N]
+    y = wl.dataset.targets[: wl.dataset.N]
+    # calibrate on a few programs, then size the sample for ~target_s
+    n_cal = 8
+    words, offs = wl.host_prefix(n_cal)
+    t0 = time.perf_counter()
+    O.eval_csr(isa, words, offs, x, y, threads=threads)
+    dt = max(time.perf_counter() - t0, 1e-4)
+    n = int(min(wl.n_programs, max(n_cal, n_cal * target_s / dt)))
+    words, offs = wl.host_prefix(n)
+    t0 = time.perf_counter()
+    fit, sse = O.eval_csr(isa, words, offs, x, y, threads=threads)
+    dt = time.perf_counter() - t0
+    ops = float(offs[-1] - offs[0]) * wl.dataset.N
+    g = gpu_fit[:n]
+    with np.errstate(all="ignore"):
+        rel = np.where(g == fit, 0.0, np.abs(g.astype(np.float64) - fit) /
+                       np.maximum(np.abs(fit.astype(np.float64)), 1e-300))
+    over = np.nonzero(rel > 1e-6)[0]
+    chaotic = 0
+    for i in over[:2000]:
+        ff, fd = O.dual_precision_fitness(isa, words[offs[i]:offs[i + 1]], x, y)
+        if abs(fd - ff) > 1e-6 * max(abs(ff), 1e-30):
+            chaotic += 1
+    return {
+        "gpops": ops / dt / 1e9, "programs": n, "seconds": dt, "threads": threads,
+        "max_rel_err": float(rel.max()) if rel.size else 0.0,
+        "within_1e-6": int((rel <= 1e-6).sum()), "over_1e-6": int(over.size),
+        "over_chaos_attributed": int(chaotic),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="plain", choices=["plain", "uneven", "casebound"])
+    ap.add_argument("--kernel", default="plain", choices=["plain", "batched"])
+    ap.add_argument("--programs", type=int, default=0, help="override population size")
+    ap.add_argument("--cases-per-lane", type=int, default=0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(populations shard across GPUs without communication; see DESIGN.md)",
+              file=sys.stderr)
+        return 2
+
+    import torch
+    from galop_amd import Context, eval_csr_device, workloads
+    from galop_amd._lib import KERNEL_BATCHED, KERNEL_PLAIN
+
+    ctx = Context(0)
+    if args.cases_per_lane:
+        ctx.set_tuning(args.cases_per_lane, 0)
+    t_setup = time.perf_counter()
+    kw = {"n_programs": args.programs} if args.programs else {}
+    make = {"plain": workloads.make_plain, "uneven": workloads.make_uneven,
+            "casebound": workloads.make_casebound}[args.workload]
+    wl = make(ctx, seed=0, **kw)
+    torch.cuda.synchronize()
+    log(f"[bench] workload {wl.name}: {wl.notes}; raw words {wl.raw_words}; "
+        f"setup {time.perf_counter() - t_setup:.1f}s")
+    P = wl.n_programs
+    fitness = torch.empty(P, dtype=torch.float32, device="cuda")
+    sse = torch.empty(P, dtype=torch.float64, device="cuda")
+    kernel = KERNEL_BATCHED if args.kernel == "batched" else KERNEL_PLAIN
+    stream = torch.cuda.current_stream().cuda_stream
+
+    def step():
+        eval_csr_device(wl.dds, wl.words_d, wl.offsets_d, P, fitness, sse, kernel=kernel,
+                        stream=stream)
+
+    for i in range(args.warmup):
+        t0 = time.perf_counter()
+        step()
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {i + 1}/{args.warmup}: {time.perf_counter() - t0:.3f}s")
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    wall0 = time.perf_counter()
+    ev0.record()
+    for i in range(args.steps):
+        step()
+        if args.steps > 1:
+            torch.cuda.synchronize()
+            log(f"[bench] step {i + 1}/{args.steps} done at {time.perf_counter() - wall0:.2f}s")
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    ms = ev0.elapsed_time(ev1) / args.steps
+    ops = wl.gp_ops_raw()
+    gpops = ops / (ms / 1e3) / 1e9
+    exec_gpops = float(wl.exec_words) * wl.n_cases / (ms / 1e3) / 1e9
+    log(f"[bench] {ms:.2f} ms/step, {gpops:.2f} GPop/s raw, {exec_gpops:.2f} exec; wall {wall:.2f}s")
+
+    props = torch.cuda.get_device_properties(0)
+    n_cu = props.multi_processor_count
+    achieved_tflops = ops / (ms / 1e3) / 1e12
+    line = {
+        "metric": "GPop/s fitness eval (batched vs plain kernel, 1e-6 rel); p50/p99 eval latency",
+        "value": round(gpops, 3),
+        "unit": "GPop/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded, generated on device)",
+        "config": {"workload": f"{wl.name}: {wl.notes}", "kernel": args.kernel,
+                   "programs": P, "cases": wl.n_cases, "raw_words": wl.raw_words,
+                   "exec_words": wl.exec_words, "num_registers": wl.isa.num_registers,
+                   "op_set": "EXT8", "parallelism": "single GPU"},
+        "gpops_exec": round(exec_gpops, 3),
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(achieved_tflops, 4),
+            "peak": FP32_VALU_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved_tflops / FP32_VALU_PEAK_TFLOPS, 5),
+            "traffic": None,
+            "note": "1 GP-op = 1 algorithmic FP32 op; kernel time from HIP events on the launch "
+                    f"stream; {n_cu} CUs",
+            "frac_of_lane_ops_peak": round(ops / (ms / 1e3) / (n_cu * 64 * 2.4e9), 5),
+        },
+    }
+    if not args.no_cpu_baseline:
+        g = fitness.cpu().numpy()
+        threads = os.cpu_count() or 1
+        try:
+            import psutil
+            threads = len(psutil.Process().cpu_affinity()) or threads
+        except Exception:
+            pass
+        threads = min(threads, 16)
+        cb1 = cpu_baseline(wl, g, args.cpu_seconds / 2, 1)
+        log(f"[bench] cpu 1 thread: {cb1}")
+        cbn = cpu_baseline(wl, g, args.cpu_seconds / 2, threads)
+        log(f"[bench] cpu {threads} threads: {cbn}")
+        line["cpu_baseline"] = {
+            "value": round(cbn["gpops"], 4), "unit": "GPop/s", "cores": threads, "kind": "port",
+            "sample": f"first {cbn['programs']} programs of the same population "
+                      f"({cbn['seconds']:.1f}s, {threads} threads); 1 thread: "
+                      f"{cb1['gpops']:.4f} GPop/s on {cb1['programs']} programs",
+            "single_thread_value": round(cb1["gpops"], 4),
+        }
+        line["parity_sample"] = {
+            "programs": cbn["programs"], "max_rel_err": cbn["max_rel_err"],
+            "within_1e-6": cbn["within_1e-6"], "over_1e-6": cbn["over_1e-6"],
+            "over_chaos_attributed": cbn["over_chaos_attributed"],
+        }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

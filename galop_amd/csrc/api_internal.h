@@ -1,0 +1,145 @@
+// api_internal.h -- host-side objects behind the opaque C-ABI handles.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <functional>
+#include <mutex>
+#include <string>
+
+#include "../../include/galop_hip.h"
+#include "galop_internal.h"
+
+namespace galop {
+
+// Device buffer that only grows (CudaEvaluator::ensure_packed_capacity keeps a
+// high-water mark with 25% headroom, src/EvaluatorCuda.cu:69-81; same here).
+template <class T>
+struct DeviceBuf {
+    T* ptr = nullptr;
+    size_t cap = 0;
+    hipError_t alloc(size_t n) {
+        release();
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc((void**)&ptr, n * sizeof(T));
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        cap = n;
+        return hipSuccess;
+    }
+    hipError_t reserve(size_t n) {
+        if (n <= cap && ptr) return hipSuccess;
+        return alloc(n + n / 4);
+    }
+    void release() {
+        if (ptr) hipFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+    size_t bytes() const { return cap * sizeof(T); }
+};
+
+template <class T>
+struct PinnedBuf {
+    T* ptr = nullptr;
+    size_t cap = 0;
+    hipError_t reserve(size_t n) {
+        if (n <= cap && ptr) return hipSuccess;
+        release();
+        n = n + n / 4;
+        if (n == 0) n = 1;
+        hipError_t e = hipHostMalloc((void**)&ptr, n * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        cap = n;
+        return hipSuccess;
+    }
+    void release() {
+        if (ptr) hipHostFree(ptr);
+        ptr = nullptr;
+        cap = 0;
+    }
+};
+
+struct BatchBufs {
+    DeviceBuf<int32_t> small;  // bucket_count[64] | bucket_cursor[64] | queue_head[1]
+    DeviceBuf<int32_t> order;
+    int64_t max_programs = 0;
+    BatchScratch view() const {
+        BatchScratch s;
+        s.bucket_count = small.ptr;
+        s.bucket_cursor = small.ptr + kBuckets;
+        s.queue_head = small.ptr + 2 * kBuckets;
+        s.order = order.ptr;
+        s.max_programs = max_programs;
+        return s;
+    }
+    void release() {
+        small.release();
+        order.release();
+        max_programs = 0;
+    }
+};
+
+// Staging of the synchronous entry points and scratch of the device ones.
+struct Staging {
+    DeviceBuf<uint32_t> d_words;
+    DeviceBuf<int64_t> d_offsets;
+    DeviceBuf<float> d_fitness;
+    DeviceBuf<double> d_sse;
+    PinnedBuf<uint32_t> h_words;
+    PinnedBuf<int64_t> h_offsets;
+    PinnedBuf<float> h_fitness;
+    PinnedBuf<double> h_sse;
+    DeviceBuf<int64_t> d_counts;
+    DeviceBuf<int64_t> d_scan_tmp;
+    DeviceBuf<int64_t> d_eff_offsets;
+    DeviceBuf<uint32_t> d_eff_words;
+    BatchBufs batch;
+    void release() {
+        d_words.release();
+        d_offsets.release();
+        d_fitness.release();
+        d_sse.release();
+        h_words.release();
+        h_offsets.release();
+        h_fitness.release();
+        h_sse.release();
+        d_counts.release();
+        d_scan_tmp.release();
+        d_eff_offsets.release();
+        d_eff_words.release();
+        batch.release();
+    }
+};
+
+int set_error(galop_ctx* ctx, int code, const std::string& msg);
+int hip_fail(galop_ctx* ctx, hipError_t e, const char* what);
+void parallel_chunks(int64_t n, int64_t min_per_thread,
+                     const std::function<void(int64_t, int64_t)>& fn);
+int ensure_batch_scratch(galop_ctx* ctx, BatchBufs& b, int64_t max_programs);
+
+}  // namespace galop
+
+struct galop_ctx {
+    int device = 0;
+    int n_cu = 0;
+    int clock_khz = 0;
+    hipStream_t stream = nullptr;
+    galop::PlainTuning tuning;
+    galop::Staging staging;
+    std::mutex err_mu;
+    std::string last_error;
+};
+
+struct galop_dataset {
+    galop_ctx* ctx = nullptr;
+    galop_isa isa{};
+    galop::DevData dev{};
+    galop::DeviceBuf<float> inputs;
+    galop::DeviceBuf<float> targets;
+    galop::DeviceBuf<float> consts;
+};

@@ -1,0 +1,207 @@
+// batch.hip -- batched persistent evaluation plans (C ABI).
+//
+// A plan owns the bucketing / work-queue scratch for one dataset and a
+// hipGraph of one generation: reset -> length histogram -> bucket starts ->
+// scatter into longest-first order -> persistent evaluation.  The graph is
+// captured on first use and replayed while the buffers it was captured with
+// stay the same; the reset is a kernel, so replays start from zeroed counters.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "api_internal.h"
+
+using namespace galop;
+
+struct galop_batch {
+    galop_ctx* ctx = nullptr;
+    const galop_dataset* ds = nullptr;
+    BatchBufs bufs;
+    BatchTuning tuning;
+    // graph cache
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    const void* key_words = nullptr;
+    const void* key_offsets = nullptr;
+    const void* key_fitness = nullptr;
+    const void* key_sse = nullptr;
+    int64_t key_n = -1;
+    hipStream_t key_stream = nullptr;
+    void drop_graph() {
+        if (exec) hipGraphExecDestroy(exec);
+        if (graph) hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+        key_n = -1;
+    }
+};
+
+namespace galop {
+int ensure_batch_scratch(galop_ctx* ctx, BatchBufs& b, int64_t max_programs) {
+    if (max_programs > (int64_t)INT32_MAX - (int64_t(1) << 22))
+        return set_error(ctx, GALOP_ERR_INVALID, "batched path: at most 2^31-2^22 programs per call");
+    if (b.small.ptr && b.max_programs >= max_programs) return GALOP_OK;
+    hipError_t e = b.small.alloc(2 * kBuckets + 64);
+    if (e == hipSuccess) e = b.order.alloc((size_t)std::max<int64_t>(max_programs, 1));
+    if (e != hipSuccess) {
+        b.release();
+        return hip_fail(ctx, e, "batched path: scratch allocation");
+    }
+    b.max_programs = max_programs;
+    return GALOP_OK;
+}
+}  // namespace galop
+
+extern "C" {
+
+int galop_batch_create(galop_ctx* ctx, const galop_dataset* ds, int64_t max_programs,
+                       galop_batch** out) {
+    if (!ctx || !ds || !out) return set_error(ctx, GALOP_ERR_INVALID, "galop_batch_create: NULL argument");
+    *out = nullptr;
+    if (ds->ctx != ctx)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_batch_create: dataset belongs to another context");
+    if (max_programs <= 0) return set_error(ctx, GALOP_ERR_INVALID, "galop_batch_create: max_programs <= 0");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    auto* b = new galop_batch();
+    b->ctx = ctx;
+    b->ds = ds;
+    const int rc = ensure_batch_scratch(ctx, b->bufs, max_programs);
+    if (rc != GALOP_OK) {
+        delete b;
+        return rc;
+    }
+    b->tuning = resolve_batch_tuning(ds->dev, ds->isa.num_registers, BatchTuning{}, ctx->n_cu);
+    *out = b;
+    return GALOP_OK;
+}
+
+void galop_batch_destroy(galop_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->ctx->device);
+    b->drop_graph();
+    b->bufs.release();
+    delete b;
+}
+
+int galop_batch_eval_device(galop_batch* b, const uint32_t* d_words, const int64_t* d_offsets,
+                            int64_t n_programs, float* d_fitness, double* d_sse, int32_t use_graph,
+                            void* stream) {
+    if (!b) return set_error(nullptr, GALOP_ERR_INVALID, "galop_batch_eval_device: NULL plan");
+    galop_ctx* ctx = b->ctx;
+    if (n_programs < 0 || n_programs > b->bufs.max_programs)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_batch_eval_device: n_programs outside [0, max_programs]");
+    if (n_programs == 0) return GALOP_OK;
+    if (!d_words || !d_offsets || !d_fitness)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_batch_eval_device: NULL device pointer");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    const DevData& dd = b->ds->dev;
+    const int nr = b->ds->isa.num_registers;
+    if (!use_graph) {
+        e = launch_batched(dd, nr, b->tuning, ctx->n_cu, b->bufs.view(), d_words, d_offsets,
+                           n_programs, d_fitness, d_sse, s);
+        return e == hipSuccess ? GALOP_OK : hip_fail(ctx, e, "launch_batched");
+    }
+    const bool hit = b->exec && b->key_words == d_words && b->key_offsets == d_offsets &&
+                     b->key_fitness == d_fitness && b->key_sse == d_sse &&
+                     b->key_n == n_programs && b->key_stream == s;
+    if (!hit) {
+        b->drop_graph();
+        e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamBeginCapture");
+        const hipError_t le = launch_batched(dd, nr, b->tuning, ctx->n_cu, b->bufs.view(), d_words,
+                                             d_offsets, n_programs, d_fitness, d_sse, s);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(s, &g);
+        if (le != hipSuccess) {
+            if (g) hipGraphDestroy(g);
+            return hip_fail(ctx, le, "launch_batched (capture)");
+        }
+        if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
+        b->graph = g;
+        e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            b->drop_graph();
+            return hip_fail(ctx, e, "hipGraphInstantiate");
+        }
+        b->key_words = d_words;
+        b->key_offsets = d_offsets;
+        b->key_fitness = d_fitness;
+        b->key_sse = d_sse;
+        b->key_n = n_programs;
+        b->key_stream = s;
+    }
+    e = hipGraphLaunch(b->exec, s);
+    return e == hipSuccess ? GALOP_OK : hip_fail(ctx, e, "hipGraphLaunch");
+}
+
+int galop_batch_autotune(galop_batch* b, const uint32_t* d_words, const int64_t* d_offsets,
+                         int64_t n_programs, float* d_fitness, int32_t* cases_per_lane,
+                         int32_t* waves_per_block, int32_t* workers_per_cu) {
+    if (!b) return set_error(nullptr, GALOP_ERR_INVALID, "galop_batch_autotune: NULL plan");
+    galop_ctx* ctx = b->ctx;
+    if (n_programs <= 0 || n_programs > b->bufs.max_programs || !d_words || !d_offsets || !d_fitness)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_batch_autotune: bad population");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    hipStream_t s = ctx->stream;
+    const DevData& dd = b->ds->dev;
+    const int nr = b->ds->isa.num_registers;
+    // A prefix sample keeps the search short; it is bucketed like a full call.
+    const int64_t sample = std::min<int64_t>(n_programs, int64_t(1) << 17);
+    std::vector<int> ks = nr == 8 ? std::vector<int>{1, 2, 4, 8} : std::vector<int>{1, 2, 4};
+    const int64_t per_lane = (dd.n_cases + kWave - 1) / kWave;
+    // more cases per lane than the dataset has only adds padded work
+    ks.erase(std::remove_if(ks.begin(), ks.end(), [&](int k) { return k > 1 && k / 2 >= per_lane; }),
+             ks.end());
+    const int wpbs[] = {1, 2, 4, 8};
+    const int workers[] = {8, 16, 24, 32};
+    hipEvent_t a, z;
+    if ((e = hipEventCreate(&a)) != hipSuccess) return hip_fail(ctx, e, "hipEventCreate");
+    if ((e = hipEventCreate(&z)) != hipSuccess) {
+        hipEventDestroy(a);
+        return hip_fail(ctx, e, "hipEventCreate");
+    }
+    float best = 1e30f;
+    BatchTuning best_t = b->tuning;
+    for (int k : ks)
+        for (int w : wpbs)
+            for (int wk : workers) {
+                BatchTuning t;
+                t.cases_per_lane = k;
+                t.waves_per_block = w;
+                t.workers_per_cu = wk;
+                t.chunk = 1;
+                float ms = 0.0f;
+                for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+                    e = hipEventRecord(a, s);
+                    if (e == hipSuccess)
+                        e = launch_batched(dd, nr, t, ctx->n_cu, b->bufs.view(), d_words, d_offsets,
+                                           sample, d_fitness, nullptr, s);
+                    if (e == hipSuccess) e = hipEventRecord(z, s);
+                    if (e == hipSuccess) e = hipEventSynchronize(z);
+                    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, z);
+                }
+                if (e != hipSuccess) break;
+                if (ms < best) {
+                    best = ms;
+                    best_t = t;
+                }
+            }
+    hipEventDestroy(a);
+    hipEventDestroy(z);
+    if (e != hipSuccess) return hip_fail(ctx, e, "galop_batch_autotune");
+    b->tuning = best_t;
+    b->drop_graph();
+    if (cases_per_lane) *cases_per_lane = best_t.cases_per_lane;
+    if (waves_per_block) *waves_per_block = best_t.waves_per_block;
+    if (workers_per_cu) *workers_per_cu = best_t.workers_per_cu;
+    return GALOP_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// galop_internal.h -- launchers shared by the C-ABI layer and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lgp_device.h"
+
+namespace galop {
+
+// Largest cases-per-lane any kernel uses; device datasets are padded to a
+// multiple of kWave * kMaxK cases so every tile load is in bounds.
+constexpr int kMaxK = 8;
+constexpr int64_t kCasePad = (int64_t)kWave * kMaxK;
+// Programs per launch window of the plain kernel (keeps grid*block < 2^32).
+constexpr int64_t kLaunchWindow = int64_t(1) << 24;
+
+struct PlainTuning {
+    int cases_per_lane = 0;   // 1,2,4,8; 0 = automatic
+    int waves_per_block = 0;  // 1..16; 0 = automatic
+};
+
+// Resolve automatic tuning for a dataset.
+PlainTuning resolve_plain_tuning(const DevData& ds, int num_registers, PlainTuning t);
+
+// Plain kernel: one program per wave over programs [0, n) of a CSR buffer.
+hipError_t launch_plain(const DevData& ds, int num_registers, PlainTuning t,
+                        const uint32_t* words, const int64_t* offsets, int64_t n,
+                        float* fitness, double* sse, hipStream_t stream);
+
+// Per-case outputs (same interpreter, writes r[OUTPUT_REG] per case).
+hipError_t launch_run_cases(const DevData& ds, int num_registers, const uint32_t* words,
+                            const int64_t* offsets, int64_t n, float* out,
+                            hipStream_t stream);
+
+// Fill consts[16..64) with the unary-on-constant table from consts[0..16).
+hipError_t launch_const_table(float* consts, hipStream_t stream);
+
+// Device intron removal: per-program effective counts, scan, scatter.
+hipError_t launch_effective_counts(int num_registers, int op_set, int input_protect,
+                                   const uint32_t* words, const int64_t* offsets,
+                                   int64_t n, int64_t* counts, hipStream_t stream);
+hipError_t launch_effective_scatter(int num_registers, int op_set, int input_protect,
+                                    const uint32_t* words, const int64_t* offsets,
+                                    int64_t n, const int64_t* eff_offsets,
+                                    uint32_t* eff_words, hipStream_t stream);
+// Exclusive scan of n int64 counts into out[0..n] (out[n] = total); tmp must
+// hold scan_temp_elems(n) int64.
+int64_t scan_temp_elems(int64_t n);
+hipError_t launch_exclusive_scan(const int64_t* counts, int64_t n, int64_t* out,
+                                 int64_t* tmp, hipStream_t stream);
+
+// Batched persistent path.
+struct BatchScratch {
+    int32_t* bucket_count;    // [kBuckets]
+    int32_t* bucket_cursor;   // [kBuckets]
+    int32_t* order;           // [max_programs]: program ids, longest bucket first
+    int32_t* queue_head;      // [1]: next order index to hand out
+    int64_t max_programs;
+};
+constexpr int kBuckets = 64;
+
+struct BatchTuning {
+    int cases_per_lane = 0;
+    int waves_per_block = 0;
+    int workers_per_cu = 0;   // persistent waves per CU (grid size)
+    int chunk = 0;            // programs handed out per queue grab
+};
+
+BatchTuning resolve_batch_tuning(const DevData& ds, int num_registers, BatchTuning t,
+                                 int n_cu);
+hipError_t launch_batched(const DevData& ds, int num_registers, BatchTuning t, int n_cu,
+                          const BatchScratch& s, const uint32_t* words,
+                          const int64_t* offsets, int64_t n, float* fitness,
+                          double* sse, hipStream_t stream);
+
+}  // namespace galop

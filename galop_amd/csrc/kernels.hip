@@ -1,0 +1,478 @@
+// kernels.hip -- gfx950 kernels of the Galop fitness evaluator.
+//
+//   plain_kernel       one program per wave64 over a CSR population; the
+//                      replacement of evaluate_kernel_packed
+//                      (src/EvaluatorCuda.cu:241-265) and, through the dense
+//                      packing of the C ABI, of evaluate_kernel (:163-190).
+//   run_cases_kernel   same interpreter, writes per-case outputs.
+//   batched_kernel     persistent waves pulling programs from a work queue
+//                      ordered longest-bucket-first; datasets that fit in one
+//                      tile stay in VGPRs across programs.
+//   effective_*        device intron removal (src/Introns.h:28-57).
+//   scan_*             int64 exclusive scan for CSR offsets.
+#include "galop_internal.h"
+
+namespace galop {
+
+// ---------------------------------------------------------------------------
+// plain kernel
+// ---------------------------------------------------------------------------
+template <int NR, int K, int EXT>
+__global__ void __launch_bounds__(256)
+plain_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ offsets,
+             int64_t first, int64_t n, DevData ds, float* __restrict__ fitness,
+             double* __restrict__ sse_out) {
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    const int wave_in_block = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t p = first + (int64_t)blockIdx.x * (int64_t)(blockDim.x >> 6) + wave_in_block;
+    if (p >= n) return;  // wave-uniform
+    const float cvec = ds.consts[lane];
+    const int64_t lo = offsets[p];
+    const int64_t hi = offsets[p + 1];
+    const double sse = eval_program_wave<NR, K, EXT>(words + lo, hi - lo, ds, lane, cvec, nullptr);
+    if (lane == 0) {
+        fitness[p] = sse_to_fitness(sse, ds.n_cases);
+        if (sse_out) sse_out[p] = sse;
+    }
+}
+
+template <int NR, int EXT>
+__global__ void __launch_bounds__(256)
+run_cases_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ offsets,
+                 int64_t first, int64_t n, DevData ds, float* __restrict__ out) {
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    const int wave_in_block = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t p = first + (int64_t)blockIdx.x * (int64_t)(blockDim.x >> 6) + wave_in_block;
+    if (p >= n) return;
+    const float cvec = ds.consts[lane];
+    const int64_t lo = offsets[p];
+    const int64_t hi = offsets[p + 1];
+    (void)eval_program_wave<NR, 1, EXT>(words + lo, hi - lo, ds, lane, cvec, out + p * ds.n_cases);
+}
+
+__global__ void const_table_kernel(float* consts) {
+    const int t = (int)threadIdx.x;  // 48 threads
+    const int u = t / 16, idx = t % 16;
+    const float b = consts[idx];
+    float v;
+    switch (u) {
+    case 0: v = op_sin(b); break;
+    case 1: v = op_cos(b); break;
+    case 2: v = op_exp(b); break;
+    default: v = op_plog(b); break;
+    }
+    consts[kUnaryTableBase + t] = v;
+}
+
+PlainTuning resolve_plain_tuning(const DevData& ds, int num_registers, PlainTuning t) {
+    if (t.cases_per_lane <= 0) {
+        // enough cases per lane to amortise scalar decode, without padding
+        // small datasets: K = ceil(N/64) capped at 4 (8 registers) / 2 (16)
+        const int64_t per_lane = (ds.n_cases + kWave - 1) / kWave;
+        int kmax = num_registers == 8 ? 4 : 2;
+        int k = 1;
+        while (k < kmax && k < per_lane) k *= 2;
+        t.cases_per_lane = k;
+    }
+    if (t.waves_per_block <= 0 || t.waves_per_block > 4) t.waves_per_block = 4;
+    return t;
+}
+
+template <int NR, int K, int EXT>
+static hipError_t launch_plain_t(const DevData& ds, int wpb, const uint32_t* words,
+                                 const int64_t* offsets, int64_t n, float* fitness,
+                                 double* sse, hipStream_t stream) {
+    for (int64_t first = 0; first < n; first += kLaunchWindow) {
+        const int64_t cnt = (n - first) < kLaunchWindow ? (n - first) : kLaunchWindow;
+        const unsigned grid = (unsigned)((cnt + wpb - 1) / wpb);
+        hipLaunchKernelGGL((plain_kernel<NR, K, EXT>), dim3(grid), dim3(wpb * kWave), 0, stream,
+                           words, offsets, first, first + cnt, ds, fitness, sse);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_plain(const DevData& ds, int num_registers, PlainTuning t,
+                        const uint32_t* words, const int64_t* offsets, int64_t n,
+                        float* fitness, double* sse, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    t = resolve_plain_tuning(ds, num_registers, t);
+    const int wpb = t.waves_per_block;
+#define GALOP_CASE(NRV, KV)                                                            \
+    if (num_registers == NRV && t.cases_per_lane == KV)                                \
+        return ds.op_set ? launch_plain_t<NRV, KV, 1>(ds, wpb, words, offsets, n, fitness, sse, stream) \
+                         : launch_plain_t<NRV, KV, 0>(ds, wpb, words, offsets, n, fitness, sse, stream);
+    GALOP_CASE(8, 1) GALOP_CASE(8, 2) GALOP_CASE(8, 4) GALOP_CASE(8, 8)
+    GALOP_CASE(16, 1) GALOP_CASE(16, 2) GALOP_CASE(16, 4) GALOP_CASE(16, 8)
+#undef GALOP_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_run_cases(const DevData& ds, int num_registers, const uint32_t* words,
+                            const int64_t* offsets, int64_t n, float* out,
+                            hipStream_t stream) {
+    const int wpb = 4;
+    for (int64_t first = 0; first < n; first += kLaunchWindow) {
+        const int64_t cnt = (n - first) < kLaunchWindow ? (n - first) : kLaunchWindow;
+        const unsigned grid = (unsigned)((cnt + wpb - 1) / wpb);
+        const dim3 g(grid), b(wpb * kWave);
+        if (num_registers == 8 && ds.op_set)
+            hipLaunchKernelGGL((run_cases_kernel<8, 1>), g, b, 0, stream, words, offsets, first, first + cnt, ds, out);
+        else if (num_registers == 8)
+            hipLaunchKernelGGL((run_cases_kernel<8, 0>), g, b, 0, stream, words, offsets, first, first + cnt, ds, out);
+        else if (ds.op_set)
+            hipLaunchKernelGGL((run_cases_kernel<16, 1>), g, b, 0, stream, words, offsets, first, first + cnt, ds, out);
+        else
+            hipLaunchKernelGGL((run_cases_kernel<16, 0>), g, b, 0, stream, words, offsets, first, first + cnt, ds, out);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_const_table(float* consts, hipStream_t stream) {
+    hipLaunchKernelGGL(const_table_kernel, dim3(1), dim3(48), 0, stream, consts);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// device intron removal (src/Introns.h:28-57): one thread per program walks
+// its program backwards with a live-register bitmask.
+// ---------------------------------------------------------------------------
+struct DecodeCfg {
+    uint32_t mask;   // NR - 1
+    int out_reg;
+    int iprot;
+    int ext;
+};
+
+__device__ __forceinline__ bool effective_step(uint32_t w, const DecodeCfg& c, uint32_t& live) {
+    const int d = c.iprot ? c.out_reg + (int)((w >> 8) & (c.mask >> 1)) : (int)((w >> 8) & c.mask);
+    if (!((live >> d) & 1u)) return false;
+    live &= ~(1u << d);
+    const int op = (int)(w & 7u);
+    const bool unary = (op == 4 || op == 5) || (c.ext && (op == 6 || op == 7));
+    if (!unary) live |= 1u << ((w >> 16) & c.mask);
+    if (!(w >> 31)) live |= 1u << ((w >> 24) & c.mask);
+    return true;
+}
+
+__global__ void effective_count_kernel(DecodeCfg cfg, const uint32_t* __restrict__ words,
+                                       const int64_t* __restrict__ offsets, int64_t n,
+                                       int64_t* __restrict__ counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t lo = offsets[i], hi = offsets[i + 1];
+    uint32_t live = 1u << cfg.out_reg;
+    int64_t cnt = 0;
+    for (int64_t j = hi - 1; j >= lo; --j) cnt += effective_step(words[j], cfg, live) ? 1 : 0;
+    counts[i] = cnt;
+}
+
+__global__ void effective_scatter_kernel(DecodeCfg cfg, const uint32_t* __restrict__ words,
+                                         const int64_t* __restrict__ offsets, int64_t n,
+                                         const int64_t* __restrict__ eff_offsets,
+                                         uint32_t* __restrict__ eff_words) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t lo = offsets[i], hi = offsets[i + 1];
+    int64_t pos = eff_offsets[i + 1];  // fill from the back
+    uint32_t live = 1u << cfg.out_reg;
+    for (int64_t j = hi - 1; j >= lo; --j) {
+        const uint32_t w = words[j];
+        if (effective_step(w, cfg, live)) eff_words[--pos] = w;
+    }
+}
+
+static DecodeCfg make_cfg(int nr, int op_set, int iprot) {
+    DecodeCfg c;
+    c.mask = (uint32_t)nr - 1u;
+    c.iprot = iprot;
+    c.out_reg = iprot ? nr / 2 : 0;
+    c.ext = op_set;
+    return c;
+}
+
+hipError_t launch_effective_counts(int num_registers, int op_set, int input_protect,
+                                   const uint32_t* words, const int64_t* offsets, int64_t n,
+                                   int64_t* counts, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(effective_count_kernel, dim3(grid), dim3(256), 0, stream,
+                       make_cfg(num_registers, op_set, input_protect), words, offsets, n, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_effective_scatter(int num_registers, int op_set, int input_protect,
+                                    const uint32_t* words, const int64_t* offsets, int64_t n,
+                                    const int64_t* eff_offsets, uint32_t* eff_words,
+                                    hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    const unsigned grid = (unsigned)((n + 255) / 256);
+    hipLaunchKernelGGL(effective_scatter_kernel, dim3(grid), dim3(256), 0, stream,
+                       make_cfg(num_registers, op_set, input_protect), words, offsets, n,
+                       eff_offsets, eff_words);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// int64 exclusive scan: block sums -> one-block scan of the sums -> rescan.
+// ---------------------------------------------------------------------------
+constexpr int kScanThreads = 256;
+constexpr int kScanItems = 8;
+constexpr int64_t kScanTile = (int64_t)kScanThreads * kScanItems;
+
+__device__ int64_t block_exclusive_scan(int64_t v, int64_t* sh, int64_t& total) {
+    // Hillis-Steele over kScanThreads entries in LDS
+    const int t = (int)threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        const int64_t x = (t >= off) ? sh[t - off] : 0;
+        __syncthreads();
+        sh[t] += x;
+        __syncthreads();
+    }
+    total = sh[kScanThreads - 1];
+    const int64_t incl = sh[t];
+    __syncthreads();
+    return incl - v;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+scan_block_sums(const int64_t* __restrict__ in, int64_t n, int64_t* __restrict__ sums) {
+    __shared__ int64_t sh[kScanThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int64_t s = 0;
+    for (int k = 0; k < kScanItems; ++k) s += (base + k < n) ? in[base + k] : 0;
+    int64_t total;
+    (void)block_exclusive_scan(s, sh, total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+scan_sums_single(int64_t* __restrict__ sums, int64_t nb) {
+    __shared__ int64_t sh[kScanThreads];
+    int64_t carry = 0;
+    for (int64_t base = 0; base < nb; base += kScanThreads) {
+        const int64_t i = base + threadIdx.x;
+        const int64_t v = (i < nb) ? sums[i] : 0;
+        int64_t total;
+        const int64_t ex = block_exclusive_scan(v, sh, total);
+        if (i < nb) sums[i] = carry + ex;
+        carry += total;
+    }
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+scan_apply(const int64_t* __restrict__ in, int64_t n, const int64_t* __restrict__ sums,
+           int64_t* __restrict__ out) {
+    __shared__ int64_t sh[kScanThreads];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)threadIdx.x * kScanItems;
+    int64_t v[kScanItems];
+    int64_t s = 0;
+    for (int k = 0; k < kScanItems; ++k) {
+        v[k] = (base + k < n) ? in[base + k] : 0;
+        s += v[k];
+    }
+    int64_t total;
+    int64_t run = sums[blockIdx.x] + block_exclusive_scan(s, sh, total);
+    for (int k = 0; k < kScanItems; ++k) {
+        if (base + k < n) out[base + k] = run;
+        run += v[k];
+    }
+    if (base <= n - 1 && n - 1 < base + kScanItems) out[n] = run;  // total after the last item
+}
+
+__global__ void scan_zero_kernel(int64_t* out) { out[0] = 0; }
+
+int64_t scan_temp_elems(int64_t n) { return (n + kScanTile - 1) / kScanTile + 1; }
+
+hipError_t launch_exclusive_scan(const int64_t* counts, int64_t n, int64_t* out, int64_t* tmp,
+                                 hipStream_t stream) {
+    if (n <= 0) {  // out[0] = 0 (a kernel, not a memset, so the call can be captured)
+        hipLaunchKernelGGL(scan_zero_kernel, dim3(1), dim3(1), 0, stream, out);
+        return hipGetLastError();
+    }
+    const int64_t nb = (n + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(scan_block_sums, dim3((unsigned)nb), dim3(kScanThreads), 0, stream,
+                       counts, n, tmp);
+    hipLaunchKernelGGL(scan_sums_single, dim3(1), dim3(kScanThreads), 0, stream, tmp, nb);
+    hipLaunchKernelGGL(scan_apply, dim3((unsigned)nb), dim3(kScanThreads), 0, stream, counts, n,
+                       tmp, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// batched persistent path
+// ---------------------------------------------------------------------------
+// Length bucket: 0..15 exact for lengths 0..15, then 4 buckets per octave
+// up to bucket 63 (lengths >= ~2^15 share the last bucket).
+__device__ __forceinline__ int length_bucket(int64_t len) {
+    if (len < 16) return (int)len;
+    const int lg = 63 - __clzll((unsigned long long)len);       // >= 4
+    const int frac = (int)((len >> (lg - 2)) & 3);               // next 2 bits
+    const int b = 16 + (lg - 4) * 4 + frac;
+    return b > kBuckets - 1 ? kBuckets - 1 : b;
+}
+
+__global__ void batch_reset_kernel(int32_t* bucket_count, int32_t* bucket_cursor,
+                                   int32_t* queue_head) {
+    const int t = (int)threadIdx.x;
+    if (t < kBuckets) {
+        bucket_count[t] = 0;
+        bucket_cursor[t] = 0;
+    }
+    if (t == 0) *queue_head = 0;
+}
+
+__global__ void batch_histogram_kernel(const int64_t* __restrict__ offsets, int64_t n,
+                                       int32_t* __restrict__ bucket_count) {
+    __shared__ int32_t local[kBuckets];
+    if (threadIdx.x < kBuckets) local[threadIdx.x] = 0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&local[length_bucket(offsets[i + 1] - offsets[i])], 1);
+    __syncthreads();
+    if (threadIdx.x < kBuckets && local[threadIdx.x])
+        atomicAdd(&bucket_count[threadIdx.x], local[threadIdx.x]);
+}
+
+// Start of each bucket in the order array, longest bucket first.
+__global__ void batch_bucket_start_kernel(const int32_t* __restrict__ bucket_count,
+                                          int32_t* __restrict__ bucket_cursor) {
+    if (threadIdx.x == 0) {
+        int32_t run = 0;
+        for (int b = kBuckets - 1; b >= 0; --b) {
+            bucket_cursor[b] = run;
+            run += bucket_count[b];
+        }
+    }
+}
+
+__global__ void batch_scatter_kernel(const int64_t* __restrict__ offsets, int64_t n,
+                                     int32_t* __restrict__ bucket_cursor,
+                                     int32_t* __restrict__ order) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int b = length_bucket(offsets[i + 1] - offsets[i]);
+    order[atomicAdd(&bucket_cursor[b], 1)] = (int32_t)i;
+}
+
+// Persistent evaluation: each wave repeatedly takes `chunk` entries of the
+// order array from the queue head and evaluates them.  Every wave leaves the
+// loop once the head passes n, so the grid always drains.
+template <int NR, int K, int EXT, bool RESIDENT>
+__global__ void __launch_bounds__(256)
+batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ offsets,
+               const int32_t* __restrict__ order, int32_t* __restrict__ queue_head,
+               int64_t n, int chunk, DevData ds, float* __restrict__ fitness,
+               double* __restrict__ sse_out) {
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    const float cvec = ds.consts[lane];
+    const DestDecode dd = dest_decode(NR, ds.input_protect);
+    const int out_reg = ds.out_reg;
+
+    // One-tile datasets stay in VGPRs for the life of the wave.
+    RegVec<NR> init[K];
+    float y[K];
+    if (RESIDENT) {
+        load_cases<NR, K>(ds, 0, lane, init);
+#pragma unroll
+        for (int k = 0; k < K; ++k) y[k] = ds.targets[k * kWave + lane];
+    }
+    for (;;) {
+        int base = 0;
+        if (lane == 0) base = atomicAdd(queue_head, chunk);
+        base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, kWave));
+        if ((int64_t)base >= n) break;
+        const int64_t end = ((int64_t)base + chunk) < n ? ((int64_t)base + chunk) : n;
+        for (int64_t q = base; q < end; ++q) {
+            const int64_t p = order[q];
+            const int64_t lo = offsets[p], hi = offsets[p + 1];
+            double sse;
+            if (RESIDENT) {
+                RegVec<NR> r[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = init[k];
+                run_program<NR, K, EXT>(words + lo, hi - lo, lane, r, cvec, dd);
+                LaneAcc acc;
+#pragma unroll
+                for (int k = 0; k < K; ++k) {
+                    if ((int64_t)(k * kWave + lane) < ds.n_cases) acc.add(r[k][out_reg], y[k]);
+                }
+                sse = wave_sum_f64(acc.total());
+            } else {
+                sse = eval_program_wave<NR, K, EXT>(words + lo, hi - lo, ds, lane, cvec, nullptr);
+            }
+            if (lane == 0) {
+                fitness[p] = sse_to_fitness(sse, ds.n_cases);
+                if (sse_out) sse_out[p] = sse;
+            }
+        }
+    }
+}
+
+BatchTuning resolve_batch_tuning(const DevData& ds, int num_registers, BatchTuning t, int n_cu) {
+    (void)n_cu;
+    if (t.cases_per_lane <= 0) {
+        PlainTuning pt;
+        pt = resolve_plain_tuning(ds, num_registers, pt);
+        t.cases_per_lane = pt.cases_per_lane;
+    }
+    if (t.waves_per_block <= 0) t.waves_per_block = 4;
+    if (t.workers_per_cu <= 0) t.workers_per_cu = 32;
+    if (t.chunk <= 0) t.chunk = 1;
+    return t;
+}
+
+template <int NR, int K, int EXT>
+static hipError_t launch_batched_t(const DevData& ds, const BatchTuning& t, int n_cu,
+                                   const BatchScratch& s, const uint32_t* words,
+                                   const int64_t* offsets, int64_t n, float* fitness,
+                                   double* sse, hipStream_t stream) {
+    const int wpb = t.waves_per_block;
+    const int64_t waves = (int64_t)n_cu * t.workers_per_cu;
+    const unsigned grid = (unsigned)((waves + wpb - 1) / wpb);
+    const bool resident = ds.n_cases <= (int64_t)kWave * K;
+    if (resident)
+        hipLaunchKernelGGL((batched_kernel<NR, K, EXT, true>), dim3(grid), dim3(wpb * kWave), 0,
+                           stream, words, offsets, s.order, s.queue_head, n, t.chunk, ds,
+                           fitness, sse);
+    else
+        hipLaunchKernelGGL((batched_kernel<NR, K, EXT, false>), dim3(grid), dim3(wpb * kWave), 0,
+                           stream, words, offsets, s.order, s.queue_head, n, t.chunk, ds,
+                           fitness, sse);
+    return hipGetLastError();
+}
+
+hipError_t launch_batched(const DevData& ds, int num_registers, BatchTuning t, int n_cu,
+                          const BatchScratch& s, const uint32_t* words, const int64_t* offsets,
+                          int64_t n, float* fitness, double* sse, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    if (n > s.max_programs || n > (int64_t)INT32_MAX - (int64_t(1) << 22)) return hipErrorInvalidValue;
+    t = resolve_batch_tuning(ds, num_registers, t, n_cu);
+    hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kBuckets), 0, stream, s.bucket_count,
+                       s.bucket_cursor, s.queue_head);
+    const unsigned hgrid = (unsigned)(((n + 255) / 256) < 1024 ? ((n + 255) / 256) : 1024);
+    hipLaunchKernelGGL(batch_histogram_kernel, dim3(hgrid), dim3(256), 0, stream, offsets, n,
+                       s.bucket_count);
+    hipLaunchKernelGGL(batch_bucket_start_kernel, dim3(1), dim3(64), 0, stream, s.bucket_count,
+                       s.bucket_cursor);
+    hipLaunchKernelGGL(batch_scatter_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       stream, offsets, n, s.bucket_cursor, s.order);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+#define GALOP_CASE(NRV, KV)                                                             \
+    if (num_registers == NRV && t.cases_per_lane == KV)                                 \
+        return ds.op_set ? launch_batched_t<NRV, KV, 1>(ds, t, n_cu, s, words, offsets, n, fitness, sse, stream) \
+                         : launch_batched_t<NRV, KV, 0>(ds, t, n_cu, s, words, offsets, n, fitness, sse, stream);
+    GALOP_CASE(8, 1) GALOP_CASE(8, 2) GALOP_CASE(8, 4) GALOP_CASE(8, 8)
+    GALOP_CASE(16, 1) GALOP_CASE(16, 2) GALOP_CASE(16, 4) GALOP_CASE(16, 8)
+#undef GALOP_CASE
+    return hipErrorInvalidValue;
+}
+
+}  // namespace galop

@@ -1,0 +1,259 @@
+// lgp_device.h -- wave64 linear-GP interpreter for gfx950.
+//
+// One program runs on one 64-lane wavefront.  Every lane evaluates K fitness
+// cases; the LGP register file of those K cases lives in VGPRs as K vectors
+// of NR floats.  Instruction words are wave-uniform: a wave loads 64 words of
+// its program with one coalesced load (lane i holds word i) and broadcasts
+// word j to SGPRs with v_readlane, so decode and dispatch run on the scalar
+// unit once per instruction and the vector unit only does register moves and
+// the operation itself.  Register indices are uniform, so the dynamic register
+// file access lowers to M0-relative moves (no scratch, no divergence).
+//
+// Semantics follow the reference exactly:
+//   decode               src/ISA.h:176-206 (INPUT_PROTECT fold :181-187)
+//   apply_op             src/ISA.h:219-231 (protected div, unary sin/cos on src2)
+//   register init        src/Interpreter.cpp:36-52 (inputs r0..k-1, rest 0)
+//   output register      src/LGPConfig.h:294-300
+// and the EXT8 table replaces opcodes 6/7 (LT/GT) by exp(src2) and
+// protected log(src2) = (b != 0 ? log|b| : 0).
+//
+// Fitness accumulation (src/Evaluator.cpp:36-44 is the oracle): per lane, the
+// squared errors of its cases are summed in a fixed order -- lane l takes
+// cases l, l+64, l+128, ... in increasing order whatever K is -- with an exact
+// FP32 residual (TwoSum / FMA) and a Neumaier-compensated FP32 sum; errors of
+// magnitude >= 2^48 go to an FP64 side sum so the FP32 partial cannot
+// overflow.  The 64 lane partials are combined in FP64 in a fixed tree, so a
+// program's fitness is bitwise independent of K, of the launch shape and of
+// whatever else runs on the device.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace galop {
+
+constexpr int kWave = 64;
+
+// Kernel-side view of a resident dataset.
+struct DevData {
+    const float* inputs;   // feature-planar: inputs[i * n_pad + c]
+    const float* targets;  // targets[c], zero padded to n_pad
+    const float* consts;   // 64 floats: [0,16) constant pool, [16,64) unary table
+    int64_t n_cases;
+    int64_t n_pad;         // multiple of 64 * kMaxK
+    int32_t n_inputs;
+    int32_t op_set;        // 0 REF8, 1 EXT8
+    int32_t input_protect;
+    int32_t out_reg;
+};
+
+// Unary-on-constant table layout in DevData::consts: entry 16 + u*16 + idx
+// holds unary op u (0 sin, 1 cos, 2 exp, 3 plog) applied to constant idx.
+// It is filled on the device by the same functions the interpreter calls,
+// so using it is bitwise identical to evaluating the op.
+constexpr int kUnaryTableBase = 16;
+
+// ---- the operations -------------------------------------------------------
+__device__ __forceinline__ float op_div(float a, float b) {
+    return (b != 0.0f) ? a / b : 1.0f;  // src/ISA.h:224 protected division
+}
+__device__ __forceinline__ float op_sin(float b) { return sinf(b); }
+__device__ __forceinline__ float op_cos(float b) { return cosf(b); }
+__device__ __forceinline__ float op_exp(float b) { return expf(b); }
+__device__ __forceinline__ float op_plog(float b) {
+    return (b != 0.0f) ? logf(fabsf(b)) : 0.0f;
+}
+
+template <int NR>
+using RegVec = float __attribute__((ext_vector_type(NR)));
+
+// Broadcast word j of a 64-word chunk held one word per lane.
+__device__ __forceinline__ uint32_t wave_word(uint32_t chunk, int j) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)chunk, j);
+}
+__device__ __forceinline__ float wave_bcast(float v, int j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+// Execute one instruction word on K register vectors.  Everything derived
+// from `w` is wave-uniform.
+// Destination decode without a branch: d = (field & dmask) + dbase, with
+// (dmask, dbase) = (NR-1, 0) normally and (NR/2-1, NR/2) under INPUT_PROTECT.
+struct DestDecode {
+    uint32_t dmask;
+    int dbase;
+};
+__device__ __forceinline__ DestDecode dest_decode(int nr, int iprot) {
+    DestDecode dd;
+    dd.dmask = iprot ? (uint32_t)(nr / 2 - 1) : (uint32_t)(nr - 1);
+    dd.dbase = iprot ? nr / 2 : 0;
+    return dd;
+}
+
+template <int NR, int K, int EXT>
+__device__ __forceinline__ void exec_word(uint32_t w, RegVec<NR> (&r)[K],
+                                          float cvec, DestDecode dd) {
+    const int op = (int)(w & 7u);
+    const int d = (int)((w >> 8) & dd.dmask) + dd.dbase;
+    const int s1 = (int)((w >> 16) & (uint32_t)(NR - 1));
+    const int s2 = (int)((w >> 24) & (uint32_t)(NR - 1));
+    const bool kc = (w >> 31) != 0u;
+
+#define GALOP_BIN(EXPR)                                                   \
+    do {                                                                  \
+        if (kc) {                                                         \
+            const float b = wave_bcast(cvec, s2);                         \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
+                const float a = r[k][s1];                                 \
+                r[k][d] = (EXPR);                                         \
+            }                                                             \
+        } else {                                                          \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
+                const float a = r[k][s1];                                 \
+                const float b = r[k][s2];                                 \
+                r[k][d] = (EXPR);                                         \
+            }                                                             \
+        }                                                                 \
+    } while (0)
+
+#define GALOP_UNARY(U, FN)                                                \
+    do {                                                                  \
+        if (kc) {                                                         \
+            const float v = wave_bcast(cvec, kUnaryTableBase + (U) * 16 + s2); \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) r[k][d] = v;    \
+        } else {                                                          \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
+                const float b = r[k][s2];                                 \
+                r[k][d] = FN(b);                                          \
+            }                                                             \
+        }                                                                 \
+    } while (0)
+
+    switch (op) {
+    case 0: GALOP_BIN(a + b); break;
+    case 1: GALOP_BIN(a - b); break;
+    case 2: GALOP_BIN(a * b); break;
+    case 3: GALOP_BIN(op_div(a, b)); break;
+    case 4: GALOP_UNARY(0, op_sin); break;
+    case 5: GALOP_UNARY(1, op_cos); break;
+    case 6:
+        if constexpr (EXT) GALOP_UNARY(2, op_exp);
+        else GALOP_BIN((a < b) ? 1.0f : 0.0f);
+        break;
+    default:
+        if constexpr (EXT) GALOP_UNARY(3, op_plog);
+        else GALOP_BIN((a > b) ? 1.0f : 0.0f);
+        break;
+    }
+#undef GALOP_BIN
+#undef GALOP_UNARY
+}
+
+// Run `len` words starting at `prog` on the K register vectors.  The next
+// 64-word chunk is loaded while the current one executes.
+template <int NR, int K, int EXT>
+__device__ __forceinline__ void run_program(const uint32_t* __restrict__ prog,
+                                            int64_t len, int lane,
+                                            RegVec<NR> (&r)[K], float cvec,
+                                            DestDecode dd) {
+    uint32_t next = (lane < len) ? prog[lane] : 0u;
+    for (int64_t base = 0; base < len; base += kWave) {
+        const uint32_t chunk = next;
+        const int64_t nb = base + kWave;
+        next = (nb + lane < len) ? prog[nb + lane] : 0u;
+        const int cnt = (int)((len - base) < kWave ? (len - base) : kWave);
+        for (int j = 0; j < cnt; ++j)
+            exec_word<NR, K, EXT>(wave_word(chunk, j), r, cvec, dd);
+    }
+}
+
+// Load the K cases of tile `tile` for this lane into registers
+// (src/Interpreter.cpp:43-52).
+template <int NR, int K>
+__device__ __forceinline__ void load_cases(const DevData& ds, int64_t tile,
+                                           int lane, RegVec<NR> (&r)[K]) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int64_t c = tile * (int64_t)(kWave * K) + k * kWave + lane;
+#pragma unroll
+        for (int i = 0; i < NR; ++i)
+            r[k][i] = (i < ds.n_inputs) ? ds.inputs[(int64_t)i * ds.n_pad + c] : 0.0f;
+    }
+}
+
+// Per-lane compensated accumulator of squared errors.
+struct LaneAcc {
+    float s = 0.0f;    // FP32 running sum
+    float c = 0.0f;    // compensation: rounding errors and low parts
+    double big = 0.0;  // squares of |err| >= 2^48, in FP64
+
+    __device__ __forceinline__ void add(float out, float y) {
+        const float d = out - y;
+        if (fabsf(d) < 0x1p48f) {
+            // exact residual of d = fl(out - y)   (TwoSum)
+            const float bb = d - out;
+            const float dlo = (out - (d - bb)) + (-y - bb);
+            // exact square split: p + plo == d*d   (FMA)
+            const float p = d * d;
+            const float plo = __builtin_fmaf(d, d, -p);
+            // Neumaier/TwoSum step of s += p
+            const float t = s + p;
+            const float z = t - s;
+            c += ((s - (t - z)) + (p - z)) + (plo + 2.0f * d * dlo);
+            s = t;
+        } else {
+            // huge or non-finite: exact FP64 difference, as the oracle does
+            const double dd = (double)out - (double)y;
+            big += dd * dd;
+        }
+    }
+    __device__ __forceinline__ double total() const {
+        return ((double)s + (double)c) + big;
+    }
+};
+
+// Fixed-order FP64 sum over the wave; the result is valid in lane 0.
+__device__ __forceinline__ double wave_sum_f64(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+    return v;
+}
+
+// Fitness from the FP64 SSE exactly as src/Evaluator.cpp:44 and :88-92:
+// mse = float(sse / N); fitness = finite ? 1/(1+mse) : 0.
+__device__ __forceinline__ float sse_to_fitness(double sse, int64_t n_cases) {
+    const float mse = (float)(sse / (double)n_cases);
+    if (!__builtin_isfinite(mse)) return 0.0f;
+    return 1.0f / (1.0f + mse);
+}
+
+// Evaluate one program on the whole dataset with one wave: returns the FP64
+// SSE in lane 0 (other lanes hold partial sums).  When case_out is non-null
+// the per-case outputs are written there instead of being accumulated.
+template <int NR, int K, int EXT>
+__device__ __forceinline__ double eval_program_wave(const uint32_t* __restrict__ prog,
+                                                    int64_t len, const DevData& ds,
+                                                    int lane, float cvec,
+                                                    float* __restrict__ case_out) {
+    const DestDecode dd = dest_decode(NR, ds.input_protect);
+    const int out_reg = ds.out_reg;
+    const int64_t tile_cases = (int64_t)kWave * K;
+    const int64_t n_tiles = (ds.n_cases + tile_cases - 1) / tile_cases;
+    LaneAcc acc;
+    for (int64_t t = 0; t < n_tiles; ++t) {
+        RegVec<NR> r[K];
+        load_cases<NR, K>(ds, t, lane, r);
+        run_program<NR, K, EXT>(prog, len, lane, r, cvec, dd);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int64_t c = t * tile_cases + k * kWave + lane;
+            if (c < ds.n_cases) {
+                const float out = r[k][out_reg];
+                if (case_out) case_out[c] = out;
+                else acc.add(out, ds.targets[c]);
+            }
+        }
+    }
+    return wave_sum_f64(acc.total());
+}
+
+}  // namespace galop

@@ -1,0 +1,185 @@
+/*
+ * galop_hip.h -- C ABI of the MI355X (gfx950) fitness evaluator for Galop's
+ * linear-GP engine.
+ *
+ * The reference evaluates a population through one virtual call,
+ *   EvalBackend::evaluate_population(instructions, lengths, fitness_out,
+ *                                    n_programs, const Dataset&)
+ * (src/EvalBackend.h:11-17), implemented on the GPU by
+ * CudaEvaluator::evaluate_population (src/EvaluatorCuda.cu:298-469).  This
+ * header is the plain-C boundary underneath the drop-in C++ adapter
+ * galop_amd/host/HipEvaluator.h, which implements EvalBackend unchanged and
+ * forwards to galop_eval_dense() below.  All signatures use plain pointers and
+ * sizes; device pointers are ordinary HIP device addresses and streams are
+ * hipStream_t passed as void*.
+ *
+ * Instruction words are the reference's 32-bit format (src/ISA.h:20-25):
+ *   [31]=src2 is constant, [30:24]=src2 index, [23:16]=src1, [15:8]=dest,
+ *   [7:0]=opcode; every field is masked at decode (src/ISA.h:176-206), so any
+ *   32-bit word is a valid instruction.
+ *
+ * Error behaviour: every entry point returns GALOP_OK (0) or a GALOP_ERR_*
+ * code and records a message readable with galop_last_error().  The reference
+ * aborts the process on any CUDA error (src/cuda_errors.h:3-15); the C++
+ * adapter restores that behaviour on top of these codes.
+ */
+#ifndef GALOP_HIP_H
+#define GALOP_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GALOP_ABI_VERSION 1
+
+/* ---- status codes ---- */
+#define GALOP_OK            0
+#define GALOP_ERR_INVALID   1  /* bad argument, size or handle            */
+#define GALOP_ERR_HIP       2  /* HIP runtime error                        */
+#define GALOP_ERR_NOMEM     3  /* device/pinned allocation failed          */
+#define GALOP_ERR_NODEVICE  4  /* no HIP device / gfx950 code missing      */
+#define GALOP_ERR_BUDGET    5  /* request larger than the server HBM budget*/
+#define GALOP_ERR_TIMEOUT   6  /* galop_wait timed out                     */
+#define GALOP_ERR_CANCELLED 7  /* request cancelled / server shut down     */
+
+/* ---- op tables (3-bit opcode field, src/ISA.h:69-72) ---- */
+#define GALOP_OPS_REF8 0  /* {+,-,*,/p,sin,cos,<,>}: the reference's table     */
+#define GALOP_OPS_EXT8 1  /* {+,-,*,/p,sin,cos,exp,logp}: opcodes 6,7 replaced */
+
+/* ---- kernel variants ---- */
+#define GALOP_KERNEL_PLAIN   0 /* one program per wave64, one launch        */
+#define GALOP_KERNEL_BATCHED 1 /* length-bucketed persistent work queue     */
+
+/* ---- dense layouts (CudaEvaluator::Layout, src/EvaluatorCuda.h:24) ---- */
+#define GALOP_LAYOUT_PACKED    1 /* CSR of the raw programs                 */
+#define GALOP_LAYOUT_EFFECTIVE 2 /* CSR of intron-stripped programs         */
+
+/* Per-problem ISA configuration: the reference's compile-time LGPConfig
+ * (NUM_REGISTERS src/LGPConfig.h:132-140, CONSTANTS :151-170, INPUT_PROTECT /
+ * OUTPUT_REG :289-300) as run-time data. */
+typedef struct galop_isa {
+    int32_t num_registers;  /* 8 or 16                                     */
+    int32_t op_set;         /* GALOP_OPS_REF8 or GALOP_OPS_EXT8             */
+    int32_t input_protect;  /* 1: dest folded into upper half, out=NR/2     */
+    int32_t reserved;       /* must be 0                                   */
+    float   constants[16];  /* constant pool, first num_registers used     */
+} galop_isa;
+
+typedef struct galop_ctx galop_ctx;
+typedef struct galop_dataset galop_dataset;
+typedef struct galop_batch galop_batch;
+typedef struct galop_server galop_server;
+
+/* ---- library / context ---------------------------------------------------
+ * A context owns one device, one stream and the staging buffers of the
+ * synchronous entry points (CudaEvaluator's ctor/dtor, src/EvaluatorCuda.cu:21-93).
+ * A context is single-caller, like the reference backend. */
+int32_t     galop_abi_version(void);
+int         galop_ctx_create(int32_t device, galop_ctx** out);
+void        galop_ctx_destroy(galop_ctx* ctx);
+const char* galop_last_error(const galop_ctx* ctx); /* ctx may be NULL */
+/* Cases per lane (1,2,4,8) and waves per workgroup (1..4) of the plain
+ * kernel; 0 = automatic.  Fitness does not depend on either. */
+int         galop_ctx_set_tuning(galop_ctx* ctx, int32_t cases_per_lane,
+                                 int32_t waves_per_block);
+void*       galop_ctx_stream(galop_ctx* ctx);
+
+/* ---- datasets ------------------------------------------------------------
+ * inputs: case-major AoS inputs[c*n_inputs+i] (src/Dataset.h:22-40, unpadded
+ * or padded -- only the first n_cases rows are read); targets[c].  The device
+ * copy is feature-planar and zero-padded; it stays resident until destroy
+ * (the reference uploads once per Dataset object, src/EvaluatorCuda.cu:96-110). */
+int     galop_dataset_create(galop_ctx* ctx, const galop_isa* isa,
+                             const float* inputs, const float* targets,
+                             int64_t n_cases, int32_t n_inputs,
+                             galop_dataset** out);
+void    galop_dataset_destroy(galop_dataset* ds);
+int64_t galop_dataset_device_bytes(const galop_dataset* ds);
+
+/* ---- synchronous host-buffer entry points --------------------------------
+ * galop_eval_dense: EvalBackend::evaluate_population (src/EvalBackend.h:11-17)
+ * with the dense program-major buffer of the engine: program i at
+ * instructions + i*stride, lengths[i] words.  layout = GALOP_LAYOUT_PACKED or
+ * GALOP_LAYOUT_EFFECTIVE (src/EvaluatorCuda.cu:335-422).  Blocking. */
+int galop_eval_dense(galop_ctx* ctx, const galop_dataset* ds,
+                     const uint32_t* instructions, const uint8_t* lengths,
+                     int32_t stride, int64_t n_programs, float* fitness_out,
+                     int32_t layout);
+/* CSR programs (src/EvaluatorCuda.cu:241-265): program i is
+ * words[offsets[i]:offsets[i+1]], int64 offsets, no length limit.
+ * sse_out (nullable) receives the FP64 sum of squared errors.  Blocking. */
+int galop_eval_csr(galop_ctx* ctx, const galop_dataset* ds,
+                   const uint32_t* words, const int64_t* offsets,
+                   int64_t n_programs, float* fitness_out, double* sse_out);
+
+/* ---- device-resident entry points (asynchronous on `stream`; NULL = the
+ * context stream).  All pointers are device memory. ---------------------- */
+int galop_eval_csr_device(galop_ctx* ctx, const galop_dataset* ds,
+                          const uint32_t* d_words, const int64_t* d_offsets,
+                          int64_t n_programs, float* d_fitness, double* d_sse,
+                          int32_t kernel, void* stream);
+/* Per-case outputs r[OUTPUT_REG] (src/Interpreter.cpp:55-57) of every program
+ * on every case: d_out[p*n_cases + c]. */
+int galop_run_cases_device(galop_ctx* ctx, const galop_dataset* ds,
+                           const uint32_t* d_words, const int64_t* d_offsets,
+                           int64_t n_programs, float* d_out, void* stream);
+/* Device intron removal (src/Introns.h:28-57 per program + a prefix scan):
+ * d_eff_offsets[n+1], d_eff_words (capacity >= total raw words).  Blocking:
+ * returns the effective word total in *eff_total. */
+int galop_strip_introns_device(galop_ctx* ctx, const galop_dataset* ds,
+                               const uint32_t* d_words, const int64_t* d_offsets,
+                               int64_t n_programs, uint32_t* d_eff_words,
+                               int64_t* d_eff_offsets, int64_t* eff_total,
+                               void* stream);
+
+/* ---- batched persistent path ----------------------------------------------
+ * A batch plan owns the work-queue and bucketing scratch for populations of
+ * up to max_programs programs on one dataset, and a hipGraph of the
+ * generation (bucket -> order -> persistent evaluation) that is re-launched
+ * while the program buffers stay the same. */
+int  galop_batch_create(galop_ctx* ctx, const galop_dataset* ds,
+                        int64_t max_programs, galop_batch** out);
+void galop_batch_destroy(galop_batch* b);
+/* Evaluate n <= max_programs CSR programs (device pointers).  use_graph=1
+ * captures the sequence on first use and replays it afterwards. */
+int  galop_batch_eval_device(galop_batch* b, const uint32_t* d_words,
+                             const int64_t* d_offsets, int64_t n_programs,
+                             float* d_fitness, double* d_sse, int32_t use_graph,
+                             void* stream);
+/* Autotune (cases-per-lane x waves-per-block x program staging) on the given
+ * population; the winner is kept in the plan.  Writes the chosen values. */
+int  galop_batch_autotune(galop_batch* b, const uint32_t* d_words,
+                          const int64_t* d_offsets, int64_t n_programs,
+                          float* d_fitness, int32_t* cases_per_lane,
+                          int32_t* waves_per_block, int32_t* workers_per_cu);
+
+/* ---- serving: many evolution runs, continuous batching -----------------
+ * A server keeps datasets and populations resident, admits requests while
+ * their device bytes fit under hbm_budget_bytes, and overlaps admitted
+ * requests on n_streams HIP streams.  galop_submit copies the population to
+ * device memory and returns at once; fitness_out/sse_out must stay valid
+ * until galop_wait reports completion. */
+int  galop_server_create(galop_ctx* ctx, int64_t hbm_budget_bytes,
+                         int32_t n_streams, galop_server** out);
+void galop_server_destroy(galop_server* s);
+int  galop_submit(galop_server* s, const galop_dataset* ds,
+                  const uint32_t* words, const int64_t* offsets,
+                  int64_t n_programs, float* fitness_out, double* sse_out,
+                  int64_t* request_id);
+/* 0 = done (status in *status), 1 = pending */
+int  galop_poll(galop_server* s, int64_t request_id, int32_t* status);
+int  galop_wait(galop_server* s, int64_t request_id, double timeout_s);
+/* submit->result-visible latency and device execution time of a completed
+ * request, in seconds. */
+int  galop_request_times(galop_server* s, int64_t request_id,
+                         double* latency_s, double* device_s);
+int  galop_server_stats(galop_server* s, int64_t* admitted, int64_t* completed,
+                        int64_t* peak_bytes, int64_t* queued);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GALOP_HIP_H */

@@ -1,0 +1,262 @@
+/*
+ * galop_oracle.c -- CPU restatement of Galop's evaluation semantics.
+ * TEST INFRASTRUCTURE ONLY (the checker); see galop_oracle.h for the list of
+ * reference functions each routine follows.
+ *
+ * Build: gcc -O2 -fPIC -ffp-contract=off -fno-fast-math -fopenmp (oracle/Makefile).
+ * Float ops are single IEEE operations in program order; sin/cos/exp/log
+ * call the host C library exactly as std::sin(float) does in the reference.
+ */
+#include "galop_oracle.h"
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define GO_CHUNK 32 /* LGPConfig::NUM_CONTEXTS (src/LGPConfig.h:309) */
+#define GO_MAXR 16
+
+/* ---- decode: src/ISA.h:176-206 ---------------------------------------- */
+int go_get_op(uint32_t w) { return (int)(w & 7u); }
+
+int go_output_reg(const go_isa* isa) {
+    /* src/LGPConfig.h:294-300: OUTPUT_REG = NR/2 under INPUT_PROTECT, else 0 */
+    return isa->input_protect ? isa->num_registers / 2 : 0;
+}
+
+int go_get_dest(const go_isa* isa, uint32_t w) {
+    const uint32_t mask = (uint32_t)isa->num_registers - 1u;
+    if (isa->input_protect)
+        return go_output_reg(isa) + (int)((w >> 8) & (mask >> 1));
+    return (int)((w >> 8) & mask);
+}
+
+int go_get_src1(const go_isa* isa, uint32_t w) {
+    return (int)((w >> 16) & ((uint32_t)isa->num_registers - 1u));
+}
+
+int go_get_src2(const go_isa* isa, uint32_t w) {
+    return (int)((w >> 24) & ((uint32_t)isa->num_registers - 1u));
+}
+
+int go_src2_is_const(uint32_t w) { return ((w >> 24) & 0x80u) != 0; }
+
+/* ---- apply_op: src/ISA.h:219-231 (REF8) plus the EXT8 table ------------ */
+float go_apply_op(int op_set, int op, float a, float b) {
+    switch (op) {
+    case 0: return a + b;
+    case 1: return a - b;
+    case 2: return a * b;
+    case 3: return (b != 0.0f) ? a / b : 1.0f;
+    case 4: return sinf(b);
+    case 5: return cosf(b);
+    case 6:
+        if (op_set == GO_OPS_EXT8) return expf(b);
+        return (a < b) ? 1.0f : 0.0f;
+    default:
+        if (op_set == GO_OPS_EXT8) return (b != 0.0f) ? logf(fabsf(b)) : 0.0f;
+        return (a > b) ? 1.0f : 0.0f;
+    }
+}
+
+static int op_is_unary(int op_set, int op) {
+    if (op == 4 || op == 5) return 1;
+    return op_set == GO_OPS_EXT8 && (op == 6 || op == 7);
+}
+
+/* ---- interpreter: src/Interpreter.cpp:11-58 ---------------------------
+ * Decode once per instruction, then apply to every context of the chunk
+ * (the reference's 32-context layout).  Registers are FP32. */
+static void run_chunk(const go_isa* isa, const uint32_t* prog, int64_t len,
+                      const float* inputs, int32_t n_inputs, int valid,
+                      float* out) {
+    const int nr = isa->num_registers;
+    float regs[GO_CHUNK][GO_MAXR];
+    for (int c = 0; c < valid; ++c) {
+        for (int i = 0; i < nr; ++i)
+            regs[c][i] = (i < n_inputs) ? inputs[(int64_t)c * n_inputs + i] : 0.0f;
+    }
+    for (int64_t pc = 0; pc < len; ++pc) {
+        const uint32_t w = prog[pc];
+        const int op = go_get_op(w);
+        const int d = go_get_dest(isa, w);
+        const int s1 = go_get_src1(isa, w);
+        const int s2 = go_get_src2(isa, w);
+        const int k2 = go_src2_is_const(w);
+        const float kval = isa->constants[s2];
+        for (int c = 0; c < valid; ++c) {
+            const float a = regs[c][s1];
+            const float b = k2 ? kval : regs[c][s2];
+            regs[c][d] = go_apply_op(isa->op_set, op, a, b);
+        }
+    }
+    const int o = go_output_reg(isa);
+    for (int c = 0; c < valid; ++c) out[c] = regs[c][o];
+}
+
+void go_run_program(const go_isa* isa, const uint32_t* prog, int64_t len,
+                    const float* inputs, int32_t n_inputs, int64_t n_cases,
+                    float* out) {
+    for (int64_t base = 0; base < n_cases; base += GO_CHUNK) {
+        const int valid = (int)((n_cases - base) < GO_CHUNK ? (n_cases - base) : GO_CHUNK);
+        run_chunk(isa, prog, len, inputs + base * n_inputs, n_inputs, valid, out + base);
+    }
+}
+
+/* ---- src/Evaluator.cpp:11-45: FP64 err and SSE in case order ---------- */
+double go_program_sse(const go_isa* isa, const uint32_t* prog, int64_t len,
+                      const float* inputs, const float* targets,
+                      int32_t n_inputs, int64_t n_cases) {
+    double sse = 0.0;
+    float outs[GO_CHUNK];
+    for (int64_t base = 0; base < n_cases; base += GO_CHUNK) {
+        const int valid = (int)((n_cases - base) < GO_CHUNK ? (n_cases - base) : GO_CHUNK);
+        run_chunk(isa, prog, len, inputs + base * n_inputs, n_inputs, valid, outs);
+        for (int c = 0; c < valid; ++c) {
+            const double err = (double)outs[c] - (double)targets[base + c];
+            sse += err * err;
+        }
+    }
+    return sse;
+}
+
+/* src/Evaluator.cpp:88-92 */
+float go_mse_to_fitness(float mse) {
+    if (!isfinite(mse)) return 0.0f;
+    return 1.0f / (1.0f + mse);
+}
+
+/* src/Evaluator.cpp:44: return float(sse / double(N)) */
+float go_sse_to_fitness(double sse, int64_t n_cases) {
+    const float mse = (float)(sse / (double)n_cases);
+    return go_mse_to_fitness(mse);
+}
+
+static int pick_threads(int32_t threads) {
+#ifdef _OPENMP
+    return threads > 0 ? threads : omp_get_num_procs();
+#else
+    (void)threads;
+    return 1;
+#endif
+}
+
+void go_eval_csr(const go_isa* isa, const uint32_t* words, const int64_t* offsets,
+                 int64_t n_programs, const float* inputs, const float* targets,
+                 int32_t n_inputs, int64_t n_cases, float* fitness_out,
+                 double* sse_out, int32_t threads) {
+    const int nt = pick_threads(threads);
+    (void)nt;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+    for (int64_t i = 0; i < n_programs; ++i) {
+        const double sse = go_program_sse(isa, words + offsets[i], offsets[i + 1] - offsets[i],
+                                          inputs, targets, n_inputs, n_cases);
+        if (sse_out) sse_out[i] = sse;
+        fitness_out[i] = go_sse_to_fitness(sse, n_cases);
+    }
+}
+
+/* src/EvaluatorCpu.cpp:7-16 */
+void go_eval_dense(const go_isa* isa, const uint32_t* instructions,
+                   const uint8_t* lengths, int32_t stride, int64_t n_programs,
+                   const float* inputs, const float* targets, int32_t n_inputs,
+                   int64_t n_cases, float* fitness_out, int32_t threads) {
+    const int nt = threads > 1 ? pick_threads(threads) : 1;
+    (void)nt;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+    for (int64_t i = 0; i < n_programs; ++i) {
+        const double sse = go_program_sse(isa, instructions + (size_t)i * (size_t)stride,
+                                          (int64_t)lengths[i], inputs, targets,
+                                          n_inputs, n_cases);
+        fitness_out[i] = go_sse_to_fitness(sse, n_cases);
+    }
+}
+
+/* ---- src/Introns.h:28-57: backward liveness from the output register --- */
+int64_t go_effective_mask(const go_isa* isa, const uint32_t* prog, int64_t len,
+                          uint8_t* mask) {
+    uint8_t live[GO_MAXR];
+    memset(live, 0, sizeof(live));
+    live[go_output_reg(isa)] = 1;
+    int64_t cnt = 0;
+    for (int64_t i = len - 1; i >= 0; --i) {
+        const uint32_t w = prog[i];
+        const int d = go_get_dest(isa, w);
+        if (!live[d]) { mask[i] = 0; continue; }
+        mask[i] = 1;
+        ++cnt;
+        live[d] = 0;                       /* killed by this write ... */
+        if (!op_is_unary(isa->op_set, go_get_op(w)))
+            live[go_get_src1(isa, w)] = 1; /* ... then its reads go live */
+        if (!go_src2_is_const(w))
+            live[go_get_src2(isa, w)] = 1;
+    }
+    return cnt;
+}
+
+int64_t go_extract_effective(const go_isa* isa, const uint32_t* prog,
+                             int64_t len, uint32_t* out) {
+    uint8_t* mask = (uint8_t*)malloc(len > 0 ? (size_t)len : 1);
+    const int64_t cnt = go_effective_mask(isa, prog, len, mask);
+    int64_t k = 0;
+    for (int64_t i = 0; i < len; ++i)
+        if (mask[i]) out[k++] = prog[i];
+    free(mask);
+    return cnt;
+}
+
+/* ---- tests/cuda_diff.cu:92-144, with protected division in the double leg */
+static double apply_op_f64(int op_set, int op, double a, double b) {
+    switch (op) {
+    case 0: return a + b;
+    case 1: return a - b;
+    case 2: return a * b;
+    case 3: return (b != 0.0) ? a / b : 1.0;
+    case 4: return sin(b);
+    case 5: return cos(b);
+    case 6:
+        if (op_set == GO_OPS_EXT8) return exp(b);
+        return (a < b) ? 1.0 : 0.0;
+    default:
+        if (op_set == GO_OPS_EXT8) return (b != 0.0) ? log(fabs(b)) : 0.0;
+        return (a > b) ? 1.0 : 0.0;
+    }
+}
+
+void go_dual_precision_fitness(const go_isa* isa, const uint32_t* prog,
+                               int64_t len, const float* inputs,
+                               const float* targets, int32_t n_inputs,
+                               int64_t n_cases, float* fit_f, float* fit_d) {
+    const int nr = isa->num_registers;
+    const int o = go_output_reg(isa);
+    double sse_f = 0.0, sse_d = 0.0;
+    for (int64_t c = 0; c < n_cases; ++c) {
+        float fr[GO_MAXR];
+        double dr[GO_MAXR];
+        for (int i = 0; i < nr; ++i) {
+            fr[i] = (i < n_inputs) ? inputs[c * n_inputs + i] : 0.0f;
+            dr[i] = (double)fr[i];
+        }
+        for (int64_t pc = 0; pc < len; ++pc) {
+            const uint32_t w = prog[pc];
+            const int op = go_get_op(w), d = go_get_dest(isa, w);
+            const int s1 = go_get_src1(isa, w), s2 = go_get_src2(isa, w);
+            const int k2 = go_src2_is_const(w);
+            const float fb = k2 ? isa->constants[s2] : fr[s2];
+            const double db = k2 ? (double)isa->constants[s2] : dr[s2];
+            fr[d] = go_apply_op(isa->op_set, op, fr[s1], fb);
+            dr[d] = apply_op_f64(isa->op_set, op, dr[s1], db);
+        }
+        const double t = (double)targets[c];
+        const double ef = (double)fr[o] - t;
+        const double ed = dr[o] - t;
+        sse_f += ef * ef;
+        sse_d += ed * ed;
+    }
+    const float mse_f = (float)(sse_f / (double)n_cases);
+    const float mse_d = (float)(sse_d / (double)n_cases);
+    *fit_f = go_mse_to_fitness(mse_f);
+    *fit_d = go_mse_to_fitness(mse_d);
+}

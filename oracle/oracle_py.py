@@ -1,0 +1,173 @@
+"""ctypes binding of the CPU oracle (libgalop_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the
+CPU-baseline leg of bench.py as the checker.  Never imported by galop_amd.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libgalop_oracle.so")
+
+OPS_REF8 = 0
+OPS_EXT8 = 1
+
+
+class go_isa(ctypes.Structure):
+    _fields_ = [("num_registers", ctypes.c_int32), ("op_set", ctypes.c_int32),
+                ("input_protect", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("constants", ctypes.c_float * 16)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            raise RuntimeError(f"{LIB} missing: run `make -C oracle`")
+        L = ctypes.CDLL(LIB)
+        vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+        pisa = ctypes.POINTER(go_isa)
+        L.go_apply_op.restype = ctypes.c_float
+        L.go_apply_op.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_float]
+        L.go_run_program.restype = None
+        L.go_run_program.argtypes = [pisa, vp, i64, vp, i32, i64, vp]
+        L.go_program_sse.restype = ctypes.c_double
+        L.go_program_sse.argtypes = [pisa, vp, i64, vp, vp, i32, i64]
+        L.go_sse_to_fitness.restype = ctypes.c_float
+        L.go_sse_to_fitness.argtypes = [ctypes.c_double, i64]
+        L.go_mse_to_fitness.restype = ctypes.c_float
+        L.go_mse_to_fitness.argtypes = [ctypes.c_float]
+        L.go_eval_csr.restype = None
+        L.go_eval_csr.argtypes = [pisa, vp, vp, i64, vp, vp, i32, i64, vp, vp, i32]
+        L.go_eval_dense.restype = None
+        L.go_eval_dense.argtypes = [pisa, vp, vp, i32, i64, vp, vp, i32, i64, vp, i32]
+        L.go_effective_mask.restype = i64
+        L.go_effective_mask.argtypes = [pisa, vp, i64, vp]
+        L.go_extract_effective.restype = i64
+        L.go_extract_effective.argtypes = [pisa, vp, i64, vp]
+        L.go_dual_precision_fitness.restype = None
+        L.go_dual_precision_fitness.argtypes = [pisa, vp, i64, vp, vp, i32, i64,
+                                                ctypes.POINTER(ctypes.c_float),
+                                                ctypes.POINTER(ctypes.c_float)]
+        for name in ("go_get_op", "go_src2_is_const"):
+            getattr(L, name).restype = ctypes.c_int
+            getattr(L, name).argtypes = [ctypes.c_uint32]
+        for name in ("go_get_dest", "go_get_src1", "go_get_src2"):
+            getattr(L, name).restype = ctypes.c_int
+            getattr(L, name).argtypes = [pisa, ctypes.c_uint32]
+        L.go_output_reg.restype = ctypes.c_int
+        L.go_output_reg.argtypes = [pisa]
+        _lib = L
+    return _lib
+
+
+def make_isa(num_registers=8, op_set=OPS_REF8, input_protect=False, constants=None):
+    s = go_isa()
+    s.num_registers = num_registers
+    s.op_set = op_set
+    s.input_protect = 1 if input_protect else 0
+    s.reserved = 0
+    if constants is None:
+        from galop_amd.evaluator import REF_CONSTANTS_8, REF_CONSTANTS_16
+        constants = REF_CONSTANTS_8 if num_registers == 8 else REF_CONSTANTS_16
+    for i in range(16):
+        s.constants[i] = float(constants[i]) if i < len(constants) else 0.0
+    return s
+
+
+def isa_from(isa):
+    """Oracle ISA from a galop_amd.Isa."""
+    return make_isa(isa.num_registers, isa.op_set, isa.input_protect, list(isa.constants))
+
+
+def _p(a):
+    return a.ctypes.data if a is not None and a.size else None
+
+
+def run_program(isa, prog, inputs, n_cases=None):
+    """Per-case outputs r[OUTPUT_REG]; inputs shape (N, k)."""
+    prog = np.ascontiguousarray(prog, np.uint32)
+    x = np.ascontiguousarray(inputs, np.float32)
+    if x.ndim == 1:
+        x = x[:, None]
+    n = x.shape[0] if n_cases is None else n_cases
+    out = np.empty(n, np.float32)
+    lib().go_run_program(ctypes.byref(isa), _p(prog), prog.size, _p(x), x.shape[1], n, _p(out))
+    return out
+
+
+def run_csr(isa, words, offsets, inputs):
+    offsets = np.asarray(offsets, np.int64)
+    words = np.asarray(words, np.uint32)
+    n = offsets.size - 1
+    x = np.ascontiguousarray(inputs, np.float32)
+    if x.ndim == 1:
+        x = x[:, None]
+    out = np.empty((n, x.shape[0]), np.float32)
+    for i in range(n):
+        out[i] = run_program(isa, words[offsets[i]:offsets[i + 1]], x)
+    return out
+
+
+def eval_csr(isa, words, offsets, inputs, targets, threads=0):
+    words = np.ascontiguousarray(words, np.uint32)
+    offsets = np.ascontiguousarray(offsets, np.int64)
+    x = np.ascontiguousarray(inputs, np.float32)
+    if x.ndim == 1:
+        x = x[:, None]
+    y = np.ascontiguousarray(targets, np.float32)
+    n = offsets.size - 1
+    fit = np.empty(n, np.float32)
+    sse = np.empty(n, np.float64)
+    lib().go_eval_csr(ctypes.byref(isa), _p(words), _p(offsets), n, _p(x), _p(y), x.shape[1],
+                      y.size, _p(fit), _p(sse), int(threads))
+    return fit, sse
+
+
+def eval_dense(isa, instructions, lengths, inputs, targets, threads=1):
+    instr = np.ascontiguousarray(instructions, np.uint32)
+    lengths = np.ascontiguousarray(lengths, np.uint8)
+    x = np.ascontiguousarray(inputs, np.float32)
+    if x.ndim == 1:
+        x = x[:, None]
+    y = np.ascontiguousarray(targets, np.float32)
+    n = lengths.size
+    fit = np.empty(n, np.float32)
+    lib().go_eval_dense(ctypes.byref(isa), _p(instr), _p(lengths), instr.shape[1], n, _p(x), _p(y),
+                        x.shape[1], y.size, _p(fit), int(threads))
+    return fit
+
+
+def effective_mask(isa, prog):
+    prog = np.ascontiguousarray(prog, np.uint32)
+    mask = np.zeros(max(prog.size, 1), np.uint8)
+    cnt = lib().go_effective_mask(ctypes.byref(isa), _p(prog), prog.size, _p(mask))
+    return mask[: prog.size], int(cnt)
+
+
+def extract_effective(isa, prog):
+    prog = np.ascontiguousarray(prog, np.uint32)
+    out = np.zeros(max(prog.size, 1), np.uint32)
+    cnt = lib().go_extract_effective(ctypes.byref(isa), _p(prog), prog.size, _p(out))
+    return out[:cnt]
+
+
+def apply_op(op_set, op, a, b):
+    return lib().go_apply_op(op_set, op, a, b)
+
+
+def dual_precision_fitness(isa, prog, inputs, targets):
+    prog = np.ascontiguousarray(prog, np.uint32)
+    x = np.ascontiguousarray(inputs, np.float32)
+    if x.ndim == 1:
+        x = x[:, None]
+    y = np.ascontiguousarray(targets, np.float32)
+    ff, fd = ctypes.c_float(), ctypes.c_float()
+    lib().go_dual_precision_fitness(ctypes.byref(isa), _p(prog), prog.size, _p(x), _p(y),
+                                    x.shape[1], y.size, ctypes.byref(ff), ctypes.byref(fd))
+    return ff.value, fd.value

@@ -1,0 +1,108 @@
+"""Write the golden-vector fixtures in tests/golden/ from the Galop reference.
+
+Runs the reference's own CPU code, compiled from its sources by
+``make -C oracle ref`` into ``oracle/_ref/`` (see oracle/ref_dump.cpp), and
+packs what it writes into small ``.npz`` files.  Only the vectors are
+committed; the reference itself never leaves the container that has it.
+
+Fixtures:
+  percase_r8.npz     REF8, 8 registers, 1 input: per-case outputs of
+                     Interpreter::run_stateless (src/Interpreter.cpp:36-58),
+                     fitness of Evaluator::evaluate_sr_mse
+                     (src/Evaluator.cpp:11-45,88-92), intron masks
+                     (src/Introns.h:28-46)
+  percase_r16.npz    same with NUM_REGISTERS_OVERRIDE=16, 5 inputs
+  percase_r16ip.npz  same with INPUT_PROTECT_OVERRIDE=1 as well
+  engine_seed42.npz  LGPEngine(42).init_evolution() population and the
+                     population after evolve_sr on datasets/nguyen1_train.csv
+                     (the cuda_diff populations, tests/cuda_diff.cu:57-73,
+                     205-222) with CpuEvaluator fitness and intron masks
+
+usage: python tests/golden/make_fixtures.py [--ref /root/reference]
+"""
+import argparse
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REPO = os.path.dirname(os.path.dirname(HERE))
+REFBIN = os.path.join(REPO, "oracle", "_ref")
+
+
+def _read(prefix, name, dtype):
+    return np.fromfile(f"{prefix}.{name}", dtype=dtype)
+
+
+def percase(tool, seed, n_programs, max_len, n_cases, n_inputs, out_name, meta):
+    with tempfile.TemporaryDirectory() as tmp:
+        prefix = os.path.join(tmp, "d")
+        subprocess.run([os.path.join(REFBIN, tool), "percase", str(seed), str(n_programs),
+                        str(max_len), str(n_cases), str(n_inputs), prefix], check=True)
+        arrays = dict(
+            offsets=_read(prefix, "offsets.i64", np.int64),
+            words=_read(prefix, "words.u32", np.uint32),
+            inputs=_read(prefix, "inputs.f32", np.float32).reshape(n_cases, n_inputs),
+            targets=_read(prefix, "targets.f32", np.float32),
+            outputs=_read(prefix, "outputs.f32", np.float32).reshape(n_programs, n_cases),
+            fitness=_read(prefix, "fitness.f32", np.float32),
+            effmask=_read(prefix, "effmask.u8", np.uint8),
+            effcount=_read(prefix, "effcount.i64", np.int64),
+            constants=_read(prefix, "constants.f32", np.float32),
+        )
+    arrays.update({k: np.array(v) for k, v in meta.items()})
+    np.savez_compressed(os.path.join(HERE, out_name), **arrays)
+    print("wrote", out_name)
+
+
+def engine(seed, csv, out_name):
+    with tempfile.TemporaryDirectory() as tmp:
+        prefix = os.path.join(tmp, "e")
+        subprocess.run([os.path.join(REFBIN, "ref_dump_r8"), "engine", str(seed), csv, prefix],
+                       check=True)
+        arrays = {}
+        stride = 50  # LGPConfig::MAX_PROGRAM_SIZE default (src/LGPConfig.h:103)
+        for tag in ("init", "evolved"):
+            dense = _read(prefix, f"{tag}.dense.u32", np.uint32)
+            lengths = _read(prefix, f"{tag}.lengths.u8", np.uint8)
+            n = lengths.size
+            dense = dense.reshape(n, stride)
+            mask = _read(prefix, f"{tag}.effmask.u8", np.uint8).reshape(n, stride)
+            offsets = np.zeros(n + 1, np.int64)
+            offsets[1:] = np.cumsum(lengths.astype(np.int64))
+            words = np.concatenate([dense[i, :lengths[i]] for i in range(n)])
+            effmask = np.concatenate([mask[i, :lengths[i]] for i in range(n)])
+            arrays[f"{tag}_offsets"] = offsets
+            arrays[f"{tag}_words"] = words
+            arrays[f"{tag}_lengths"] = lengths
+            arrays[f"{tag}_fitness"] = _read(prefix, f"{tag}.fitness.f32", np.float32)
+            arrays[f"{tag}_effmask"] = effmask
+            arrays[f"{tag}_effcount"] = _read(prefix, f"{tag}.effcount.i64", np.int64)
+        arrays["inputs"] = _read(prefix, "inputs.f32", np.float32)
+        arrays["targets"] = _read(prefix, "targets.f32", np.float32)
+        arrays["stride"] = np.array(stride)
+    np.savez_compressed(os.path.join(HERE, out_name), **arrays)
+    print("wrote", out_name)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref", default="/root/reference")
+    args = ap.parse_args()
+    subprocess.run(["make", "-C", os.path.join(REPO, "oracle"), "ref", f"REF={args.ref}"],
+                   check=True)
+    percase("ref_dump_r8", 1, 512, 48, 64, 1, "percase_r8.npz",
+            dict(num_registers=8, input_protect=0, op_set=0, seed=1))
+    percase("ref_dump_r16", 2, 256, 48, 64, 5, "percase_r16.npz",
+            dict(num_registers=16, input_protect=0, op_set=0, seed=2))
+    percase("ref_dump_r16ip", 3, 256, 48, 64, 5, "percase_r16ip.npz",
+            dict(num_registers=16, input_protect=1, op_set=0, seed=3))
+    engine(42, os.path.join(args.ref, "datasets", "nguyen1_train.csv"), "engine_seed42.npz")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
