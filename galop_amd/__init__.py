@@ -6,6 +6,6 @@ a ctypes binding that mirrors the reference's evaluation interface, and the
 synthetic-workload helpers used by the benchmark.
 """
 from ._lib import GalopError, load  # noqa: F401
-from .evaluator import (BatchPlan, Context, Dataset, DeviceDataset, HipEvaluator,  # noqa: F401
+from .evaluator import (BatchPlan, Context, Dataset, DeviceDataset, HipEvaluator, Server,  # noqa: F401,E501
                         Isa, eval_csr, eval_csr_device, eval_dense, run_cases_device,
                         strip_introns_device)

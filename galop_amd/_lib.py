@@ -25,6 +25,7 @@ KERNEL_PLAIN = 0
 KERNEL_BATCHED = 1
 LAYOUT_PACKED = 1
 LAYOUT_EFFECTIVE = 2
+BATCH_STRIP_INTRONS = 1
 
 _ERR_NAMES = {1: "INVALID", 2: "HIP", 3: "NOMEM", 4: "NODEVICE", 5: "BUDGET", 6: "TIMEOUT",
               7: "CANCELLED"}
@@ -67,7 +68,7 @@ SIGNATURES = {
     "galop_run_cases_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "galop_strip_introns_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _pi64,
                                                   _vp]),
-    "galop_batch_create": (ctypes.c_int, [_vp, _vp, _i64, ctypes.POINTER(_vp)]),
+    "galop_batch_create": (ctypes.c_int, [_vp, _vp, _i64, _i64, _i32, ctypes.POINTER(_vp)]),
     "galop_batch_destroy": (None, [_vp]),
     "galop_batch_eval_device": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i32, _vp]),
     "galop_batch_autotune": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _pi32, _pi32, _pi32]),

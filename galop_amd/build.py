@@ -7,6 +7,9 @@ Every HIP source is compiled for gfx950 only, with:
   amdgpu-use-divergent-register-indexing
                            register-file accesses with a wave-uniform index use
                            GPR-index-mode moves instead of compare/select chains
+  structurizecfg-skip-uniform-regions
+                           the opcode switch is wave-uniform: keep it as plain
+                           scalar branches instead of structurized flow blocks
 
 usage: python -m galop_amd.build   (or galop_amd.build.build())
 """
@@ -33,6 +36,7 @@ DEVICE_FLAGS = [
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-mllvm", "-amdgpu-use-divergent-register-indexing=1",
+    "-mllvm", "-structurizecfg-skip-uniform-regions=true",
     "-Wall",
 ]
 
@@ -65,6 +69,19 @@ def build_library(verbose=False):
     return LIB
 
 
+def build_math_host(verbose=False):
+    """The interpreter's unary operations (csrc/lgp_math.h) compiled for the
+    host: tests check the device against it bit for bit."""
+    out = os.path.join(PKG, "libgalop_math_host.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+          "-Wall", "-o", tmp, os.path.join(CSRC, "math_host.cpp"), "-lm"])
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 def build_oracle(verbose=False):
     """The CPU checker (oracle/).  The reference builds under oracle/_ref are
     made only where the reference sources exist."""
@@ -95,6 +112,7 @@ def build_adapter_driver(verbose=False):
 def build(verbose=False):
     build_oracle(verbose)
     build_library(verbose)
+    build_math_host(verbose)
     build_adapter_driver(verbose)
 
 

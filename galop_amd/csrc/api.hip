@@ -260,7 +260,7 @@ static int strip_into_staging(galop_ctx* ctx, const galop_dataset* ds, const uin
                                          stream));
     GALOP_HIP(ctx, launch_effective_scatter(isa.num_registers, isa.op_set, isa.input_protect, d_words,
                                             d_offsets, n, st.d_eff_offsets.ptr, st.d_eff_words.ptr,
-                                            stream));
+                                            (int64_t)st.d_eff_words.cap, stream));
     *eff_words = st.d_eff_words.ptr;
     *eff_offsets = st.d_eff_offsets.ptr;
     return GALOP_OK;
@@ -418,8 +418,19 @@ int galop_strip_introns_device(galop_ctx* ctx, const galop_dataset* ds, const ui
     GALOP_HIP(ctx, launch_effective_counts(isa.num_registers, isa.op_set, isa.input_protect, d_words,
                                            d_offsets, n_programs, st.d_counts.ptr, s));
     GALOP_HIP(ctx, launch_exclusive_scan(st.d_counts.ptr, n_programs, d_eff_offsets, st.d_scan_tmp.ptr, s));
+    // the caller guarantees d_eff_words holds the raw word total
+    int64_t raw_total = 0;
+    if (n_programs > 0) {
+        int64_t ends[2];
+        GALOP_HIP(ctx, hipMemcpyAsync(&ends[0], d_offsets, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        GALOP_HIP(ctx, hipMemcpyAsync(&ends[1], d_offsets + n_programs, sizeof(int64_t),
+                                      hipMemcpyDeviceToHost, s));
+        GALOP_HIP(ctx, hipStreamSynchronize(s));
+        raw_total = ends[1] - ends[0];  // effective offsets start at 0
+    }
     GALOP_HIP(ctx, launch_effective_scatter(isa.num_registers, isa.op_set, isa.input_protect, d_words,
-                                            d_offsets, n_programs, d_eff_offsets, d_eff_words, s));
+                                            d_offsets, n_programs, d_eff_offsets, d_eff_words,
+                                            raw_total, s));
     GALOP_HIP(ctx, hipMemcpyAsync(eff_total, d_eff_offsets + n_programs, sizeof(int64_t),
                                   hipMemcpyDeviceToHost, s));
     GALOP_HIP(ctx, hipStreamSynchronize(s));

@@ -21,6 +21,11 @@ struct galop_batch {
     const galop_dataset* ds = nullptr;
     BatchBufs bufs;
     BatchTuning tuning;
+    // intron removal inside the generation
+    bool strip = false;
+    int64_t max_words = 0;
+    DeviceBuf<int64_t> counts, scan_tmp, eff_offsets;
+    DeviceBuf<uint32_t> eff_words;
     // graph cache
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -58,7 +63,7 @@ int ensure_batch_scratch(galop_ctx* ctx, BatchBufs& b, int64_t max_programs) {
 extern "C" {
 
 int galop_batch_create(galop_ctx* ctx, const galop_dataset* ds, int64_t max_programs,
-                       galop_batch** out) {
+                       int64_t max_words, int32_t flags, galop_batch** out) {
     if (!ctx || !ds || !out) return set_error(ctx, GALOP_ERR_INVALID, "galop_batch_create: NULL argument");
     *out = nullptr;
     if (ds->ctx != ctx)
@@ -66,12 +71,25 @@ int galop_batch_create(galop_ctx* ctx, const galop_dataset* ds, int64_t max_prog
     if (max_programs <= 0) return set_error(ctx, GALOP_ERR_INVALID, "galop_batch_create: max_programs <= 0");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    if ((flags & ~GALOP_BATCH_STRIP_INTRONS) != 0)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_batch_create: unknown flags");
+    if ((flags & GALOP_BATCH_STRIP_INTRONS) && max_words <= 0)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_batch_create: intron removal needs max_words > 0");
     auto* b = new galop_batch();
     b->ctx = ctx;
     b->ds = ds;
-    const int rc = ensure_batch_scratch(ctx, b->bufs, max_programs);
+    b->strip = (flags & GALOP_BATCH_STRIP_INTRONS) != 0;
+    b->max_words = max_words;
+    int rc = ensure_batch_scratch(ctx, b->bufs, max_programs);
+    if (rc == GALOP_OK && b->strip) {
+        hipError_t ae = b->counts.alloc((size_t)max_programs + 1);
+        if (ae == hipSuccess) ae = b->scan_tmp.alloc((size_t)scan_temp_elems(max_programs));
+        if (ae == hipSuccess) ae = b->eff_offsets.alloc((size_t)max_programs + 1);
+        if (ae == hipSuccess) ae = b->eff_words.alloc((size_t)max_words);
+        if (ae != hipSuccess) rc = hip_fail(ctx, ae, "galop_batch_create: intron buffers");
+    }
     if (rc != GALOP_OK) {
-        delete b;
+        galop_batch_destroy(b);
         return rc;
     }
     b->tuning = resolve_batch_tuning(ds->dev, ds->isa.num_registers, BatchTuning{}, ctx->n_cu);
@@ -84,7 +102,35 @@ void galop_batch_destroy(galop_batch* b) {
     hipSetDevice(b->ctx->device);
     b->drop_graph();
     b->bufs.release();
+    b->counts.release();
+    b->scan_tmp.release();
+    b->eff_offsets.release();
+    b->eff_words.release();
     delete b;
+}
+
+// One generation on stream s: optional intron removal, then the bucketed
+// persistent evaluation.  Pure stream work (kernels only), so it can be
+// captured.
+static hipError_t batch_generation(galop_batch* b, const uint32_t* d_words, const int64_t* d_offsets,
+                                   int64_t n, float* d_fitness, double* d_sse, hipStream_t s) {
+    const DevData& dd = b->ds->dev;
+    const galop_isa& isa = b->ds->isa;
+    if (b->strip) {
+        hipError_t e = launch_effective_counts(isa.num_registers, isa.op_set, isa.input_protect, d_words,
+                                               d_offsets, n, b->counts.ptr, s);
+        if (e == hipSuccess)
+            e = launch_exclusive_scan(b->counts.ptr, n, b->eff_offsets.ptr, b->scan_tmp.ptr, s);
+        if (e == hipSuccess)
+            e = launch_effective_scatter(isa.num_registers, isa.op_set, isa.input_protect, d_words,
+                                         d_offsets, n, b->eff_offsets.ptr, b->eff_words.ptr,
+                                         (int64_t)b->eff_words.cap, s);
+        if (e != hipSuccess) return e;
+        d_words = b->eff_words.ptr;
+        d_offsets = b->eff_offsets.ptr;
+    }
+    return launch_batched(dd, isa.num_registers, b->tuning, b->ctx->n_cu, b->bufs.view(), d_words,
+                          d_offsets, n, d_fitness, d_sse, s);
 }
 
 int galop_batch_eval_device(galop_batch* b, const uint32_t* d_words, const int64_t* d_offsets,
@@ -100,12 +146,9 @@ int galop_batch_eval_device(galop_batch* b, const uint32_t* d_words, const int64
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-    const DevData& dd = b->ds->dev;
-    const int nr = b->ds->isa.num_registers;
     if (!use_graph) {
-        e = launch_batched(dd, nr, b->tuning, ctx->n_cu, b->bufs.view(), d_words, d_offsets,
-                           n_programs, d_fitness, d_sse, s);
-        return e == hipSuccess ? GALOP_OK : hip_fail(ctx, e, "launch_batched");
+        e = batch_generation(b, d_words, d_offsets, n_programs, d_fitness, d_sse, s);
+        return e == hipSuccess ? GALOP_OK : hip_fail(ctx, e, "batched generation");
     }
     const bool hit = b->exec && b->key_words == d_words && b->key_offsets == d_offsets &&
                      b->key_fitness == d_fitness && b->key_sse == d_sse &&
@@ -114,8 +157,7 @@ int galop_batch_eval_device(galop_batch* b, const uint32_t* d_words, const int64
         b->drop_graph();
         e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamBeginCapture");
-        const hipError_t le = launch_batched(dd, nr, b->tuning, ctx->n_cu, b->bufs.view(), d_words,
-                                             d_offsets, n_programs, d_fitness, d_sse, s);
+        const hipError_t le = batch_generation(b, d_words, d_offsets, n_programs, d_fitness, d_sse, s);
         hipGraph_t g = nullptr;
         e = hipStreamEndCapture(s, &g);
         if (le != hipSuccess) {
@@ -159,7 +201,7 @@ int galop_batch_autotune(galop_batch* b, const uint32_t* d_words, const int64_t*
     // more cases per lane than the dataset has only adds padded work
     ks.erase(std::remove_if(ks.begin(), ks.end(), [&](int k) { return k > 1 && k / 2 >= per_lane; }),
              ks.end());
-    const int wpbs[] = {1, 2, 4, 8};
+    const int wpbs[] = {1, 2, 4};  // kernels are built for at most 256 threads per block
     const int workers[] = {8, 16, 24, 32};
     hipEvent_t a, z;
     if ((e = hipEventCreate(&a)) != hipSuccess) return hip_fail(ctx, e, "hipEventCreate");
@@ -178,15 +220,17 @@ int galop_batch_autotune(galop_batch* b, const uint32_t* d_words, const int64_t*
                 t.workers_per_cu = wk;
                 t.chunk = 1;
                 float ms = 0.0f;
+                const BatchTuning saved = b->tuning;
+                b->tuning = t;
                 for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
                     e = hipEventRecord(a, s);
                     if (e == hipSuccess)
-                        e = launch_batched(dd, nr, t, ctx->n_cu, b->bufs.view(), d_words, d_offsets,
-                                           sample, d_fitness, nullptr, s);
+                        e = batch_generation(b, d_words, d_offsets, sample, d_fitness, nullptr, s);
                     if (e == hipSuccess) e = hipEventRecord(z, s);
                     if (e == hipSuccess) e = hipEventSynchronize(z);
                     if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, z);
                 }
+                b->tuning = saved;
                 if (e != hipSuccess) break;
                 if (ms < best) {
                     best = ms;

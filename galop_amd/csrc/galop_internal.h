@@ -39,10 +39,11 @@ hipError_t launch_const_table(float* consts, hipStream_t stream);
 hipError_t launch_effective_counts(int num_registers, int op_set, int input_protect,
                                    const uint32_t* words, const int64_t* offsets,
                                    int64_t n, int64_t* counts, hipStream_t stream);
+// Writes at most `cap` words; offsets beyond `cap` are clamped to it.
 hipError_t launch_effective_scatter(int num_registers, int op_set, int input_protect,
                                     const uint32_t* words, const int64_t* offsets,
-                                    int64_t n, const int64_t* eff_offsets,
-                                    uint32_t* eff_words, hipStream_t stream);
+                                    int64_t n, int64_t* eff_offsets,
+                                    uint32_t* eff_words, int64_t cap, hipStream_t stream);
 // Exclusive scan of n int64 counts into out[0..n] (out[n] = total); tmp must
 // hold scan_temp_elems(n) int64.
 int64_t scan_temp_elems(int64_t n);

@@ -74,6 +74,8 @@ PlainTuning resolve_plain_tuning(const DevData& ds, int num_registers, PlainTuni
         while (k < kmax && k < per_lane) k *= 2;
         t.cases_per_lane = k;
     }
+    // 16 registers x 8 cases would exceed the 32-float register groups x 4
+    if (num_registers == 16 && t.cases_per_lane > 4) t.cases_per_lane = 4;
     if (t.waves_per_block <= 0 || t.waves_per_block > 4) t.waves_per_block = 4;
     return t;
 }
@@ -104,7 +106,7 @@ hipError_t launch_plain(const DevData& ds, int num_registers, PlainTuning t,
         return ds.op_set ? launch_plain_t<NRV, KV, 1>(ds, wpb, words, offsets, n, fitness, sse, stream) \
                          : launch_plain_t<NRV, KV, 0>(ds, wpb, words, offsets, n, fitness, sse, stream);
     GALOP_CASE(8, 1) GALOP_CASE(8, 2) GALOP_CASE(8, 4) GALOP_CASE(8, 8)
-    GALOP_CASE(16, 1) GALOP_CASE(16, 2) GALOP_CASE(16, 4) GALOP_CASE(16, 8)
+    GALOP_CASE(16, 1) GALOP_CASE(16, 2) GALOP_CASE(16, 4)
 #undef GALOP_CASE
     return hipErrorInvalidValue;
 }
@@ -173,7 +175,7 @@ __global__ void effective_count_kernel(DecodeCfg cfg, const uint32_t* __restrict
 __global__ void effective_scatter_kernel(DecodeCfg cfg, const uint32_t* __restrict__ words,
                                          const int64_t* __restrict__ offsets, int64_t n,
                                          const int64_t* __restrict__ eff_offsets,
-                                         uint32_t* __restrict__ eff_words) {
+                                         uint32_t* __restrict__ eff_words, int64_t cap) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t lo = offsets[i], hi = offsets[i + 1];
@@ -181,8 +183,18 @@ __global__ void effective_scatter_kernel(DecodeCfg cfg, const uint32_t* __restri
     uint32_t live = 1u << cfg.out_reg;
     for (int64_t j = hi - 1; j >= lo; --j) {
         const uint32_t w = words[j];
-        if (effective_step(w, cfg, live)) eff_words[--pos] = w;
+        if (effective_step(w, cfg, live)) {
+            --pos;
+            if (pos < cap) eff_words[pos] = w;  // never write past the buffer
+        }
     }
+}
+
+// Offsets past the effective-word buffer are cut to its end, so that a
+// population larger than the buffer can never make a kernel read past it.
+__global__ void clamp_offsets_kernel(int64_t* offsets, int64_t n1, int64_t cap) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n1 && offsets[i] > cap) offsets[i] = cap;
 }
 
 static DecodeCfg make_cfg(int nr, int op_set, int iprot) {
@@ -206,13 +218,15 @@ hipError_t launch_effective_counts(int num_registers, int op_set, int input_prot
 
 hipError_t launch_effective_scatter(int num_registers, int op_set, int input_protect,
                                     const uint32_t* words, const int64_t* offsets, int64_t n,
-                                    const int64_t* eff_offsets, uint32_t* eff_words,
+                                    int64_t* eff_offsets, uint32_t* eff_words, int64_t cap,
                                     hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     const unsigned grid = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(effective_scatter_kernel, dim3(grid), dim3(256), 0, stream,
                        make_cfg(num_registers, op_set, input_protect), words, offsets, n,
-                       eff_offsets, eff_words);
+                       eff_offsets, eff_words, cap);
+    hipLaunchKernelGGL(clamp_offsets_kernel, dim3((unsigned)((n + 256) / 256)), dim3(256), 0, stream,
+                       eff_offsets, n + 1, cap);
     return hipGetLastError();
 }
 
@@ -376,7 +390,7 @@ batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ o
     const int out_reg = ds.out_reg;
 
     // One-tile datasets stay in VGPRs for the life of the wave.
-    RegVec<NR> init[K];
+    RegFile<NR, K> init;
     float y[K];
     if (RESIDENT) {
         load_cases<NR, K>(ds, 0, lane, init);
@@ -394,14 +408,12 @@ batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ o
             const int64_t lo = offsets[p], hi = offsets[p + 1];
             double sse;
             if (RESIDENT) {
-                RegVec<NR> r[K];
-#pragma unroll
-                for (int k = 0; k < K; ++k) r[k] = init[k];
+                RegFile<NR, K> r = init;
                 run_program<NR, K, EXT>(words + lo, hi - lo, lane, r, cvec, dd);
                 LaneAcc acc;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
-                    if ((int64_t)(k * kWave + lane) < ds.n_cases) acc.add(r[k][out_reg], y[k]);
+                    if ((int64_t)(k * kWave + lane) < ds.n_cases) acc.add(r.get(k, out_reg), y[k]);
                 }
                 sse = wave_sum_f64(acc.total());
             } else {
@@ -422,7 +434,8 @@ BatchTuning resolve_batch_tuning(const DevData& ds, int num_registers, BatchTuni
         pt = resolve_plain_tuning(ds, num_registers, pt);
         t.cases_per_lane = pt.cases_per_lane;
     }
-    if (t.waves_per_block <= 0) t.waves_per_block = 4;
+    if (num_registers == 16 && t.cases_per_lane > 4) t.cases_per_lane = 4;
+    if (t.waves_per_block <= 0 || t.waves_per_block > 4) t.waves_per_block = 4;
     if (t.workers_per_cu <= 0) t.workers_per_cu = 32;
     if (t.chunk <= 0) t.chunk = 1;
     return t;
@@ -470,7 +483,7 @@ hipError_t launch_batched(const DevData& ds, int num_registers, BatchTuning t, i
         return ds.op_set ? launch_batched_t<NRV, KV, 1>(ds, t, n_cu, s, words, offsets, n, fitness, sse, stream) \
                          : launch_batched_t<NRV, KV, 0>(ds, t, n_cu, s, words, offsets, n, fitness, sse, stream);
     GALOP_CASE(8, 1) GALOP_CASE(8, 2) GALOP_CASE(8, 4) GALOP_CASE(8, 8)
-    GALOP_CASE(16, 1) GALOP_CASE(16, 2) GALOP_CASE(16, 4) GALOP_CASE(16, 8)
+    GALOP_CASE(16, 1) GALOP_CASE(16, 2) GALOP_CASE(16, 4)
 #undef GALOP_CASE
     return hipErrorInvalidValue;
 }

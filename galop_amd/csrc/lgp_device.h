@@ -29,6 +29,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "lgp_math.h"
+
 namespace galop {
 
 constexpr int kWave = 64;
@@ -56,15 +58,50 @@ constexpr int kUnaryTableBase = 16;
 __device__ __forceinline__ float op_div(float a, float b) {
     return (b != 0.0f) ? a / b : 1.0f;  // src/ISA.h:224 protected division
 }
-__device__ __forceinline__ float op_sin(float b) { return sinf(b); }
-__device__ __forceinline__ float op_cos(float b) { return cosf(b); }
-__device__ __forceinline__ float op_exp(float b) { return expf(b); }
-__device__ __forceinline__ float op_plog(float b) {
-    return (b != 0.0f) ? logf(fabsf(b)) : 0.0f;
-}
+// Unary ops on src2 (src/ISA.h:225-226 and the EXT8 table), evaluated in FP64
+// and rounded once (lgp_math.h).
+__device__ __forceinline__ float op_sin(float b) { return galop_math::sinf_r(b); }
+__device__ __forceinline__ float op_cos(float b) { return galop_math::cosf_r(b); }
+__device__ __forceinline__ float op_exp(float b) { return galop_math::expf_r(b); }
+__device__ __forceinline__ float op_plog(float b) { return galop_math::plogf_r(b); }
 
-template <int NR>
-using RegVec = float __attribute__((ext_vector_type(NR)));
+// The register files of the K cases of a lane, as one or two vector values
+// of at most 32 floats (register i of case k lives in group k / kPer
+// at (k % kPer) * NR + i).  They are never arrays in memory, so they stay in
+// VGPRs, and a wave-uniform register index becomes one indexed register move
+// per case with the constant part folded into the base register.  Wider
+// vectors are not used: dynamic indexing of vectors above 32 elements is
+// lowered through scratch memory.
+template <int NR, int K>
+struct RegGeom {
+    static constexpr int kPer = (K * NR < 32 ? K * NR : 32) / NR;  // cases per group
+    static constexpr int kGroups = (K + kPer - 1) / kPer;
+};
+
+template <int NR, int K, int G = RegGeom<NR, K>::kGroups>
+struct RegFile;
+
+template <int NR, int K>
+struct RegFile<NR, K, 1> {
+    typedef float Vec __attribute__((ext_vector_type(K * NR)));
+    Vec g0;
+    __device__ __forceinline__ float get(int k, int i) const { return g0[k * NR + i]; }
+    __device__ __forceinline__ void set(int k, int i, float x) { g0[k * NR + i] = x; }
+};
+
+template <int NR, int K>
+struct RegFile<NR, K, 2> {
+    static constexpr int kPer = RegGeom<NR, K>::kPer;
+    typedef float Vec __attribute__((ext_vector_type(kPer * NR)));
+    Vec g0, g1;
+    __device__ __forceinline__ float get(int k, int i) const {
+        return k < kPer ? g0[k * NR + i] : g1[(k - kPer) * NR + i];
+    }
+    __device__ __forceinline__ void set(int k, int i, float x) {
+        if (k < kPer) g0[k * NR + i] = x;
+        else g1[(k - kPer) * NR + i] = x;
+    }
+};
 
 // Broadcast word j of a 64-word chunk held one word per lane.
 __device__ __forceinline__ uint32_t wave_word(uint32_t chunk, int j) {
@@ -90,7 +127,7 @@ __device__ __forceinline__ DestDecode dest_decode(int nr, int iprot) {
 }
 
 template <int NR, int K, int EXT>
-__device__ __forceinline__ void exec_word(uint32_t w, RegVec<NR> (&r)[K],
+__device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                                           float cvec, DestDecode dd) {
     const int op = (int)(w & 7u);
     const int d = (int)((w >> 8) & dd.dmask) + dd.dbase;
@@ -100,31 +137,33 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegVec<NR> (&r)[K],
 
 #define GALOP_BIN(EXPR)                                                   \
     do {                                                                  \
+        float av[K], bv[K];                                               \
+        _Pragma("unroll") for (int k = 0; k < K; ++k) av[k] = r.get(k, s1);   \
         if (kc) {                                                         \
-            const float b = wave_bcast(cvec, s2);                         \
-            _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
-                const float a = r[k][s1];                                 \
-                r[k][d] = (EXPR);                                         \
-            }                                                             \
+            const float bc = wave_bcast(cvec, s2);                        \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) bv[k] = bc;     \
         } else {                                                          \
-            _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
-                const float a = r[k][s1];                                 \
-                const float b = r[k][s2];                                 \
-                r[k][d] = (EXPR);                                         \
-            }                                                             \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) bv[k] = r.get(k, s2); \
         }                                                                 \
+        float cv[K];                                                      \
+        _Pragma("unroll") for (int k = 0; k < K; ++k) {                   \
+            const float a = av[k];                                        \
+            const float b = bv[k];                                        \
+            cv[k] = (EXPR);                                               \
+        }                                                                 \
+        _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]);    \
     } while (0)
 
 #define GALOP_UNARY(U, FN)                                                \
     do {                                                                  \
         if (kc) {                                                         \
             const float v = wave_bcast(cvec, kUnaryTableBase + (U) * 16 + s2); \
-            _Pragma("unroll") for (int k = 0; k < K; ++k) r[k][d] = v;    \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, v);    \
         } else {                                                          \
-            _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
-                const float b = r[k][s2];                                 \
-                r[k][d] = FN(b);                                          \
-            }                                                             \
+            float bv[K], cv[K];                                           \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) bv[k] = r.get(k, s2); \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) cv[k] = FN(bv[k]); \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]); \
         }                                                                 \
     } while (0)
 
@@ -153,7 +192,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegVec<NR> (&r)[K],
 template <int NR, int K, int EXT>
 __device__ __forceinline__ void run_program(const uint32_t* __restrict__ prog,
                                             int64_t len, int lane,
-                                            RegVec<NR> (&r)[K], float cvec,
+                                            RegFile<NR, K>& r, float cvec,
                                             DestDecode dd) {
     uint32_t next = (lane < len) ? prog[lane] : 0u;
     for (int64_t base = 0; base < len; base += kWave) {
@@ -170,13 +209,13 @@ __device__ __forceinline__ void run_program(const uint32_t* __restrict__ prog,
 // (src/Interpreter.cpp:43-52).
 template <int NR, int K>
 __device__ __forceinline__ void load_cases(const DevData& ds, int64_t tile,
-                                           int lane, RegVec<NR> (&r)[K]) {
+                                           int lane, RegFile<NR, K>& r) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int64_t c = tile * (int64_t)(kWave * K) + k * kWave + lane;
 #pragma unroll
         for (int i = 0; i < NR; ++i)
-            r[k][i] = (i < ds.n_inputs) ? ds.inputs[(int64_t)i * ds.n_pad + c] : 0.0f;
+            r.set(k, i, (i < ds.n_inputs) ? ds.inputs[(int64_t)i * ds.n_pad + c] : 0.0f);
     }
 }
 
@@ -240,14 +279,14 @@ __device__ __forceinline__ double eval_program_wave(const uint32_t* __restrict__
     const int64_t n_tiles = (ds.n_cases + tile_cases - 1) / tile_cases;
     LaneAcc acc;
     for (int64_t t = 0; t < n_tiles; ++t) {
-        RegVec<NR> r[K];
+        RegFile<NR, K> r;
         load_cases<NR, K>(ds, t, lane, r);
         run_program<NR, K, EXT>(prog, len, lane, r, cvec, dd);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int64_t c = t * tile_cases + k * kWave + lane;
             if (c < ds.n_cases) {
-                const float out = r[k][out_reg];
+                const float out = r.get(k, out_reg);
                 if (case_out) case_out[c] = out;
                 else acc.add(out, ds.targets[c]);
             }

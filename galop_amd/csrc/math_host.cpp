@@ -1,0 +1,21 @@
+// math_host.cpp -- the interpreter's unary operations (lgp_math.h) compiled
+// for the host, so tests can check that the device computes the same bits and
+// measure their accuracy against the host C library and exact values.
+#include "lgp_math.h"
+
+extern "C" {
+float gm_sinf(float x) { return galop_math::sinf_r(x); }
+float gm_cosf(float x) { return galop_math::cosf_r(x); }
+float gm_expf(float x) { return galop_math::expf_r(x); }
+float gm_plogf(float x) { return galop_math::plogf_r(x); }
+void gm_map(int op, const float* x, float* y, long n) {
+    for (long i = 0; i < n; ++i) {
+        switch (op) {
+        case 4: y[i] = galop_math::sinf_r(x[i]); break;
+        case 5: y[i] = galop_math::cosf_r(x[i]); break;
+        case 6: y[i] = galop_math::expf_r(x[i]); break;
+        default: y[i] = galop_math::plogf_r(x[i]); break;
+        }
+    }
+}
+}

@@ -233,12 +233,13 @@ def strip_introns_device(ds, d_words, d_offsets, n, d_eff_words, d_eff_offsets, 
 class BatchPlan:
     """Batched persistent evaluation plan (bucketing + work queue + hipGraph)."""
 
-    def __init__(self, ds, max_programs):
+    def __init__(self, ds, max_programs, max_words=0, strip_introns=False):
         self.ds = ds
         self.lib = ds.ctx.lib
         h = ctypes.c_void_p()
+        flags = _lib.BATCH_STRIP_INTRONS if strip_introns else 0
         check(self.lib.galop_batch_create(ds.ctx.handle, ds.handle, int(max_programs),
-                                          ctypes.byref(h)), ds.ctx.handle)
+                                          int(max_words), flags, ctypes.byref(h)), ds.ctx.handle)
         self.handle = h
 
     def eval(self, d_words, d_offsets, n, d_fitness, d_sse=None, use_graph=True, stream=None):
@@ -295,3 +296,66 @@ class HipEvaluator:
         stride = instr.shape[1] if instr.ndim == 2 else instr.size // max(len(lengths), 1)
         instr = instr.reshape(-1, stride)[:n_programs]
         fitness_out[:n_programs] = eval_dense(dds, instr, lengths, self.layout)
+
+
+class Server:
+    """Continuous-batching evaluation server (galop_server_* in the C ABI).
+
+    submit() returns at once; the population and output arrays passed in must
+    stay alive until wait() reports the request done."""
+
+    def __init__(self, ctx, hbm_budget_bytes, n_streams=4):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = ctypes.c_void_p()
+        check(self.lib.galop_server_create(ctx.handle, int(hbm_budget_bytes), int(n_streams),
+                                           ctypes.byref(h)), ctx.handle)
+        self.handle = h
+        self._keep = {}
+
+    def submit(self, dds, words, offsets, fitness_out, sse_out=None):
+        words = np.ascontiguousarray(words, np.uint32)
+        offsets = np.ascontiguousarray(offsets, np.int64)
+        rid = ctypes.c_int64()
+        check(self.lib.galop_submit(self.handle, dds.handle, _ptr(words), _ptr(offsets),
+                                    offsets.size - 1, _ptr(fitness_out), _ptr(sse_out),
+                                    ctypes.byref(rid)), self.ctx.handle)
+        self._keep[rid.value] = (words, offsets, fitness_out, sse_out)
+        return rid.value
+
+    def wait(self, rid, timeout_s=-1.0):
+        rc = self.lib.galop_wait(self.handle, int(rid), float(timeout_s))
+        if rc == _lib.GALOP_OK:
+            self._keep.pop(rid, None)
+        return rc
+
+    def poll(self, rid):
+        st = ctypes.c_int32(0)
+        pending = self.lib.galop_poll(self.handle, int(rid), ctypes.byref(st))
+        return None if pending == 1 else st.value
+
+    def times(self, rid):
+        lat = ctypes.c_double()
+        dev = ctypes.c_double()
+        check(self.lib.galop_request_times(self.handle, int(rid), ctypes.byref(lat),
+                                           ctypes.byref(dev)), self.ctx.handle)
+        return lat.value, dev.value
+
+    def stats(self):
+        a, c, p, q = (ctypes.c_int64() for _ in range(4))
+        check(self.lib.galop_server_stats(self.handle, ctypes.byref(a), ctypes.byref(c),
+                                          ctypes.byref(p), ctypes.byref(q)), self.ctx.handle)
+        return {"admitted": a.value, "completed": c.value, "peak_bytes": p.value,
+                "queued": q.value}
+
+    def close(self):
+        if self.handle:
+            self.lib.galop_server_destroy(self.handle)
+            self.handle = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

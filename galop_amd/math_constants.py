@@ -1,0 +1,131 @@
+"""Derivation of the constants in galop_amd/csrc/lgp_math.h.
+
+Everything is computed here from first principles with exact integer and
+rational arithmetic -- pi by Machin's formula, ln 2 by its binary series,
+Taylor coefficients as exact rationals -- and rounded to IEEE double by
+Python's correctly rounded int/Fraction -> float conversion.  Running this
+module prints the C literals; tests/test_math_constants.py re-derives them
+and checks that the header holds exactly these values.
+"""
+from fractions import Fraction
+
+PREC = 1400  # bits of working precision
+
+
+def _atan_inv(n, prec):
+    """floor(atan(1/n) * 2^prec) via the alternating series, integer only."""
+    one = 1 << prec
+    total = 0
+    term = one // n
+    k = 1
+    sign = 1
+    n2 = n * n
+    while term:
+        total += sign * (term // k)
+        term //= n2
+        k += 2
+        sign = -sign
+    return total
+
+
+def pi_fixed(prec=PREC):
+    """pi * 2^prec (Machin: pi = 16 atan(1/5) - 4 atan(1/239)), +-few ulps."""
+    guard = 32
+    p = prec + guard
+    return (16 * _atan_inv(5, p) - 4 * _atan_inv(239, p)) >> guard
+
+
+def ln2_fixed(prec=PREC):
+    """ln 2 * 2^prec from ln 2 = sum_{k>=1} 1 / (k 2^k)."""
+    guard = 32
+    p = prec + guard
+    total = 0
+    k = 1
+    while True:
+        term = (1 << p) // (k << k)
+        if term == 0:
+            break
+        total += term
+        k += 1
+    return total >> guard
+
+
+def two_over_pi_words(n_words=12):
+    """The first n_words 32-bit words of the binary expansion of 2/pi
+    (word 0 holds bits 2^-1 .. 2^-32)."""
+    pi = pi_fixed()
+    frac = (2 << (2 * PREC)) // pi  # 2/pi * 2^PREC
+    out = []
+    for j in range(n_words):
+        shift = PREC - 32 * (j + 1)
+        out.append((frac >> shift) & 0xFFFFFFFF)
+    return out
+
+
+def split(value_fixed, prec, bits):
+    """Split value = value_fixed/2^prec into doubles c1 + c2 + c3 where c1 and
+    c2 carry `bits` significant bits each (so small-integer multiples of them
+    are exact) and c3 is the rounded remainder."""
+    v = Fraction(value_fixed, 1 << prec)
+    parts = []
+    for _ in range(2):
+        # round v to `bits` significant bits
+        e = v.numerator.bit_length() - v.denominator.bit_length()
+        scale = Fraction(2) ** (bits - 1 - e)
+        q = round(v * scale)
+        c = Fraction(q) / scale
+        parts.append(float(c))
+        assert Fraction(parts[-1]) == c
+        v -= c
+    parts.append(float(v))
+    return parts
+
+
+def taylor(kind, n_terms):
+    """Coefficients (as doubles) of the series used by lgp_math.h."""
+    from math import factorial
+    if kind == "exp":
+        return [float(Fraction(1, factorial(k))) for k in range(n_terms)]
+    if kind == "sin":
+        return [float(Fraction((-1) ** k, factorial(2 * k + 1))) for k in range(n_terms)]
+    if kind == "cos":
+        return [float(Fraction((-1) ** k, factorial(2 * k))) for k in range(n_terms)]
+    if kind == "atanh":  # log m = 2 atanh(s): 2 * s^(2k+1)/(2k+1)
+        return [float(Fraction(2, 2 * k + 1)) for k in range(n_terms)]
+    raise ValueError(kind)
+
+
+def constants():
+    pi = pi_fixed()
+    ln2 = ln2_fixed()
+    half_pi = pi  # pi/2 = pi * 2^-1
+    c = {
+        "PIO2": split(half_pi, PREC + 1, 28),
+        "TWO_OVER_PI": float(Fraction(2 << PREC, pi)),
+        "PIO2_SCALED62": float(Fraction(pi, 1 << (PREC + 1)) / (1 << 62)),
+        "LN2": split(ln2, PREC, 32),
+        "INV_LN2": float(Fraction(1 << PREC, ln2)),
+        "TWO_OVER_PI_WORDS": two_over_pi_words(12),
+        "EXP_POLY": taylor("exp", 13),
+        "SIN_POLY": taylor("sin", 8),
+        "COS_POLY": taylor("cos", 9),
+        "LOG_POLY": taylor("atanh", 9),
+    }
+    return c
+
+
+def c_literals():
+    c = constants()
+    lines = []
+    for k, v in c.items():
+        if k == "TWO_OVER_PI_WORDS":
+            lines.append(f"{k} = {{" + ", ".join(f"0x{w:08X}u" for w in v) + "}")
+        elif isinstance(v, list):
+            lines.append(f"{k} = {{" + ", ".join(f"{x!r}" if False else float.hex(x) for x in v) + "}")
+        else:
+            lines.append(f"{k} = {float.hex(v)}")
+    return "\n".join(lines)
+
+
+if __name__ == "__main__":
+    print(c_literals())

@@ -137,11 +137,18 @@ int galop_strip_introns_device(galop_ctx* ctx, const galop_dataset* ds,
 
 /* ---- batched persistent path ----------------------------------------------
  * A batch plan owns the work-queue and bucketing scratch for populations of
- * up to max_programs programs on one dataset, and a hipGraph of the
- * generation (bucket -> order -> persistent evaluation) that is re-launched
- * while the program buffers stay the same. */
+ * up to max_programs programs (and max_words instruction words) on one
+ * dataset, and a hipGraph of one generation that is re-launched while the
+ * program buffers stay the same:
+ *   [intron removal: count -> scan -> scatter]   (flags & GALOP_BATCH_STRIP_INTRONS)
+ *   length histogram -> bucket starts -> longest-first order -> persistent
+ *   work-queue evaluation.
+ * Intron removal is exact (src/Introns.h:28-57), so fitness is bitwise that
+ * of the raw programs. */
+#define GALOP_BATCH_STRIP_INTRONS 1
 int  galop_batch_create(galop_ctx* ctx, const galop_dataset* ds,
-                        int64_t max_programs, galop_batch** out);
+                        int64_t max_programs, int64_t max_words, int32_t flags,
+                        galop_batch** out);
 void galop_batch_destroy(galop_batch* b);
 /* Evaluate n <= max_programs CSR programs (device pointers).  use_graph=1
  * captures the sequence on first use and replays it afterwards. */

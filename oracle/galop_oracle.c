@@ -43,6 +43,22 @@ int go_get_src2(const go_isa* isa, uint32_t w) {
 
 int go_src2_is_const(uint32_t w) { return ((w >> 24) & 0x80u) != 0; }
 
+/* ---- unary math: the host C library by default (the reference's
+ * std::sin / std::cos, src/ISA.h:225-226).  A test may substitute other
+ * implementations to check an interpreter bit for bit against a device that
+ * uses them; go_set_unary_math(NULL x4) restores the C library. */
+typedef float (*go_unary_fn)(float);
+static float plog_libm(float b) { return (b != 0.0f) ? logf(fabsf(b)) : 0.0f; }
+static go_unary_fn g_sin = sinf, g_cos = cosf, g_exp = expf, g_plog = plog_libm;
+
+void go_set_unary_math(go_unary_fn sin_fn, go_unary_fn cos_fn, go_unary_fn exp_fn,
+                       go_unary_fn plog_fn) {
+    g_sin = sin_fn ? sin_fn : sinf;
+    g_cos = cos_fn ? cos_fn : cosf;
+    g_exp = exp_fn ? exp_fn : expf;
+    g_plog = plog_fn ? plog_fn : plog_libm;
+}
+
 /* ---- apply_op: src/ISA.h:219-231 (REF8) plus the EXT8 table ------------ */
 float go_apply_op(int op_set, int op, float a, float b) {
     switch (op) {
@@ -50,13 +66,13 @@ float go_apply_op(int op_set, int op, float a, float b) {
     case 1: return a - b;
     case 2: return a * b;
     case 3: return (b != 0.0f) ? a / b : 1.0f;
-    case 4: return sinf(b);
-    case 5: return cosf(b);
+    case 4: return g_sin(b);
+    case 5: return g_cos(b);
     case 6:
-        if (op_set == GO_OPS_EXT8) return expf(b);
+        if (op_set == GO_OPS_EXT8) return g_exp(b);
         return (a < b) ? 1.0f : 0.0f;
     default:
-        if (op_set == GO_OPS_EXT8) return (b != 0.0f) ? logf(fabsf(b)) : 0.0f;
+        if (op_set == GO_OPS_EXT8) return g_plog(b);
         return (a > b) ? 1.0f : 0.0f;
     }
 }
@@ -259,4 +275,59 @@ void go_dual_precision_fitness(const go_isa* isa, const uint32_t* prog,
     const float mse_d = (float)(sse_d / (double)n_cases);
     *fit_f = go_mse_to_fitness(mse_f);
     *fit_d = go_mse_to_fitness(mse_d);
+}
+
+/* ---- libm-sensitivity probe -------------------------------------------
+ * Fitness of one program when every result of a unary transcendental op
+ * (sin, cos and, under EXT8, exp and log) is moved by a pseudo-random number
+ * of ulps in [-ulps, ulps] (deterministic in seed, case and position).  A
+ * program whose fitness moves by more than a tolerance under such a
+ * perturbation cannot have that tolerance against two libraries that differ
+ * by `ulps` per operation; tests use it to attribute such differences. */
+static uint32_t mix32(uint64_t z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return (uint32_t)((z ^ (z >> 31)) >> 32);
+}
+
+static float nudge(float v, int delta) {
+    if (!isfinite(v) || v == 0.0f || delta == 0) return v;
+    union { float f; int32_t i; } u;
+    u.f = v;
+    /* move along the ordered float line, staying on the same side of 0 */
+    int32_t m = u.i & 0x7fffffff;
+    int64_t moved = (int64_t)m + (u.i < 0 ? -delta : delta);
+    if (moved < 1) moved = 1;
+    if (moved > 0x7f7fffff) moved = 0x7f7fffff;
+    u.i = (int32_t)((u.i & (int32_t)0x80000000) | (int32_t)moved);
+    return u.f;
+}
+
+float go_perturbed_fitness(const go_isa* isa, const uint32_t* prog, int64_t len,
+                           const float* inputs, const float* targets, int32_t n_inputs,
+                           int64_t n_cases, int32_t ulps, uint64_t seed) {
+    const int nr = isa->num_registers;
+    const int o = go_output_reg(isa);
+    double sse = 0.0;
+    for (int64_t c = 0; c < n_cases; ++c) {
+        float r[GO_MAXR];
+        for (int i = 0; i < nr; ++i) r[i] = (i < n_inputs) ? inputs[c * n_inputs + i] : 0.0f;
+        for (int64_t pc = 0; pc < len; ++pc) {
+            const uint32_t w = prog[pc];
+            const int op = go_get_op(w), d = go_get_dest(isa, w);
+            const float a = r[go_get_src1(isa, w)];
+            const float b = go_src2_is_const(w) ? isa->constants[go_get_src2(isa, w)]
+                                                : r[go_get_src2(isa, w)];
+            float v = go_apply_op(isa->op_set, op, a, b);
+            if (op_is_unary(isa->op_set, op)) {
+                const uint32_t h = mix32(seed * 0x9E3779B97F4A7C15ULL + (uint64_t)c * 1000003ULL +
+                                         (uint64_t)pc);
+                v = nudge(v, (int)(h % (uint32_t)(2 * ulps + 1)) - ulps);
+            }
+            r[d] = v;
+        }
+        const double err = (double)r[o] - (double)targets[c];
+        sse += err * err;
+    }
+    return go_sse_to_fitness(sse, n_cases);
 }

@@ -55,6 +55,11 @@ int go_output_reg(const go_isa* isa);
 
 /* ---- one operation (src/ISA.h:219-231 + EXT8) ---- */
 float go_apply_op(int op_set, int op, float a, float b);
+/* Replace the unary math (sin, cos, exp, protected log) used by every
+ * evaluation; NULL restores the host C library (the default).  Not
+ * thread-safe: set it before evaluating. */
+void go_set_unary_math(float (*sin_fn)(float), float (*cos_fn)(float),
+                       float (*exp_fn)(float), float (*plog_fn)(float));
 
 /* Per-case outputs r[OUTPUT_REG] of one program over n_cases cases.
  * inputs are case-major AoS: inputs[c*n_inputs + i]. */
@@ -97,6 +102,12 @@ void go_dual_precision_fitness(const go_isa* isa, const uint32_t* prog,
                                int64_t len, const float* inputs,
                                const float* targets, int32_t n_inputs,
                                int64_t n_cases, float* fit_f, float* fit_d);
+
+/* Fitness with every unary transcendental result moved by a pseudo-random
+ * number of ulps in [-ulps, ulps] (libm-sensitivity probe). */
+float go_perturbed_fitness(const go_isa* isa, const uint32_t* prog, int64_t len,
+                           const float* inputs, const float* targets, int32_t n_inputs,
+                           int64_t n_cases, int32_t ulps, uint64_t seed);
 
 #ifdef __cplusplus
 }

@@ -62,6 +62,10 @@ def lib():
             getattr(L, name).argtypes = [pisa, ctypes.c_uint32]
         L.go_output_reg.restype = ctypes.c_int
         L.go_output_reg.argtypes = [pisa]
+        L.go_perturbed_fitness.restype = ctypes.c_float
+        L.go_perturbed_fitness.argtypes = [pisa, vp, i64, vp, vp, i32, i64, i32, ctypes.c_uint64]
+        L.go_set_unary_math.restype = None
+        L.go_set_unary_math.argtypes = [vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -171,3 +175,53 @@ def dual_precision_fitness(isa, prog, inputs, targets):
     lib().go_dual_precision_fitness(ctypes.byref(isa), _p(prog), prog.size, _p(x), _p(y),
                                     x.shape[1], y.size, ctypes.byref(ff), ctypes.byref(fd))
     return ff.value, fd.value
+
+
+def libm_sensitivity(isa, prog, inputs, targets, ulps=1, seeds=8):
+    """Largest fitness change over `seeds` random +-ulps perturbations of the
+    program's transcendental results."""
+    prog = np.ascontiguousarray(prog, np.uint32)
+    x = np.ascontiguousarray(inputs, np.float32)
+    if x.ndim == 1:
+        x = x[:, None]
+    y = np.ascontiguousarray(targets, np.float32)
+    base = lib().go_perturbed_fitness(ctypes.byref(isa), _p(prog), prog.size, _p(x), _p(y),
+                                      x.shape[1], y.size, 0, 0)
+    worst = 0.0
+    for s in range(1, seeds + 1):
+        f = lib().go_perturbed_fitness(ctypes.byref(isa), _p(prog), prog.size, _p(x), _p(y),
+                                       x.shape[1], y.size, int(ulps), s)
+        worst = max(worst, abs(f - base)) if np.isfinite(f) and np.isfinite(base) else float("inf")
+    return base, worst
+
+
+def attributed(isa, prog, inputs, targets, tol_rel):
+    """The reference's chaos rule (float vs double registers,
+    tests/cuda_diff.cu:162-181) or libm sensitivity at 1 ulp: True when the
+    program's fitness is not defined to tol_rel by its arithmetic."""
+    ff, fd = dual_precision_fitness(isa, prog, inputs, targets)
+    scale = max(abs(ff), 1e-30)
+    if abs(fd - ff) > tol_rel * scale:
+        return True
+    _, worst = libm_sensitivity(isa, prog, inputs, targets)
+    return worst > tol_rel * scale
+
+
+class unary_math:
+    """Context manager: evaluate with the unary ops of a host library that
+    exports gm_sinf/gm_cosf/gm_expf/gm_plogf (galop_amd/libgalop_math_host.so,
+    the device's functions compiled for the host)."""
+
+    def __init__(self, path):
+        self.path = path
+
+    def __enter__(self):
+        m = ctypes.CDLL(self.path)
+        self._keep = m
+        fns = [ctypes.cast(getattr(m, n), ctypes.c_void_p) for n in
+               ("gm_sinf", "gm_cosf", "gm_expf", "gm_plogf")]
+        lib().go_set_unary_math(*fns)
+        return self
+
+    def __exit__(self, *a):
+        lib().go_set_unary_math(None, None, None, None)

@@ -6,11 +6,23 @@ import torch
 from galop_amd import evaluator as E
 
 
+_STREAM = None
+
+
 def stream():
-    return torch.cuda.current_stream().cuda_stream
+    """A dedicated (non-default) torch stream, made current, so that torch
+    work, our launches and events share one ordered stream.  (A NULL stream
+    handle means 'the context stream' to the C ABI, and torch's default
+    stream handle is NULL.)"""
+    global _STREAM
+    if _STREAM is None:
+        _STREAM = torch.cuda.Stream()
+        torch.cuda.set_stream(_STREAM)
+    return _STREAM.cuda_stream
 
 
 def dev_words(words):
+    stream()
     return torch.from_numpy(np.ascontiguousarray(words, np.uint32).view(np.int32)).cuda()
 
 
@@ -58,3 +70,9 @@ def rel_err(a, b):
     den = np.maximum(np.abs(b), 1e-300)
     r = np.abs(a - b) / den
     return np.where(a == b, 0.0, r)
+
+
+def math_host_path():
+    import os
+    here = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    return os.path.join(here, "galop_amd", "libgalop_math_host.so")

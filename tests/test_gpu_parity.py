@@ -1,21 +1,30 @@
 """HIP evaluator vs the CPU oracle, through the C ABI, on the GPU.
 
-Bars:
-* per-case outputs r[OUTPUT_REG]: bit-exact when the effective code uses only
-  +, -, *, protected /, <, > (NaN payloads aside); every unary op within
-  2 ulp of the oracle (host libm) and of the correctly rounded value;
-* fitness: within 1e-6 relative of the oracle's FP64-SSE fitness; a program
-  that misses it must be precision-chaotic (float vs double registers move
-  its fitness by more than the tolerance, tests/cuda_diff.cu:92-181), and
-  such programs must be at most 1% of the population;
-* the oracle's SSE (FP64, case order) within 1e-6 relative where finite.
+Three bars:
+1. The interpreter is exact: per-case outputs r[OUTPUT_REG] equal, bit for
+   bit, those of the oracle running with the device's own unary functions
+   (galop_amd/libgalop_math_host.so, the same source compiled for the host) --
+   for every program, every op.  With the host C library, the outputs of
+   programs whose effective code avoids sin/cos/exp/log equal the
+   reference's own vectors (tests/golden/percase_*.npz).
+2. The unary functions are faithful: each is within 1 ulp of the host C
+   library the reference uses (src/ISA.h:225-226) and of the correctly
+   rounded value, on special and random arguments.
+3. Fitness and SSE are within 1e-6 relative of the oracle.  Against the
+   device-math oracle this holds for every program (only the summation
+   differs); against the host-library oracle a program may miss it only if
+   its fitness is not defined to 1e-6 by its arithmetic -- precision-chaotic
+   under float vs double registers (the reference's rule,
+   tests/cuda_diff.cu:162-181) or moved beyond 1e-6 by a 1-ulp perturbation
+   of its transcendental results -- and such programs stay at most 1% of the
+   population.
 """
 import numpy as np
 import pytest
 
 import oracle_py as O
 from conftest import load_fixture, same_bits, ulp_distance
-from gpu_util import gpu_cases, gpu_fitness, random_population, rel_err
+from gpu_util import gpu_cases, gpu_fitness, math_host_path, random_population, rel_err
 
 pytestmark = pytest.mark.gpu
 
@@ -46,7 +55,8 @@ def _transcendental_free(oisa, words, offsets, op_set):
 
 @pytest.mark.parametrize("name", ["percase_r8.npz", "percase_r16.npz", "percase_r16ip.npz"])
 def test_percase_fixture_outputs(gpu_ctx, name):
-    """Per-case outputs vs the reference's Interpreter::run_stateless vectors."""
+    """Per-case outputs vs the reference's Interpreter::run_stateless vectors
+    and vs the oracle with the device's unary functions."""
     f = load_fixture(name)
     isa = _isa(int(f["num_registers"]), 0, bool(f["input_protect"]), list(f["constants"]))
     dds = _dds(gpu_ctx, isa, f["inputs"], f["targets"])
@@ -57,12 +67,12 @@ def test_percase_fixture_outputs(gpu_ctx, name):
     assert free.sum() > 20
     bad = ~ok[free]
     assert not bad.any(), f"{bad.sum()} outputs of transcendental-free programs differ"
-    # programs with sin/cos: device and host libm may differ in the last bit
-    # (the per-op bound is test_unary_op_ulp) and chained ops amplify that, so
-    # only a gross-error floor is asserted here; the fraction is reported
     frac = ok[~free].mean() if (~free).any() else 1.0
-    print(f"{name}: sin/cos programs bit-identical per case: {frac:.4f}")
-    assert frac > 0.5
+    print(f"{name}: sin/cos programs bit-identical to the reference per case: {frac:.4f}")
+    with O.unary_math(math_host_path()):
+        exact = O.run_csr(oisa, f["words"], f["offsets"], f["inputs"])
+    same = same_bits(out, exact)
+    assert same.all(), f"{(~same).sum()} per-case outputs differ from the device-math oracle"
 
 
 def _special_inputs(rng, n):
@@ -79,8 +89,9 @@ def _special_inputs(rng, n):
 
 @pytest.mark.parametrize("op_set,op", [(0, 4), (0, 5), (1, 6), (1, 7)])
 def test_unary_op_ulp(gpu_ctx, op_set, op):
-    """r0 = op(r0): within 2 ulp of the host libm and of the correctly
-    rounded value (double-precision libm rounded to float)."""
+    """r0 = op(r0): bitwise the host build of the device function; within
+    1 ulp of the host C library and of the correctly rounded value
+    (double-precision libm rounded to float)."""
     rng = np.random.default_rng(10 + op)
     x = _special_inputs(rng, 1 << 16)
     isa = _isa(8, op_set)
@@ -88,7 +99,11 @@ def test_unary_op_ulp(gpu_ctx, op_set, op):
     word = np.array([op], np.uint32)  # dest 0, src1 0, src2 r0 (register mode)
     offsets = np.array([0, 1], np.int64)
     out = gpu_cases(dds, word, offsets)[0]
-    ref = O.run_program(O.isa_from(isa), word, x)
+    oisa = O.isa_from(isa)
+    ref = O.run_program(oisa, word, x)
+    with O.unary_math(math_host_path()):
+        host = O.run_program(oisa, word, x)
+    assert same_bits(out, host).all(), "device differs from the host build of its function"
     d_host = ulp_distance(out, ref)
     with np.errstate(all="ignore"):
         xd = x.astype(np.float64)
@@ -97,8 +112,10 @@ def test_unary_op_ulp(gpu_ctx, op_set, op):
         exact = fn(xd).astype(np.float32)
     d_exact = ulp_distance(out, exact)
     assert same_bits(np.isnan(out), np.isnan(ref)).all()
-    assert d_host.max() <= 2, f"max {d_host.max()} ulp vs host libm at x={x[d_host.argmax()]!r}"
-    assert d_exact.max() <= 2, f"max {d_exact.max()} ulp vs correctly rounded at x={x[d_exact.argmax()]!r}"
+    assert d_host.max() <= 1, f"max {d_host.max()} ulp vs host libm at x={x[d_host.argmax()]!r}"
+    assert d_exact.max() <= 1, f"max {d_exact.max()} ulp vs correctly rounded at x={x[d_exact.argmax()]!r}"
+    print(f"op {op}: identical to host libm on {(d_host == 0).mean():.4%}, "
+          f"to correctly rounded on {(d_exact == 0).mean():.4%}")
 
 
 @pytest.mark.parametrize("op_set,op", [(0, 0), (0, 1), (0, 2), (0, 3), (0, 6), (0, 7)])
@@ -141,21 +158,26 @@ def test_fitness_vs_oracle(gpu_ctx, nr, op_set, ip, N, k, P, lo, hi):
     dds = _dds(gpu_ctx, isa, x, y)
     fit, sse = gpu_fitness(dds, words, offsets)
     oisa = O.isa_from(isa)
+    # 1) same unary functions: every program within 1e-6 (summation only)
+    with O.unary_math(math_host_path()):
+        mfit, msse = O.eval_csr(oisa, words, offsets, x, y, threads=4)
+    r = rel_err(fit, mfit)
+    assert (r <= REL_TOL).all(), f"max rel {r.max()} vs the device-math oracle"
+    fin = np.isfinite(msse)
+    rs = rel_err(sse[fin], msse[fin])
+    assert (rs <= REL_TOL).all(), f"SSE max rel {rs.max()}"
+    assert same_bits(np.isfinite(sse), np.isfinite(msse)).all()
+    # 2) the reference semantics (host C library)
     ofit, osse = O.eval_csr(oisa, words, offsets, x, y, threads=4)
     free = _transcendental_free(oisa, words, offsets, op_set)
     r = rel_err(fit, ofit)
     assert (r[free] <= REL_TOL).all(), f"max rel {r[free].max()} on transcendental-free programs"
-    fin = np.isfinite(osse) & free
-    rs = rel_err(sse[fin], osse[fin])
-    assert (rs <= REL_TOL).all(), f"SSE max rel {rs.max()}"
     over = np.nonzero(r > REL_TOL)[0]
-    chaotic = 0
-    for i in over:
-        ff, fd = O.dual_precision_fitness(oisa, words[offsets[i]:offsets[i + 1]], x, y)
-        if abs(fd - ff) > REL_TOL * max(abs(ff), 1e-30):
-            chaotic += 1
-    print(f"over tol: {over.size}/{P}, chaos-attributed {chaotic}")
-    assert chaotic == over.size, "a precision-stable program disagrees with the oracle"
+    attributed = sum(O.attributed(oisa, words[offsets[i]:offsets[i + 1]], x, y, REL_TOL)
+                     for i in over)
+    print(f"vs host libm: over 1e-6 {over.size}/{P}, attributed {attributed}, "
+          f"max rel {r.max():.2e}")
+    assert attributed == over.size, "a well-defined program disagrees with the oracle"
     assert over.size <= max(1, P // 100)
 
 
@@ -184,8 +206,8 @@ def test_engine_population_dense_layouts(gpu_ctx, tag):
     oisa = O.make_isa(8)
     over = np.nonzero(r > REL_TOL)[0]
     for i in over:
-        ff, fd = O.dual_precision_fitness(oisa, words[offs[i]:offs[i + 1]], f["inputs"], f["targets"])
-        assert abs(fd - ff) > REL_TOL * max(abs(ff), 1e-30), f"stable program {i} disagrees"
+        assert O.attributed(oisa, words[offs[i]:offs[i + 1]], f["inputs"], f["targets"], REL_TOL), \
+            f"well-defined program {i} disagrees"
     assert over.size <= lengths.size // 100
     assert np.abs(got["packed"] - ref).max() < 1e-4  # the reference's own cuda_diff bar
 

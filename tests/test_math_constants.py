@@ -1,0 +1,127 @@
+"""The unary functions of the interpreter (galop_amd/csrc/lgp_math.h), on the
+CPU:
+
+* every constant in the header equals the value galop_amd/math_constants.py
+  derives from exact integer arithmetic (Machin's formula for pi, the binary
+  series of ln 2, exact Taylor coefficients);
+* the host build of the functions (libgalop_math_host.so, the same source the
+  device compiles) is within 1 ulp of the correctly rounded value and of the
+  host C library on special and random arguments, and keeps the special cases
+  (NaN, infinities, signed zero, protected log of 0);
+* a planted mistake (a wrong 2/pi word) is caught by the derivation check.
+"""
+import ctypes
+import os
+import re
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import REPO, ulp_distance
+
+sys.path.insert(0, os.path.join(REPO, "galop_amd"))
+import math_constants as MC  # noqa: E402
+
+HEADER = os.path.join(REPO, "galop_amd", "csrc", "lgp_math.h")
+HOSTLIB = os.path.join(REPO, "galop_amd", "libgalop_math_host.so")
+
+
+def _header_values():
+    text = open(HEADER).read()
+    vals = {}
+    for name, body in re.findall(r"constexpr double (\w+)(?:\[\d+\])? = \{?([^;]*?)\}?;", text, re.S):
+        nums = [float.fromhex(t.strip()) for t in body.split(",") if t.strip()]
+        vals[name] = nums if len(nums) > 1 else nums[0]
+    words = re.search(r"TWO_OVER_PI_W\[8\] = \{([^}]*)\}", text).group(1)
+    vals["TWO_OVER_PI_W"] = [int(w.strip().rstrip("u"), 16) for w in words.split(",")]
+    return vals
+
+
+def test_constants_match_derivation():
+    c = MC.constants()
+    h = _header_values()
+    assert [h["PIO2_0"], h["PIO2_1"], h["PIO2_2"]] == c["PIO2"]
+    assert [h["LN2_0"], h["LN2_1"], h["LN2_2"]] == c["LN2"]
+    assert h["TWO_OVER_PI"] == c["TWO_OVER_PI"]
+    assert h["PIO2_SCALED62"] == c["PIO2_SCALED62"]
+    assert h["INV_LN2"] == c["INV_LN2"]
+    assert h["EXP_C"] == c["EXP_POLY"]
+    assert h["SIN_C"] == c["SIN_POLY"]
+    assert h["COS_C"] == c["COS_POLY"]
+    assert h["LOG_C"] == c["LOG_POLY"]
+    assert h["TWO_OVER_PI_W"] == c["TWO_OVER_PI_WORDS"][:8]
+    # the splits are exact decompositions with short leading parts
+    from fractions import Fraction
+    for parts, bits in ((c["PIO2"], 28), (c["LN2"], 32)):
+        for p in parts[:2]:
+            m = Fraction(p)
+            while m.denominator != 1:
+                m *= 2
+            assert abs(m.numerator).bit_length() <= bits
+    import math
+    assert sum(c["PIO2"]) == math.pi / 2 and sum(c["LN2"]) == math.log(2)
+
+
+def test_planted_wrong_word_is_caught():
+    c = MC.constants()
+    bad = list(c["TWO_OVER_PI_WORDS"][:8])
+    bad[3] ^= 1 << 7
+    assert bad != _header_values()["TWO_OVER_PI_W"]
+
+
+def _host():
+    if not os.path.exists(HOSTLIB):
+        pytest.skip("libgalop_math_host.so not built")
+    L = ctypes.CDLL(HOSTLIB)
+    L.gm_map.argtypes = [ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
+    for n in ("gm_sinf", "gm_cosf", "gm_expf", "gm_plogf"):
+        getattr(L, n).restype = ctypes.c_float
+        getattr(L, n).argtypes = [ctypes.c_float]
+    return L
+
+
+def _args(rng, n):
+    a = np.concatenate([rng.uniform(-10, 10, n), rng.uniform(-110, 90, n),
+                        10.0 ** rng.uniform(-45, 38.5, n) * rng.choice([-1, 1], n),
+                        # near multiples of pi/2, the hard cases of the reduction
+                        (np.rint(rng.uniform(-1e7, 1e7, n)) * (np.pi / 2)) * (1 + 1e-9)])
+    return a.astype(np.float32)
+
+
+@pytest.mark.parametrize("op", [4, 5, 6, 7])
+def test_host_functions_accuracy(op):
+    import oracle_py as O
+    L = _host()
+    rng = np.random.default_rng(100 + op)
+    x = _args(rng, 250000)
+    y = np.empty_like(x)
+    L.gm_map(op, x.ctypes.data, y.ctypes.data, x.size)
+    xd = x.astype(np.float64)
+    with np.errstate(all="ignore"):
+        fn = {4: np.sin, 5: np.cos, 6: np.exp,
+              7: lambda v: np.where(v != 0, np.log(np.abs(v)), 0.0)}[op]
+        exact = fn(xd).astype(np.float32)
+    d = ulp_distance(y, exact)
+    assert d.max() <= 1
+    libm = O.run_program(O.make_isa(8, 1 if op >= 6 else 0), np.array([op], np.uint32), x[:, None])
+    dl = ulp_distance(y, libm)
+    assert dl.max() <= 1
+    print(f"op {op}: correctly rounded on {(d == 0).mean():.5%}, equal to host libm on "
+          f"{(dl == 0).mean():.5%}")
+
+
+def test_special_cases():
+    L = _host()
+    inf, nan = float("inf"), float("nan")
+    for f in (L.gm_sinf, L.gm_cosf):
+        assert np.isnan(f(inf)) and np.isnan(f(-inf)) and np.isnan(f(nan))
+    assert np.signbit(L.gm_sinf(-0.0)) and L.gm_sinf(-0.0) == 0.0
+    assert L.gm_cosf(0.0) == 1.0 and L.gm_cosf(-0.0) == 1.0
+    assert L.gm_expf(0.0) == 1.0 and L.gm_expf(-inf) == 0.0 and L.gm_expf(inf) == inf
+    assert L.gm_expf(89.0) == inf and L.gm_expf(-104.5) == 0.0 and np.isnan(L.gm_expf(nan))
+    assert L.gm_expf(-103.0) > 0.0  # subnormal result kept
+    assert L.gm_plogf(0.0) == 0.0 and L.gm_plogf(-0.0) == 0.0
+    assert L.gm_plogf(inf) == inf and L.gm_plogf(-inf) == inf and np.isnan(L.gm_plogf(nan))
+    assert L.gm_plogf(1.0) == 0.0 and L.gm_plogf(-1.0) == 0.0
+    assert L.gm_plogf(1e-45) < -103.0  # subnormal argument
