@@ -79,8 +79,8 @@ def cpu_baseline(wl, gpu_fit, target_s, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="plain", choices=["plain", "uneven", "casebound"])
     ap.add_argument("--kernel", default="plain", choices=["plain", "batched"])
     ap.add_argument("--programs", type=int, default=0, help="override population size")
@@ -159,9 +159,6 @@ def main():
     ev0.record()
     for i in range(args.steps):
         step()
-        if args.steps > 1:
-            torch.cuda.synchronize()
-            log(f"[bench] step {i + 1}/{args.steps} done at {time.perf_counter() - wall0:.2f}s")
     ev1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - wall0

@@ -110,10 +110,10 @@ def build_adapter_driver(verbose=False):
 
 
 def build(verbose=False):
-    build_oracle(verbose)
     build_library(verbose)
     build_math_host(verbose)
     build_adapter_driver(verbose)
+    build_oracle(verbose)  # last: its engine driver links the library
 
 
 if __name__ == "__main__":

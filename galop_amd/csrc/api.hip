@@ -166,7 +166,7 @@ int galop_dataset_create(galop_ctx* ctx, const galop_isa* isa, const float* inpu
             planar[(size_t)i * n_pad + c] = inputs[c * n_inputs + i];
     std::vector<float> tpad((size_t)n_pad, 0.0f);
     if (n_cases > 0) std::memcpy(tpad.data(), targets, (size_t)n_cases * sizeof(float));
-    float consts[64];
+    float consts[kConstSlots];
     std::memset(consts, 0, sizeof(consts));
     for (int i = 0; i < isa->num_registers; ++i) consts[i] = isa->constants[i];
 
@@ -179,7 +179,7 @@ int galop_dataset_create(galop_ctx* ctx, const galop_isa* isa, const float* inpu
     };
     hipError_t e = ds->inputs.alloc(planar.size());
     if (e == hipSuccess) e = ds->targets.alloc(tpad.size());
-    if (e == hipSuccess) e = ds->consts.alloc(64);
+    if (e == hipSuccess) e = ds->consts.alloc(kConstSlots);
     if (e != hipSuccess) return cleanup(hip_fail(ctx, e, "galop_dataset_create: hipMalloc"));
     e = hipMemcpyAsync(ds->inputs.ptr, planar.data(), planar.size() * sizeof(float),
                        hipMemcpyHostToDevice, ctx->stream);

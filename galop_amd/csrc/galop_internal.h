@@ -32,7 +32,7 @@ hipError_t launch_run_cases(const DevData& ds, int num_registers, const uint32_t
                             const int64_t* offsets, int64_t n, float* out,
                             hipStream_t stream);
 
-// Fill consts[16..64) with the unary-on-constant table from consts[0..16).
+// Fill consts[64..128) with the unary-on-constant table from consts[0..16).
 hipError_t launch_const_table(float* consts, hipStream_t stream);
 
 // Device intron removal: per-program effective counts, scan, scatter.

@@ -26,10 +26,10 @@ plain_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ off
     const int wave_in_block = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int64_t p = first + (int64_t)blockIdx.x * (int64_t)(blockDim.x >> 6) + wave_in_block;
     if (p >= n) return;  // wave-uniform
-    const float cvec = ds.consts[lane];
+    const ConstRegs cr = load_const_regs(ds.consts, lane);
     const int64_t lo = offsets[p];
     const int64_t hi = offsets[p + 1];
-    const double sse = eval_program_wave<NR, K, EXT>(words + lo, hi - lo, ds, lane, cvec, nullptr);
+    const double sse = eval_program_wave<NR, K, EXT>(words + lo, hi - lo, ds, lane, cr, nullptr);
     if (lane == 0) {
         fitness[p] = sse_to_fitness(sse, ds.n_cases);
         if (sse_out) sse_out[p] = sse;
@@ -44,14 +44,14 @@ run_cases_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__
     const int wave_in_block = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int64_t p = first + (int64_t)blockIdx.x * (int64_t)(blockDim.x >> 6) + wave_in_block;
     if (p >= n) return;
-    const float cvec = ds.consts[lane];
+    const ConstRegs cr = load_const_regs(ds.consts, lane);
     const int64_t lo = offsets[p];
     const int64_t hi = offsets[p + 1];
-    (void)eval_program_wave<NR, 1, EXT>(words + lo, hi - lo, ds, lane, cvec, out + p * ds.n_cases);
+    (void)eval_program_wave<NR, 1, EXT>(words + lo, hi - lo, ds, lane, cr, out + p * ds.n_cases);
 }
 
 __global__ void const_table_kernel(float* consts) {
-    const int t = (int)threadIdx.x;  // 48 threads
+    const int t = (int)threadIdx.x;  // 64 threads: unary op u on constant idx
     const int u = t / 16, idx = t % 16;
     const float b = consts[idx];
     float v;
@@ -66,10 +66,11 @@ __global__ void const_table_kernel(float* consts) {
 
 PlainTuning resolve_plain_tuning(const DevData& ds, int num_registers, PlainTuning t) {
     if (t.cases_per_lane <= 0) {
-        // enough cases per lane to amortise scalar decode, without padding
-        // small datasets: K = ceil(N/64) capped at 4 (8 registers) / 2 (16)
+        // enough cases per lane to amortise the scalar decode and dispatch,
+        // without padding small datasets: K = ceil(N/64) capped at 8 (8
+        // registers) / 4 (16 registers)
         const int64_t per_lane = (ds.n_cases + kWave - 1) / kWave;
-        int kmax = num_registers == 8 ? 4 : 2;
+        int kmax = num_registers == 8 ? 8 : 4;
         int k = 1;
         while (k < kmax && k < per_lane) k *= 2;
         t.cases_per_lane = k;
@@ -134,7 +135,7 @@ hipError_t launch_run_cases(const DevData& ds, int num_registers, const uint32_t
 }
 
 hipError_t launch_const_table(float* consts, hipStream_t stream) {
-    hipLaunchKernelGGL(const_table_kernel, dim3(1), dim3(48), 0, stream, consts);
+    hipLaunchKernelGGL(const_table_kernel, dim3(1), dim3(64), 0, stream, consts);
     return hipGetLastError();
 }
 
@@ -385,7 +386,7 @@ batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ o
                int64_t n, int chunk, DevData ds, float* __restrict__ fitness,
                double* __restrict__ sse_out) {
     const int lane = (int)(threadIdx.x & (kWave - 1));
-    const float cvec = ds.consts[lane];
+    const ConstRegs cr = load_const_regs(ds.consts, lane);
     const DestDecode dd = dest_decode(NR, ds.input_protect);
     const int out_reg = ds.out_reg;
 
@@ -409,7 +410,7 @@ batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ o
             double sse;
             if (RESIDENT) {
                 RegFile<NR, K> r = init;
-                run_program<NR, K, EXT>(words + lo, hi - lo, lane, r, cvec, dd);
+                run_program<NR, K, EXT>(words + lo, hi - lo, lane, r, cr, dd);
                 LaneAcc acc;
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
@@ -417,7 +418,7 @@ batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ o
                 }
                 sse = wave_sum_f64(acc.total());
             } else {
-                sse = eval_program_wave<NR, K, EXT>(words + lo, hi - lo, ds, lane, cvec, nullptr);
+                sse = eval_program_wave<NR, K, EXT>(words + lo, hi - lo, ds, lane, cr, nullptr);
             }
             if (lane == 0) {
                 fitness[p] = sse_to_fitness(sse, ds.n_cases);

@@ -39,7 +39,7 @@ constexpr int kWave = 64;
 struct DevData {
     const float* inputs;   // feature-planar: inputs[i * n_pad + c]
     const float* targets;  // targets[c], zero padded to n_pad
-    const float* consts;   // 64 floats: [0,16) constant pool, [16,64) unary table
+    const float* consts;   // 128 floats: [0,16) constant pool, [64,128) unary table
     int64_t n_cases;
     int64_t n_pad;         // multiple of 64 * kMaxK
     int32_t n_inputs;
@@ -48,11 +48,25 @@ struct DevData {
     int32_t out_reg;
 };
 
-// Unary-on-constant table layout in DevData::consts: entry 16 + u*16 + idx
-// holds unary op u (0 sin, 1 cos, 2 exp, 3 plog) applied to constant idx.
-// It is filled on the device by the same functions the interpreter calls,
-// so using it is bitwise identical to evaluating the op.
-constexpr int kUnaryTableBase = 16;
+// DevData::consts: entries [0,16) are the constant pool; entry 64 + u*16 + idx
+// holds unary op u (0 sin, 1 cos, 2 exp, 3 plog) applied to constant idx.  The
+// table is filled on the device by the same functions the interpreter calls,
+// so using it is bitwise identical to evaluating the op.  A wave keeps both
+// halves in two VGPRs (lane i holds entry i and entry 64+i) and broadcasts an
+// entry with v_readlane.
+constexpr int kConstSlots = 128;
+constexpr int kUnaryTableBase = 64;
+
+struct ConstRegs {
+    float pool;   // consts[lane]
+    float table;  // consts[64 + lane]
+};
+__device__ __forceinline__ ConstRegs load_const_regs(const float* consts, int lane) {
+    ConstRegs c;
+    c.pool = consts[lane];
+    c.table = consts[kUnaryTableBase + lane];
+    return c;
+}
 
 // ---- the operations -------------------------------------------------------
 __device__ __forceinline__ float op_div(float a, float b) {
@@ -128,7 +142,7 @@ __device__ __forceinline__ DestDecode dest_decode(int nr, int iprot) {
 
 template <int NR, int K, int EXT>
 __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
-                                          float cvec, DestDecode dd) {
+                                          ConstRegs cr, DestDecode dd) {
     const int op = (int)(w & 7u);
     const int d = (int)((w >> 8) & dd.dmask) + dd.dbase;
     const int s1 = (int)((w >> 16) & (uint32_t)(NR - 1));
@@ -140,7 +154,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
         float av[K], bv[K];                                               \
         _Pragma("unroll") for (int k = 0; k < K; ++k) av[k] = r.get(k, s1);   \
         if (kc) {                                                         \
-            const float bc = wave_bcast(cvec, s2);                        \
+            const float bc = wave_bcast(cr.pool, s2);                     \
             _Pragma("unroll") for (int k = 0; k < K; ++k) bv[k] = bc;     \
         } else {                                                          \
             _Pragma("unroll") for (int k = 0; k < K; ++k) bv[k] = r.get(k, s2); \
@@ -157,7 +171,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
 #define GALOP_UNARY(U, FN)                                                \
     do {                                                                  \
         if (kc) {                                                         \
-            const float v = wave_bcast(cvec, kUnaryTableBase + (U) * 16 + s2); \
+            const float v = wave_bcast(cr.table, (U) * 16 + s2);          \
             _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, v);    \
         } else {                                                          \
             float bv[K], cv[K];                                           \
@@ -192,7 +206,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
 template <int NR, int K, int EXT>
 __device__ __forceinline__ void run_program(const uint32_t* __restrict__ prog,
                                             int64_t len, int lane,
-                                            RegFile<NR, K>& r, float cvec,
+                                            RegFile<NR, K>& r, ConstRegs cr,
                                             DestDecode dd) {
     uint32_t next = (lane < len) ? prog[lane] : 0u;
     for (int64_t base = 0; base < len; base += kWave) {
@@ -201,7 +215,7 @@ __device__ __forceinline__ void run_program(const uint32_t* __restrict__ prog,
         next = (nb + lane < len) ? prog[nb + lane] : 0u;
         const int cnt = (int)((len - base) < kWave ? (len - base) : kWave);
         for (int j = 0; j < cnt; ++j)
-            exec_word<NR, K, EXT>(wave_word(chunk, j), r, cvec, dd);
+            exec_word<NR, K, EXT>(wave_word(chunk, j), r, cr, dd);
     }
 }
 
@@ -271,7 +285,7 @@ __device__ __forceinline__ float sse_to_fitness(double sse, int64_t n_cases) {
 template <int NR, int K, int EXT>
 __device__ __forceinline__ double eval_program_wave(const uint32_t* __restrict__ prog,
                                                     int64_t len, const DevData& ds,
-                                                    int lane, float cvec,
+                                                    int lane, ConstRegs cr,
                                                     float* __restrict__ case_out) {
     const DestDecode dd = dest_decode(NR, ds.input_protect);
     const int out_reg = ds.out_reg;
@@ -281,7 +295,7 @@ __device__ __forceinline__ double eval_program_wave(const uint32_t* __restrict__
     for (int64_t t = 0; t < n_tiles; ++t) {
         RegFile<NR, K> r;
         load_cases<NR, K>(ds, t, lane, r);
-        run_program<NR, K, EXT>(prog, len, lane, r, cvec, dd);
+        run_program<NR, K, EXT>(prog, len, lane, r, cr, dd);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const int64_t c = t * tile_cases + k * kWave + lane;

@@ -195,20 +195,24 @@ GM_HD double cos_poly(double r) {
     return fma_d(r2, p, 1.0);
 }
 
+// sin x (want_cos = 0) or cos x (want_cos = 1) from the reduced argument.
+// Both series are evaluated and the result selected per lane: the quadrant
+// differs from lane to lane, and a branch on it would run both anyway.
+GM_HD double sincos_from_reduced(double r, int q, int want_cos) {
+    const double s = sin_poly(r);
+    const double c = cos_poly(r);
+    const int qq = q + want_cos;  // cos x = sin(x + pi/2)
+    const double v = (qq & 1) ? c : s;
+    return (qq & 2) ? -v : v;
+}
+
 GM_HD float sinf_r(float x) {
     if (x == 0.0f) return x;  // keeps the sign of zero
     const float ax = u2f(f2u(x) & 0x7fffffffu);
     if (!(ax < u2f(0x7f800000u))) return x - x;  // inf, NaN -> NaN
     int q;
     const double r = reduce_pio2(x, q);
-    double v;
-    switch (q) {
-    case 0: v = sin_poly(r); break;
-    case 1: v = cos_poly(r); break;
-    case 2: v = -sin_poly(r); break;
-    default: v = -cos_poly(r); break;
-    }
-    return (float)v;
+    return (float)sincos_from_reduced(r, q, 0);
 }
 
 GM_HD float cosf_r(float x) {
@@ -216,14 +220,7 @@ GM_HD float cosf_r(float x) {
     if (!(ax < u2f(0x7f800000u))) return x - x;
     int q;
     const double r = reduce_pio2(x, q);
-    double v;
-    switch (q) {
-    case 0: v = cos_poly(r); break;
-    case 1: v = -sin_poly(r); break;
-    case 2: v = -cos_poly(r); break;
-    default: v = sin_poly(r); break;
-    }
-    return (float)v;
+    return (float)sincos_from_reduced(r, q, 1);
 }
 
 }  // namespace galop_math

@@ -1,0 +1,87 @@
+// ref_hip_run.cpp -- the reference engine (LGPEngine::evolve_sr) on the
+// engine-only regression configuration, with either its own CpuEvaluator or
+// this repository's HipEvaluator behind EvalBackend (the drop-in).  Built by
+// oracle/Makefile (target `ref`) from the reference sources where they lie,
+// with -DPOPULATION_SIZE_OVERRIDE=1000 -DMAX_PROGRAM_SIZE_OVERRIDE=64
+// -DTOURNAMENT_SIZE_OVERRIDE=4 -DMAX_GENERATIONS_OVERRIDE=50; run only by tests.
+//
+// Dataset: x ~ U[-1,1] (std::mt19937 seed 0), 20 cases; y = outputs of a
+// random 20-instruction program over {+,-,*,/} on 8 registers drawn from the
+// same generator (redrawn until every output is finite and not constant).
+// The reference's fixed starting length is kept (STARTING_PROGRAM_SIZE = 8);
+// it has no option for initial lengths drawn from a range.
+//
+// usage: ref_hip_run <cpu|hip|hip-effective> <seed> <history.csv>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+
+#include "Dataset.h"
+#include "EvaluatorCpu.h"
+#include "HipEvaluator.h"
+#include "ISA.h"
+#include "Interpreter.h"
+#include "LGPConfig.h"
+#include "LGPEngine.h"
+
+static Dataset make_dataset() {
+    std::mt19937 rng(0);
+    std::uniform_real_distribution<float> ux(-1.0f, 1.0f);
+    Dataset ds;
+    ds.N = 20;
+    ds.num_inputs = 1;
+    ds.inputs.assign(ds.padded_N(), 0.0f);
+    ds.targets.assign(ds.padded_N(), 0.0f);
+    for (int c = 0; c < ds.N; ++c) ds.inputs[c] = ux(rng);
+    float out[LGPConfig::NUM_CONTEXTS];
+    for (int attempt = 0; attempt < 1000; ++attempt) {
+        uint32_t prog[20];
+        for (auto& w : prog) w = ISA::encode_from_random((uint32_t)rng()) & ~0x4u;  // ops 0..3
+        Interpreter::run_stateless(prog, 20, ds.inputs.data(), 1, out);
+        bool ok = true, varies = false;
+        for (int c = 0; c < ds.N; ++c) {
+            ok &= std::isfinite(out[c]);
+            varies |= out[c] != out[0];
+        }
+        if (ok && varies) {
+            for (int c = 0; c < ds.N; ++c) ds.targets[c] = out[c];
+            break;
+        }
+    }
+    double mean = 0.0;
+    for (int c = 0; c < ds.N; ++c) mean += ds.targets[c];
+    mean /= ds.N;
+    for (int c = 0; c < ds.N; ++c) ds.ss_tot += (ds.targets[c] - mean) * (ds.targets[c] - mean);
+    return ds;
+}
+
+int main(int argc, char** argv) {
+    if (argc != 4) {
+        std::fprintf(stderr, "usage: %s <cpu|hip|hip-effective> <seed> <history.csv>\n", argv[0]);
+        return 2;
+    }
+    const std::string mode = argv[1];
+    const uint32_t seed = (uint32_t)std::stoul(argv[2]);
+    std::unique_ptr<EvalBackend> backend;
+    if (mode == "cpu") backend = std::make_unique<CpuEvaluator>();
+    else if (mode == "hip") backend = std::make_unique<HipEvaluator>(HipEvaluator::Layout::Packed);
+    else if (mode == "hip-effective")
+        backend = std::make_unique<HipEvaluator>(HipEvaluator::Layout::PackedEffective);
+    else return 2;
+    const Dataset ds = make_dataset();
+    LGPEngine engine(seed, std::move(backend));
+    engine.evolve_sr(ds);
+    FILE* f = std::fopen(argv[3], "w");
+    if (!f) return 2;
+    std::fprintf(f, "gen,best_fitness,mean_fitness,best_length,mean_length\n");
+    for (const auto& h : engine.history_view())
+        std::fprintf(f, "%d,%.9g,%.9g,%d,%.6g\n", h.generation, h.best_fitness, h.mean_fitness,
+                     h.best_length, h.mean_length);
+    std::fclose(f);
+    std::printf("%s seed %u: %zu generations, final best fitness %.9g\n", mode.c_str(), seed,
+                engine.history_view().size(), engine.history_view().back().best_fitness);
+    return 0;
+}

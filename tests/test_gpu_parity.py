@@ -118,6 +118,28 @@ def test_unary_op_ulp(gpu_ctx, op_set, op):
           f"to correctly rounded on {(d_exact == 0).mean():.4%}")
 
 
+@pytest.mark.parametrize("nr", [8, 16])
+@pytest.mark.parametrize("op_set", [0, 1])
+def test_unary_on_every_constant(gpu_ctx, nr, op_set):
+    """r_d = op(C[idx]) for every unary op and constant index: the device
+    takes these from a table it fills with its own functions; the result must
+    be the function applied to the constant, bit for bit."""
+    rng = np.random.default_rng(60 + nr + op_set)
+    consts = np.concatenate([[0.0, -0.0, 1e-40, -3.5e38, 88.0, -90.0, np.pi, 1e30],
+                             rng.uniform(-5, 5, 8)]).astype(np.float32)
+    isa = _isa(nr, op_set, False, consts)
+    x = rng.uniform(-1, 1, (70, 1)).astype(np.float32)
+    dds = _dds(gpu_ctx, isa, x, np.zeros(70, np.float32))
+    ops = [4, 5] + ([6, 7] if op_set == 1 else [])
+    words = np.array([op | ((0x80 | idx) << 24) for op in ops for idx in range(nr)], np.uint32)
+    offsets = np.arange(words.size + 1, dtype=np.int64)
+    out = gpu_cases(dds, words, offsets)
+    oisa = O.isa_from(isa)
+    with O.unary_math(math_host_path()):
+        exact = O.run_csr(oisa, words, offsets, x)
+    assert same_bits(out, exact).all()
+
+
 @pytest.mark.parametrize("op_set,op", [(0, 0), (0, 1), (0, 2), (0, 3), (0, 6), (0, 7)])
 def test_binary_op_bit_exact(gpu_ctx, op_set, op):
     """r0 = r0 op r1 on special and random pairs: bit-exact."""
