@@ -11,6 +11,10 @@ tests/cuda_evo_bench.cu:5-7).
 Prints ONE JSON line on stdout; progress goes to stderr.
   python bench.py [--steps K] [--warmup W] [--workload plain|uneven|casebound]
                   [--kernel plain|batched] [--programs P] [--no-cpu-baseline]
+                  [--no-uneven] [--no-serving] [--calibrate-copy BYTES]
+The default run also measures, in the same process: the uneven-length
+population with the plain kernel vs the batched persistent generation, and
+the 64-run serving mix (p50/p99 latency).
 """
 import argparse
 import json
@@ -76,6 +80,60 @@ def cpu_baseline(wl, gpu_fit, target_s, threads):
     }
 
 
+def time_steps(fn, steps, warmup):
+    """Average milliseconds of fn() over `steps` calls, HIP events on the
+    current stream (the one every launch here uses)."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(steps):
+        fn()
+    ev1.record()
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / steps
+
+
+def bench_uneven(ctx, steps, warmup, programs=0):
+    """Uneven lengths: plain kernel on the raw programs vs the batched
+    generation (device intron removal -> length buckets -> persistent work
+    queue, captured in a hipGraph, tuned once).  Both are charged the same raw
+    GP-ops, the reference's convention (tests/cuda_evo_bench.cu:5-7)."""
+    import torch
+    from galop_amd import BatchPlan, eval_csr_device, workloads
+    kw = {"n_programs": programs} if programs else {}
+    wl = workloads.make_uneven(ctx, seed=0, **kw)
+    P = wl.n_programs
+    stream = torch.cuda.current_stream().cuda_stream
+    fit_p = torch.empty(P, dtype=torch.float32, device="cuda")
+    fit_b = torch.empty(P, dtype=torch.float32, device="cuda")
+    ms_plain = time_steps(lambda: eval_csr_device(wl.dds, wl.words_d, wl.offsets_d, P, fit_p,
+                                                  stream=stream), steps, warmup)
+    plan = BatchPlan(wl.dds, P, max_words=wl.raw_words, strip_introns=True)
+    tuned = plan.autotune(wl.words_d, wl.offsets_d, P, fit_b)
+    ms_batched = time_steps(lambda: plan.eval(wl.words_d, wl.offsets_d, P, fit_b, use_graph=True,
+                                              stream=stream), steps, warmup)
+    same = bool(torch.equal(fit_p.view(torch.int32), fit_b.view(torch.int32)))
+    raw_ops = float(wl.raw_words) * wl.n_cases
+    exec_ops = float(wl.exec_words) * wl.n_cases
+    res = {
+        "workload": wl.notes, "programs": P, "raw_words": wl.raw_words,
+        "effective_words": wl.exec_words, "cases": wl.n_cases,
+        "plain_ms": round(ms_plain, 3), "batched_ms": round(ms_batched, 3),
+        "plain_gpops_raw": round(raw_ops / ms_plain / 1e6, 3),
+        "batched_gpops_raw": round(raw_ops / ms_batched / 1e6, 3),
+        "plain_gpops_exec": round(raw_ops / ms_plain / 1e6, 3),
+        "batched_gpops_exec": round(exec_ops / ms_batched / 1e6, 3),
+        "speedup_batched_vs_plain": round(ms_plain / ms_batched, 3),
+        "fitness_bitwise_equal": same, "tuning": tuned,
+    }
+    plan.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -87,14 +145,17 @@ def main():
     ap.add_argument("--cases-per-lane", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
-    ap.add_argument("--serving", action="store_true",
-                    help="also run the 64-run serving mix and report p50/p99 latency")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the 64-run serving mix (p50/p99 latency)")
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--serving-runs", type=int, default=64)
     ap.add_argument("--serving-generations", type=int, default=3)
     ap.add_argument("--serving-pop-scale", type=float, default=1.0 / 64)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="run one device copy of this many bytes first (HBM counter calibration)")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven-length batched vs plain comparison")
+    ap.add_argument("--uneven-programs", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -230,9 +291,15 @@ def main():
             "vs_device_math_oracle": {"max_rel_err": cbn["device_math_max_rel_err"],
                                       "within_1e-6": cbn["device_math_within_1e-6"]},
         }
-    if args.serving:
+    del fitness, sse
+    if not args.no_uneven and args.workload == "plain":
+        log("[bench] uneven lengths: plain vs batched")
+        res = bench_uneven(ctx, max(3, args.steps // 4), 1, args.uneven_programs)
+        log(f"[bench] uneven: {res}")
+        line["uneven"] = res
+    if not args.no_serving and args.workload == "plain":
         from galop_amd.serving_mix import run_mix
-        del fitness, sse
+        log("[bench] serving mix")
         res = run_mix(ctx, n_runs=args.serving_runs, generations=args.serving_generations,
                       pop_scale=args.serving_pop_scale, log=log)
         log(f"[bench] serving: {res}")

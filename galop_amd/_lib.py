@@ -71,7 +71,7 @@ SIGNATURES = {
     "galop_batch_create": (ctypes.c_int, [_vp, _vp, _i64, _i64, _i32, ctypes.POINTER(_vp)]),
     "galop_batch_destroy": (None, [_vp]),
     "galop_batch_eval_device": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i32, _vp]),
-    "galop_batch_autotune": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _pi32, _pi32, _pi32]),
+    "galop_batch_autotune": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _pi32, _pi32, _pi32, _pi32]),
     "galop_server_create": (ctypes.c_int, [_vp, _i64, _i32, ctypes.POINTER(_vp)]),
     "galop_server_destroy": (None, [_vp]),
     "galop_submit": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _pi64]),

@@ -4,6 +4,11 @@ Every HIP source is compiled for gfx950 only, with:
   -ffp-contract=off        interpreter ops are single IEEE operations, as in the
                            reference CPU interpreter (no FMA contraction)
   -packed-fp32-ops off     no packed-FP32 instructions
+  -fmacf64-inst off        FP64 FMAs stay three-address (v_fma_f64), so the
+                           polynomial coefficients of the unary operations are
+                           read straight from SGPR pairs; the two-address
+                           v_fmac_f64 needs its addend in VGPRs and costs two
+                           extra moves per coefficient
   amdgpu-use-divergent-register-indexing
                            register-file accesses with a wave-uniform index use
                            GPR-index-mode moves instead of compare/select chains
@@ -35,6 +40,7 @@ DEVICE_FLAGS = [
     "-fPIC",
     "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Xclang", "-target-feature", "-Xclang", "-fmacf64-inst",
     "-mllvm", "-amdgpu-use-divergent-register-indexing=1",
     "-mllvm", "-structurizecfg-skip-uniform-regions=true",
     "-Wall",
@@ -44,7 +50,8 @@ DEVICE_FLAGS = [
 def _run(cmd):
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in r.stdout.splitlines()
-                    if "packed-fp32-ops" not in l and "hip-link" not in l)
+                    if "packed-fp32-ops" not in l and "fmacf64-inst" not in l
+                    and "hip-link" not in l)
     if r.returncode != 0:
         raise RuntimeError(f"build failed: {' '.join(cmd)}\n{out}")
     return out

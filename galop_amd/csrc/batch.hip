@@ -2,9 +2,11 @@
 //
 // A plan owns the bucketing / work-queue scratch for one dataset and a
 // hipGraph of one generation: reset -> length histogram -> bucket starts ->
-// scatter into longest-first order -> persistent evaluation.  The graph is
-// captured on first use and replayed while the buffers it was captured with
-// stay the same; the reset is a kernel, so replays start from zeroed counters.
+// scatter into longest-first order -> [intron removal in that order: counts,
+// scan, compaction, laid out by order position] -> persistent evaluation.
+// The graph is captured on first use and replayed while the buffers it was
+// captured with stay the same; the reset is a kernel, so replays start from
+// zeroed counters.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -109,28 +111,29 @@ void galop_batch_destroy(galop_batch* b) {
     delete b;
 }
 
-// One generation on stream s: optional intron removal, then the bucketed
-// persistent evaluation.  Pure stream work (kernels only), so it can be
-// captured.
+// One generation on stream s: the length order, optional intron removal in
+// that order, then the persistent evaluation.  Pure stream work (kernels
+// only), so it can be captured.
 static hipError_t batch_generation(galop_batch* b, const uint32_t* d_words, const int64_t* d_offsets,
                                    int64_t n, float* d_fitness, double* d_sse, hipStream_t s) {
     const DevData& dd = b->ds->dev;
     const galop_isa& isa = b->ds->isa;
-    if (b->strip) {
-        hipError_t e = launch_effective_counts(isa.num_registers, isa.op_set, isa.input_protect, d_words,
-                                               d_offsets, n, b->counts.ptr, s);
-        if (e == hipSuccess)
-            e = launch_exclusive_scan(b->counts.ptr, n, b->eff_offsets.ptr, b->scan_tmp.ptr, s);
-        if (e == hipSuccess)
-            e = launch_effective_scatter(isa.num_registers, isa.op_set, isa.input_protect, d_words,
-                                         d_offsets, n, b->eff_offsets.ptr, b->eff_words.ptr,
-                                         (int64_t)b->eff_words.cap, s);
-        if (e != hipSuccess) return e;
-        d_words = b->eff_words.ptr;
-        d_offsets = b->eff_offsets.ptr;
-    }
-    return launch_batched(dd, isa.num_registers, b->tuning, b->ctx->n_cu, b->bufs.view(), d_words,
-                          d_offsets, n, d_fitness, d_sse, s);
+    const BatchScratch sc = b->bufs.view();
+    hipError_t e = launch_batch_order(d_offsets, n, sc, s);
+    if (e != hipSuccess) return e;
+    if (!b->strip)
+        return launch_batched_eval(dd, isa.num_registers, b->tuning, b->ctx->n_cu, sc, d_words,
+                                   d_offsets, 0, n, d_fitness, d_sse, s);
+    e = launch_effective_counts(isa.num_registers, isa.op_set, isa.input_protect, d_words, d_offsets,
+                                n, b->counts.ptr, s, sc.order);
+    if (e == hipSuccess) e = launch_exclusive_scan(b->counts.ptr, n, b->eff_offsets.ptr, b->scan_tmp.ptr, s);
+    if (e == hipSuccess)
+        e = launch_effective_scatter(isa.num_registers, isa.op_set, isa.input_protect, d_words,
+                                     d_offsets, n, b->eff_offsets.ptr, b->eff_words.ptr,
+                                     (int64_t)b->eff_words.cap, s, sc.order);
+    if (e != hipSuccess) return e;
+    return launch_batched_eval(dd, isa.num_registers, b->tuning, b->ctx->n_cu, sc, b->eff_words.ptr,
+                               b->eff_offsets.ptr, 1, n, d_fitness, d_sse, s);
 }
 
 int galop_batch_eval_device(galop_batch* b, const uint32_t* d_words, const int64_t* d_offsets,
@@ -184,7 +187,7 @@ int galop_batch_eval_device(galop_batch* b, const uint32_t* d_words, const int64
 
 int galop_batch_autotune(galop_batch* b, const uint32_t* d_words, const int64_t* d_offsets,
                          int64_t n_programs, float* d_fitness, int32_t* cases_per_lane,
-                         int32_t* waves_per_block, int32_t* workers_per_cu) {
+                         int32_t* waves_per_block, int32_t* workers_per_cu, int32_t* chunk) {
     if (!b) return set_error(nullptr, GALOP_ERR_INVALID, "galop_batch_autotune: NULL plan");
     galop_ctx* ctx = b->ctx;
     if (n_programs <= 0 || n_programs > b->bufs.max_programs || !d_words || !d_offsets || !d_fitness)
@@ -203,6 +206,7 @@ int galop_batch_autotune(galop_batch* b, const uint32_t* d_words, const int64_t*
              ks.end());
     const int wpbs[] = {1, 2, 4};  // kernels are built for at most 256 threads per block
     const int workers[] = {8, 16, 24, 32};
+    const int chunks[] = {1, 4, 16};
     hipEvent_t a, z;
     if ((e = hipEventCreate(&a)) != hipSuccess) return hip_fail(ctx, e, "hipEventCreate");
     if ((e = hipEventCreate(&z)) != hipSuccess) {
@@ -213,12 +217,13 @@ int galop_batch_autotune(galop_batch* b, const uint32_t* d_words, const int64_t*
     BatchTuning best_t = b->tuning;
     for (int k : ks)
         for (int w : wpbs)
-            for (int wk : workers) {
+            for (int wk : workers)
+            for (int ch : chunks) {
                 BatchTuning t;
                 t.cases_per_lane = k;
                 t.waves_per_block = w;
                 t.workers_per_cu = wk;
-                t.chunk = 1;
+                t.chunk = ch;
                 float ms = 0.0f;
                 const BatchTuning saved = b->tuning;
                 b->tuning = t;
@@ -245,6 +250,7 @@ int galop_batch_autotune(galop_batch* b, const uint32_t* d_words, const int64_t*
     if (cases_per_lane) *cases_per_lane = best_t.cases_per_lane;
     if (waves_per_block) *waves_per_block = best_t.waves_per_block;
     if (workers_per_cu) *workers_per_cu = best_t.workers_per_cu;
+    if (chunk) *chunk = best_t.chunk;
     return GALOP_OK;
 }
 

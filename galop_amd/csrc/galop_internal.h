@@ -35,15 +35,19 @@ hipError_t launch_run_cases(const DevData& ds, int num_registers, const uint32_t
 // Fill consts[64..128) with the unary-on-constant table from consts[0..16).
 hipError_t launch_const_table(float* consts, hipStream_t stream);
 
-// Device intron removal: per-program effective counts, scan, scatter.
+// Device intron removal: per-program effective counts, scan, scatter.  With
+// `order`, entry i is program order[i] and counts / eff_offsets are indexed
+// by i; without, by program.
 hipError_t launch_effective_counts(int num_registers, int op_set, int input_protect,
                                    const uint32_t* words, const int64_t* offsets,
-                                   int64_t n, int64_t* counts, hipStream_t stream);
+                                   int64_t n, int64_t* counts, hipStream_t stream,
+                                   const int32_t* order = nullptr);
 // Writes at most `cap` words; offsets beyond `cap` are clamped to it.
 hipError_t launch_effective_scatter(int num_registers, int op_set, int input_protect,
                                     const uint32_t* words, const int64_t* offsets,
                                     int64_t n, int64_t* eff_offsets,
-                                    uint32_t* eff_words, int64_t cap, hipStream_t stream);
+                                    uint32_t* eff_words, int64_t cap, hipStream_t stream,
+                                    const int32_t* order = nullptr);
 // Exclusive scan of n int64 counts into out[0..n] (out[n] = total); tmp must
 // hold scan_temp_elems(n) int64.
 int64_t scan_temp_elems(int64_t n);
@@ -69,6 +73,17 @@ struct BatchTuning {
 
 BatchTuning resolve_batch_tuning(const DevData& ds, int num_registers, BatchTuning t,
                                  int n_cu);
+// Order array: programs by length bucket, longest bucket first; also resets
+// the queue head.
+hipError_t launch_batch_order(const int64_t* offsets, int64_t n, const BatchScratch& s,
+                              hipStream_t stream);
+// Persistent evaluation over the order array (after launch_batch_order).  The
+// CSR `offs` is indexed by order position when by_position, else by program.
+hipError_t launch_batched_eval(const DevData& ds, int num_registers, BatchTuning t, int n_cu,
+                               const BatchScratch& s, const uint32_t* words,
+                               const int64_t* offs, int by_position, int64_t n,
+                               float* fitness, double* sse, hipStream_t stream);
+// launch_batch_order + launch_batched_eval over a CSR indexed by program.
 hipError_t launch_batched(const DevData& ds, int num_registers, BatchTuning t, int n_cu,
                           const BatchScratch& s, const uint32_t* words,
                           const int64_t* offsets, int64_t n, float* fitness,

@@ -54,12 +54,13 @@ __global__ void const_table_kernel(float* consts) {
     const int t = (int)threadIdx.x;  // 64 threads: unary op u on constant idx
     const int u = t / 16, idx = t % 16;
     const float b = consts[idx];
+    const galop_math::CoefPtr C = galop_math::coef_table();
     float v;
     switch (u) {
-    case 0: v = op_sin(b); break;
-    case 1: v = op_cos(b); break;
-    case 2: v = op_exp(b); break;
-    default: v = op_plog(b); break;
+    case 0: v = op_sin(b, C); break;
+    case 1: v = op_cos(b, C); break;
+    case 2: v = op_exp(b, C); break;
+    default: v = op_plog(b, C); break;
     }
     consts[kUnaryTableBase + t] = v;
 }
@@ -141,7 +142,10 @@ hipError_t launch_const_table(float* consts, hipStream_t stream) {
 
 // ---------------------------------------------------------------------------
 // device intron removal (src/Introns.h:28-57): one thread per program walks
-// its program backwards with a live-register bitmask.
+// its program backwards with a live-register bitmask.  With an order array
+// (the bucket order of the batched path) thread i takes program order[i], so
+// the lanes of a wave walk programs of similar length, and the counts and the
+// compacted programs are laid out by order position i.
 // ---------------------------------------------------------------------------
 struct DecodeCfg {
     uint32_t mask;   // NR - 1
@@ -161,34 +165,99 @@ __device__ __forceinline__ bool effective_step(uint32_t w, const DecodeCfg& c, u
     return true;
 }
 
+// Calls f(word) for the words of [lo, hi), last to first, reading the
+// 16-byte aligned blocks that hold them (one load per four words), the next
+// block's load issued before the current block is processed.  A block never
+// crosses a page, so its other words are readable whenever the one in range
+// is; they are not used.
+template <typename F>
+__device__ __forceinline__ void walk_backward(const uint32_t* __restrict__ words, int64_t lo,
+                                              int64_t hi, F f) {
+    if (hi <= lo) return;
+    const uintptr_t last = (uintptr_t)(words + (hi - 1));
+    uintptr_t a = last & ~(uintptr_t)15;
+    int64_t blk = (hi - 1) - (int64_t)((last - a) >> 2);  // word index of the block's first word
+    uint4 v = *reinterpret_cast<const uint4*>(a);
+    int64_t j = hi;
+    for (;;) {
+        const bool more = blk > lo;  // word blk-1 is in range: its block is readable
+        uint4 vn = v;
+        if (more) vn = *reinterpret_cast<const uint4*>(a - 16);
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int t = 3; t >= 0; --t) {
+            const int64_t idx = blk + t;
+            if (idx < j && idx >= lo) f(w[t]);
+        }
+        if (!more) break;
+        j = blk;
+        blk -= 4;
+        a -= 16;
+        v = vn;
+    }
+}
+
 __global__ void effective_count_kernel(DecodeCfg cfg, const uint32_t* __restrict__ words,
-                                       const int64_t* __restrict__ offsets, int64_t n,
+                                       const int64_t* __restrict__ offsets,
+                                       const int32_t* __restrict__ order, int64_t n,
                                        int64_t* __restrict__ counts) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int64_t lo = offsets[i], hi = offsets[i + 1];
+    const int64_t p = order ? (int64_t)order[i] : i;
     uint32_t live = 1u << cfg.out_reg;
     int64_t cnt = 0;
-    for (int64_t j = hi - 1; j >= lo; --j) cnt += effective_step(words[j], cfg, live) ? 1 : 0;
+    walk_backward(words, offsets[p], offsets[p + 1],
+                  [&](uint32_t w) { cnt += effective_step(w, cfg, live) ? 1 : 0; });
     counts[i] = cnt;
 }
 
+// The compacted words go out through a 16-byte staging block: blocks that lie
+// wholly inside this program's range [start, end) are written with one
+// 16-byte store, the partial blocks at either end word by word (their other
+// words belong to neighbouring programs).  A caller's buffer that is not
+// 16-byte aligned is written word by word throughout.
 __global__ void effective_scatter_kernel(DecodeCfg cfg, const uint32_t* __restrict__ words,
-                                         const int64_t* __restrict__ offsets, int64_t n,
+                                         const int64_t* __restrict__ offsets,
+                                         const int32_t* __restrict__ order, int64_t n,
                                          const int64_t* __restrict__ eff_offsets,
                                          uint32_t* __restrict__ eff_words, int64_t cap) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const int64_t lo = offsets[i], hi = offsets[i + 1];
-    int64_t pos = eff_offsets[i + 1];  // fill from the back
-    uint32_t live = 1u << cfg.out_reg;
-    for (int64_t j = hi - 1; j >= lo; --j) {
-        const uint32_t w = words[j];
-        if (effective_step(w, cfg, live)) {
-            --pos;
-            if (pos < cap) eff_words[pos] = w;  // never write past the buffer
+    const int64_t p = order ? (int64_t)order[i] : i;
+    const int64_t start = eff_offsets[i];
+    const int64_t end = eff_offsets[i + 1];
+    const int64_t lim = end < cap ? end : cap;  // never write past the buffer
+    int64_t pos = end;                          // fill from the back
+    int64_t sblk = -1;                          // staged block (first word index)
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    const bool vec_ok = ((uintptr_t)eff_words & 15) == 0;
+    auto flush = [&]() {
+        if (sblk < 0) return;
+        if (vec_ok && sblk >= start && sblk + 4 <= lim) {
+            *reinterpret_cast<uint4*>(eff_words + sblk) = make_uint4(s0, s1, s2, s3);
+        } else {
+            const uint32_t sv[4] = {s0, s1, s2, s3};
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (sblk + t >= start && sblk + t < lim) eff_words[sblk + t] = sv[t];
         }
-    }
+    };
+    uint32_t live = 1u << cfg.out_reg;
+    walk_backward(words, offsets[p], offsets[p + 1], [&](uint32_t w) {
+        if (!effective_step(w, cfg, live)) return;
+        --pos;
+        const int64_t b = pos & ~(int64_t)3;
+        if (b != sblk) {
+            flush();
+            sblk = b;
+        }
+        const int o = (int)(pos & 3);
+        s0 = o == 0 ? w : s0;
+        s1 = o == 1 ? w : s1;
+        s2 = o == 2 ? w : s2;
+        s3 = o == 3 ? w : s3;
+    });
+    flush();
 }
 
 // Offsets past the effective-word buffer are cut to its end, so that a
@@ -209,22 +278,23 @@ static DecodeCfg make_cfg(int nr, int op_set, int iprot) {
 
 hipError_t launch_effective_counts(int num_registers, int op_set, int input_protect,
                                    const uint32_t* words, const int64_t* offsets, int64_t n,
-                                   int64_t* counts, hipStream_t stream) {
+                                   int64_t* counts, hipStream_t stream, const int32_t* order) {
     if (n <= 0) return hipSuccess;
     const unsigned grid = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(effective_count_kernel, dim3(grid), dim3(256), 0, stream,
-                       make_cfg(num_registers, op_set, input_protect), words, offsets, n, counts);
+                       make_cfg(num_registers, op_set, input_protect), words, offsets, order, n,
+                       counts);
     return hipGetLastError();
 }
 
 hipError_t launch_effective_scatter(int num_registers, int op_set, int input_protect,
                                     const uint32_t* words, const int64_t* offsets, int64_t n,
                                     int64_t* eff_offsets, uint32_t* eff_words, int64_t cap,
-                                    hipStream_t stream) {
+                                    hipStream_t stream, const int32_t* order) {
     if (n <= 0) return hipSuccess;
     const unsigned grid = (unsigned)((n + 255) / 256);
     hipLaunchKernelGGL(effective_scatter_kernel, dim3(grid), dim3(256), 0, stream,
-                       make_cfg(num_registers, op_set, input_protect), words, offsets, n,
+                       make_cfg(num_registers, op_set, input_protect), words, offsets, order, n,
                        eff_offsets, eff_words, cap);
     hipLaunchKernelGGL(clamp_offsets_kernel, dim3((unsigned)((n + 256) / 256)), dim3(256), 0, stream,
                        eff_offsets, n + 1, cap);
@@ -367,59 +437,84 @@ __global__ void batch_bucket_start_kernel(const int32_t* __restrict__ bucket_cou
     }
 }
 
-__global__ void batch_scatter_kernel(const int64_t* __restrict__ offsets, int64_t n,
-                                     int32_t* __restrict__ bucket_cursor,
-                                     int32_t* __restrict__ order) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int b = length_bucket(offsets[i + 1] - offsets[i]);
-    order[atomicAdd(&bucket_cursor[b], 1)] = (int32_t)i;
+// Order array by bucket.  Each block ranks its tile of programs inside the
+// tile with LDS atomics, then reserves one range per bucket with a single
+// global atomic, so the global counters see at most kBuckets atomics per
+// tile rather than one per program.
+constexpr int kScatterThreads = 256;
+constexpr int kScatterItems = 16;
+constexpr int64_t kScatterTile = (int64_t)kScatterThreads * kScatterItems;
+
+__global__ void __launch_bounds__(kScatterThreads)
+batch_scatter_kernel(const int64_t* __restrict__ offsets, int64_t n,
+                     int32_t* __restrict__ bucket_cursor, int32_t* __restrict__ order) {
+    __shared__ int32_t cnt[kBuckets];
+    __shared__ int32_t base[kBuckets];
+    const int t = (int)threadIdx.x;
+    if (t < kBuckets) cnt[t] = 0;
+    __syncthreads();
+    const int64_t first = (int64_t)blockIdx.x * kScatterTile;
+    int bucket[kScatterItems], rank[kScatterItems];
+#pragma unroll
+    for (int it = 0; it < kScatterItems; ++it) {
+        const int64_t i = first + (int64_t)it * kScatterThreads + t;
+        bucket[it] = 0;
+        rank[it] = 0;
+        if (i < n) {
+            bucket[it] = length_bucket(offsets[i + 1] - offsets[i]);
+            rank[it] = atomicAdd(&cnt[bucket[it]], 1);
+        }
+    }
+    __syncthreads();
+    if (t < kBuckets && cnt[t]) base[t] = atomicAdd(&bucket_cursor[t], cnt[t]);
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kScatterItems; ++it) {
+        const int64_t i = first + (int64_t)it * kScatterThreads + t;
+        if (i < n) order[base[bucket[it]] + rank[it]] = (int32_t)i;
+    }
 }
 
-// Persistent evaluation: each wave repeatedly takes `chunk` entries of the
-// order array from the queue head and evaluates them.  Every wave leaves the
-// loop once the head passes n, so the grid always drains.
-template <int NR, int K, int EXT, bool RESIDENT>
+__device__ __forceinline__ int64_t readlane_i64(int64_t v, int j) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)((uint64_t)v >> 32), j);
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
+// Persistent evaluation: each wave repeatedly takes `chunk` (<= 64) entries of
+// the order array from the queue head, loads their program ids and CSR
+// bounds one entry per lane, and evaluates them one after the other.  With
+// by_position the CSR is indexed by order position (the compacted programs of
+// the intron-removing generation), otherwise by program id.  Every wave
+// leaves the loop once the head passes n, so the grid always drains.
+template <int NR, int K, int EXT>
 __global__ void __launch_bounds__(256)
-batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ offsets,
+batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ offs,
                const int32_t* __restrict__ order, int32_t* __restrict__ queue_head,
-               int64_t n, int chunk, DevData ds, float* __restrict__ fitness,
+               int64_t n, int chunk, int by_position, DevData ds, float* __restrict__ fitness,
                double* __restrict__ sse_out) {
     const int lane = (int)(threadIdx.x & (kWave - 1));
     const ConstRegs cr = load_const_regs(ds.consts, lane);
-    const DestDecode dd = dest_decode(NR, ds.input_protect);
-    const int out_reg = ds.out_reg;
-
-    // One-tile datasets stay in VGPRs for the life of the wave.
-    RegFile<NR, K> init;
-    float y[K];
-    if (RESIDENT) {
-        load_cases<NR, K>(ds, 0, lane, init);
-#pragma unroll
-        for (int k = 0; k < K; ++k) y[k] = ds.targets[k * kWave + lane];
-    }
     for (;;) {
         int base = 0;
         if (lane == 0) base = atomicAdd(queue_head, chunk);
         base = __builtin_amdgcn_readfirstlane(__shfl(base, 0, kWave));
         if ((int64_t)base >= n) break;
-        const int64_t end = ((int64_t)base + chunk) < n ? ((int64_t)base + chunk) : n;
-        for (int64_t q = base; q < end; ++q) {
-            const int64_t p = order[q];
-            const int64_t lo = offsets[p], hi = offsets[p + 1];
-            double sse;
-            if (RESIDENT) {
-                RegFile<NR, K> r = init;
-                run_program<NR, K, EXT>(words + lo, hi - lo, lane, r, cr, dd);
-                LaneAcc acc;
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    if ((int64_t)(k * kWave + lane) < ds.n_cases) acc.add(r.get(k, out_reg), y[k]);
-                }
-                sse = wave_sum_f64(acc.total());
-            } else {
-                sse = eval_program_wave<NR, K, EXT>(words + lo, hi - lo, ds, lane, cr, nullptr);
-            }
+        const int cnt = (int)(((int64_t)base + chunk) < n ? chunk : (n - base));
+        int32_t p_l = 0;
+        int64_t lo_l = 0, hi_l = 0;
+        if (lane < cnt) {
+            const int64_t q = (int64_t)base + lane;
+            p_l = order[q];
+            const int64_t e = by_position ? q : (int64_t)p_l;
+            lo_l = offs[e];
+            hi_l = offs[e + 1];
+        }
+        for (int j = 0; j < cnt; ++j) {
+            const int64_t p = __builtin_amdgcn_readlane(p_l, j);
+            const int64_t lo = readlane_i64(lo_l, j);
+            const int64_t hi = readlane_i64(hi_l, j);
+            const double sse = eval_program_wave<NR, K, EXT>(words + lo, hi - lo, ds, lane, cr, nullptr);
             if (lane == 0) {
                 fitness[p] = sse_to_fitness(sse, ds.n_cases);
                 if (sse_out) sse_out[p] = sse;
@@ -438,36 +533,29 @@ BatchTuning resolve_batch_tuning(const DevData& ds, int num_registers, BatchTuni
     if (num_registers == 16 && t.cases_per_lane > 4) t.cases_per_lane = 4;
     if (t.waves_per_block <= 0 || t.waves_per_block > 4) t.waves_per_block = 4;
     if (t.workers_per_cu <= 0) t.workers_per_cu = 32;
-    if (t.chunk <= 0) t.chunk = 1;
+    if (t.chunk <= 0) t.chunk = 4;
+    if (t.chunk > kWave) t.chunk = kWave;
     return t;
 }
 
 template <int NR, int K, int EXT>
 static hipError_t launch_batched_t(const DevData& ds, const BatchTuning& t, int n_cu,
                                    const BatchScratch& s, const uint32_t* words,
-                                   const int64_t* offsets, int64_t n, float* fitness,
-                                   double* sse, hipStream_t stream) {
+                                   const int64_t* offs, int by_position, int64_t n,
+                                   float* fitness, double* sse, hipStream_t stream) {
     const int wpb = t.waves_per_block;
     const int64_t waves = (int64_t)n_cu * t.workers_per_cu;
     const unsigned grid = (unsigned)((waves + wpb - 1) / wpb);
-    const bool resident = ds.n_cases <= (int64_t)kWave * K;
-    if (resident)
-        hipLaunchKernelGGL((batched_kernel<NR, K, EXT, true>), dim3(grid), dim3(wpb * kWave), 0,
-                           stream, words, offsets, s.order, s.queue_head, n, t.chunk, ds,
-                           fitness, sse);
-    else
-        hipLaunchKernelGGL((batched_kernel<NR, K, EXT, false>), dim3(grid), dim3(wpb * kWave), 0,
-                           stream, words, offsets, s.order, s.queue_head, n, t.chunk, ds,
-                           fitness, sse);
+    hipLaunchKernelGGL((batched_kernel<NR, K, EXT>), dim3(grid), dim3(wpb * kWave), 0, stream,
+                       words, offs, s.order, s.queue_head, n, t.chunk, by_position, ds, fitness,
+                       sse);
     return hipGetLastError();
 }
 
-hipError_t launch_batched(const DevData& ds, int num_registers, BatchTuning t, int n_cu,
-                          const BatchScratch& s, const uint32_t* words, const int64_t* offsets,
-                          int64_t n, float* fitness, double* sse, hipStream_t stream) {
+hipError_t launch_batch_order(const int64_t* offsets, int64_t n, const BatchScratch& s,
+                              hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     if (n > s.max_programs || n > (int64_t)INT32_MAX - (int64_t(1) << 22)) return hipErrorInvalidValue;
-    t = resolve_batch_tuning(ds, num_registers, t, n_cu);
     hipLaunchKernelGGL(batch_reset_kernel, dim3(1), dim3(kBuckets), 0, stream, s.bucket_count,
                        s.bucket_cursor, s.queue_head);
     const unsigned hgrid = (unsigned)(((n + 255) / 256) < 1024 ? ((n + 255) / 256) : 1024);
@@ -475,18 +563,35 @@ hipError_t launch_batched(const DevData& ds, int num_registers, BatchTuning t, i
                        s.bucket_count);
     hipLaunchKernelGGL(batch_bucket_start_kernel, dim3(1), dim3(64), 0, stream, s.bucket_count,
                        s.bucket_cursor);
-    hipLaunchKernelGGL(batch_scatter_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
-                       stream, offsets, n, s.bucket_cursor, s.order);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(batch_scatter_kernel, dim3((unsigned)((n + kScatterTile - 1) / kScatterTile)),
+                       dim3(kScatterThreads), 0, stream, offsets, n, s.bucket_cursor, s.order);
+    return hipGetLastError();
+}
+
+hipError_t launch_batched_eval(const DevData& ds, int num_registers, BatchTuning t, int n_cu,
+                               const BatchScratch& s, const uint32_t* words, const int64_t* offs,
+                               int by_position, int64_t n, float* fitness, double* sse,
+                               hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    if (n > s.max_programs || n > (int64_t)INT32_MAX - (int64_t(1) << 22)) return hipErrorInvalidValue;
+    t = resolve_batch_tuning(ds, num_registers, t, n_cu);
 #define GALOP_CASE(NRV, KV)                                                             \
     if (num_registers == NRV && t.cases_per_lane == KV)                                 \
-        return ds.op_set ? launch_batched_t<NRV, KV, 1>(ds, t, n_cu, s, words, offsets, n, fitness, sse, stream) \
-                         : launch_batched_t<NRV, KV, 0>(ds, t, n_cu, s, words, offsets, n, fitness, sse, stream);
+        return ds.op_set ? launch_batched_t<NRV, KV, 1>(ds, t, n_cu, s, words, offs, by_position, n, fitness, sse, stream) \
+                         : launch_batched_t<NRV, KV, 0>(ds, t, n_cu, s, words, offs, by_position, n, fitness, sse, stream);
     GALOP_CASE(8, 1) GALOP_CASE(8, 2) GALOP_CASE(8, 4) GALOP_CASE(8, 8)
     GALOP_CASE(16, 1) GALOP_CASE(16, 2) GALOP_CASE(16, 4)
 #undef GALOP_CASE
     return hipErrorInvalidValue;
+}
+
+hipError_t launch_batched(const DevData& ds, int num_registers, BatchTuning t, int n_cu,
+                          const BatchScratch& s, const uint32_t* words, const int64_t* offsets,
+                          int64_t n, float* fitness, double* sse, hipStream_t stream) {
+    hipError_t e = launch_batch_order(offsets, n, s, stream);
+    if (e != hipSuccess) return e;
+    return launch_batched_eval(ds, num_registers, t, n_cu, s, words, offsets, 0, n, fitness, sse,
+                               stream);
 }
 
 }  // namespace galop

@@ -74,10 +74,18 @@ __device__ __forceinline__ float op_div(float a, float b) {
 }
 // Unary ops on src2 (src/ISA.h:225-226 and the EXT8 table), evaluated in FP64
 // and rounded once (lgp_math.h).
-__device__ __forceinline__ float op_sin(float b) { return galop_math::sinf_r(b); }
-__device__ __forceinline__ float op_cos(float b) { return galop_math::cosf_r(b); }
-__device__ __forceinline__ float op_exp(float b) { return galop_math::expf_r(b); }
-__device__ __forceinline__ float op_plog(float b) { return galop_math::plogf_r(b); }
+__device__ __forceinline__ float op_sin(float b, galop_math::CoefPtr C) {
+    return galop_math::sinf_r(b, C);
+}
+__device__ __forceinline__ float op_cos(float b, galop_math::CoefPtr C) {
+    return galop_math::cosf_r(b, C);
+}
+__device__ __forceinline__ float op_exp(float b, galop_math::CoefPtr C) {
+    return galop_math::expf_r(b, C);
+}
+__device__ __forceinline__ float op_plog(float b, galop_math::CoefPtr C) {
+    return galop_math::plogf_r(b, C);
+}
 
 // The register files of the K cases of a lane, as one or two vector values
 // of at most 32 floats (register i of case k lives in group k / kPer
@@ -174,9 +182,10 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
             const float v = wave_bcast(cr.table, (U) * 16 + s2);          \
             _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, v);    \
         } else {                                                          \
+            const galop_math::CoefPtr C = galop_math::coef_table();       \
             float bv[K], cv[K];                                           \
             _Pragma("unroll") for (int k = 0; k < K; ++k) bv[k] = r.get(k, s2); \
-            _Pragma("unroll") for (int k = 0; k < K; ++k) cv[k] = FN(bv[k]); \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) cv[k] = FN(bv[k], C); \
             _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]); \
         }                                                                 \
     } while (0)
@@ -221,15 +230,22 @@ __device__ __forceinline__ void run_program(const uint32_t* __restrict__ prog,
 
 // Load the K cases of tile `tile` for this lane into registers
 // (src/Interpreter.cpp:43-52).
+// Addresses are a wave-uniform column pointer (SGPRs) plus the lane offset,
+// so no per-lane 64-bit pointers stay live across the program loop.
 template <int NR, int K>
 __device__ __forceinline__ void load_cases(const DevData& ds, int64_t tile,
                                            int lane, RegFile<NR, K>& r) {
+    const int64_t first = tile * (int64_t)(kWave * K);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int64_t c = tile * (int64_t)(kWave * K) + k * kWave + lane;
+    for (int i = 0; i < NR; ++i) {
+        if (i < ds.n_inputs) {
+            const float* __restrict__ col = ds.inputs + (int64_t)i * ds.n_pad + first;
 #pragma unroll
-        for (int i = 0; i < NR; ++i)
-            r.set(k, i, (i < ds.n_inputs) ? ds.inputs[(int64_t)i * ds.n_pad + c] : 0.0f);
+            for (int k = 0; k < K; ++k) r.set(k, i, col[k * kWave + lane]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k) r.set(k, i, 0.0f);
+        }
     }
 }
 
@@ -296,13 +312,14 @@ __device__ __forceinline__ double eval_program_wave(const uint32_t* __restrict__
         RegFile<NR, K> r;
         load_cases<NR, K>(ds, t, lane, r);
         run_program<NR, K, EXT>(prog, len, lane, r, cr, dd);
+        const int64_t first = t * tile_cases;
+        const float* __restrict__ tgt = ds.targets + first;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const int64_t c = t * tile_cases + k * kWave + lane;
-            if (c < ds.n_cases) {
+            if (first + k * kWave + lane < ds.n_cases) {
                 const float out = r.get(k, out_reg);
-                if (case_out) case_out[c] = out;
-                else acc.add(out, ds.targets[c]);
+                if (case_out) case_out[first + k * kWave + lane] = out;
+                else acc.add(out, tgt[k * kWave + lane]);
             }
         }
     }

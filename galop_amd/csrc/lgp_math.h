@@ -77,6 +77,51 @@ constexpr double LOG_C[9] = {
 GM_CONST uint32_t TWO_OVER_PI_W[8] = {0xA2F9836Eu, 0x4E441529u, 0xFC2757D1u, 0xF534DDC0u,
                                       0xDB629599u, 0x3C439041u, 0xFE5163ABu, 0xDEBBC561u};
 
+// Every FP64 constant above in one table; the functions below take it as an
+// argument.  On the device the table sits in constant memory and is reached
+// through coef_table(), a pointer the compiler cannot see through, so each
+// instruction word of the interpreter reads the coefficients it needs with a
+// few scalar loads into SGPRs.  Left as literals, the ~50 doubles are hoisted
+// out of the interpreter loop into SGPR and VGPR pairs (a VOP3 instruction on
+// gfx950 takes no 64-bit literal), about 60 VGPRs that cost occupancy.
+enum CoefSlot : int {
+    kExpC = 0,        // EXP_C[0..12]
+    kSinC = 13,       // SIN_C[0..7]
+    kCosC = 21,       // COS_C[0..8]
+    kLogC = 30,       // LOG_C[0..8]
+    kPio2 = 39,       // PIO2_0, PIO2_1, PIO2_2
+    kTwoOverPi = 42,
+    kPio2Scaled62 = 43,
+    kLn2 = 44,        // LN2_0, LN2_1, LN2_2
+    kInvLn2 = 47,
+    kSqrtHalf = 48,
+    kCoefCount = 49
+};
+#define GM_COEF_INIT                                                              \
+    {EXP_C[0], EXP_C[1], EXP_C[2], EXP_C[3], EXP_C[4], EXP_C[5], EXP_C[6],        \
+     EXP_C[7], EXP_C[8], EXP_C[9], EXP_C[10], EXP_C[11], EXP_C[12],               \
+     SIN_C[0], SIN_C[1], SIN_C[2], SIN_C[3], SIN_C[4], SIN_C[5], SIN_C[6], SIN_C[7], \
+     COS_C[0], COS_C[1], COS_C[2], COS_C[3], COS_C[4], COS_C[5], COS_C[6], COS_C[7], \
+     COS_C[8],                                                                    \
+     LOG_C[0], LOG_C[1], LOG_C[2], LOG_C[3], LOG_C[4], LOG_C[5], LOG_C[6], LOG_C[7], \
+     LOG_C[8],                                                                    \
+     PIO2_0, PIO2_1, PIO2_2, TWO_OVER_PI, PIO2_SCALED62,                          \
+     LN2_0, LN2_1, LN2_2, INV_LN2, SQRT_HALF}
+
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef const __attribute__((address_space(4))) double* CoefPtr;
+__device__ __constant__ double COEF_TABLE[kCoefCount] = GM_COEF_INIT;
+__device__ __forceinline__ CoefPtr coef_table() {
+    CoefPtr p = (CoefPtr)COEF_TABLE;
+    __asm__ volatile("" : "+s"(p));
+    return p;
+}
+#else
+typedef const double* CoefPtr;
+static const double COEF_TABLE[kCoefCount] = GM_COEF_INIT;
+static inline CoefPtr coef_table() { return COEF_TABLE; }
+#endif
+
 GM_HD double fma_d(double a, double b, double c) { return __builtin_fma(a, b, c); }
 GM_HD double rint_d(double a) { return __builtin_rint(a); }
 
@@ -92,57 +137,57 @@ GM_HD float u2f(uint32_t u) {
 }
 
 // ---- exp -------------------------------------------------------------------
-GM_HD float expf_r(float x) {
+GM_HD float expf_r(float x, CoefPtr C) {
     if (x != x) return x;
     if (x > 89.0f) return u2f(0x7f800000u);   // exp(89) > FLT_MAX
     if (x < -104.0f) return 0.0f;             // exp(-104) < half the least subnormal
     const double xd = (double)x;
-    const double k = rint_d(xd * INV_LN2);
-    double r = fma_d(-k, LN2_0, xd);
-    r = fma_d(-k, LN2_1, r);
-    r = fma_d(-k, LN2_2, r);
-    double p = EXP_C[12];
-    for (int n = 11; n >= 0; --n) p = fma_d(p, r, EXP_C[n]);
+    const double k = rint_d(xd * C[kInvLn2]);
+    double r = fma_d(-k, C[kLn2], xd);
+    r = fma_d(-k, C[kLn2 + 1], r);
+    r = fma_d(-k, C[kLn2 + 2], r);
+    double p = C[kExpC + 12];
+    for (int n = 11; n >= 0; --n) p = fma_d(p, r, C[kExpC + n]);
     return (float)ldexp(p, (int)k);
 }
 
 // ---- log of a positive, finite, non-zero float --------------------------
-GM_HD float logf_pos(float x) {
+GM_HD float logf_pos(float x, CoefPtr C) {
     int e;
     double m = frexp((double)x, &e);  // m in [0.5, 1)
-    if (m < SQRT_HALF) {
+    if (m < C[kSqrtHalf]) {
         m *= 2.0;
         e -= 1;
     }
     const double s = (m - 1.0) / (m + 1.0);
     const double s2 = s * s;
-    double p = LOG_C[8];
-    for (int k = 7; k >= 0; --k) p = fma_d(p, s2, LOG_C[k]);
+    double p = C[kLogC + 8];
+    for (int k = 7; k >= 0; --k) p = fma_d(p, s2, C[kLogC + k]);
     const double lm = p * s;
     const double ed = (double)e;
-    return (float)fma_d(ed, LN2_0, fma_d(ed, LN2_1, lm));
+    return (float)fma_d(ed, C[kLn2], fma_d(ed, C[kLn2 + 1], lm));
 }
 
 // protected log of the interpreter's EXT8 table: b != 0 ? log|b| : 0
-GM_HD float plogf_r(float b) {
+GM_HD float plogf_r(float b, CoefPtr C) {
     if (b != b) return b;
     if (b == 0.0f) return 0.0f;
     const float a = u2f(f2u(b) & 0x7fffffffu);
     if (a == u2f(0x7f800000u)) return a;
-    return logf_pos(a);
+    return logf_pos(a, C);
 }
 
 // ---- sin / cos ---------------------------------------------------------------
 // Reduce a finite x to r in [-pi/4, pi/4] and a quadrant q with x = q pi/2 + r (mod 2 pi).
-GM_HD double reduce_pio2(float x, int& q) {
+GM_HD double reduce_pio2(float x, int& q, CoefPtr C) {
     const double xd = (double)x;
     const uint32_t u = f2u(x);
     const uint32_t ex = (u >> 23) & 0xffu;
     if (ex < 127u + 25u) {  // |x| < 2^25: Cody-Waite
-        const double k = rint_d(xd * TWO_OVER_PI);
-        double r = fma_d(-k, PIO2_0, xd);
-        r = fma_d(-k, PIO2_1, r);
-        r = fma_d(-k, PIO2_2, r);
+        const double k = rint_d(xd * C[kTwoOverPi]);
+        double r = fma_d(-k, C[kPio2], xd);
+        r = fma_d(-k, C[kPio2 + 1], r);
+        r = fma_d(-k, C[kPio2 + 2], r);
         q = ((int)k) & 3;
         return r;
     }
@@ -172,7 +217,7 @@ GM_HD double reduce_pio2(float x, int& q) {
         F -= int64_t(1) << 62;
         q += 1;
     }
-    double r = (double)F * PIO2_SCALED62;
+    double r = (double)F * C[kPio2Scaled62];
     if (xd < 0.0) {
         r = -r;
         q = -q;
@@ -181,46 +226,46 @@ GM_HD double reduce_pio2(float x, int& q) {
     return r;
 }
 
-GM_HD double sin_poly(double r) {
+GM_HD double sin_poly(double r, CoefPtr C) {
     const double r2 = r * r;
-    double p = SIN_C[7];
-    for (int k = 6; k >= 1; --k) p = fma_d(p, r2, SIN_C[k]);
+    double p = C[kSinC + 7];
+    for (int k = 6; k >= 1; --k) p = fma_d(p, r2, C[kSinC + k]);
     return fma_d(r * r2, p, r);
 }
 
-GM_HD double cos_poly(double r) {
+GM_HD double cos_poly(double r, CoefPtr C) {
     const double r2 = r * r;
-    double p = COS_C[8];
-    for (int k = 7; k >= 1; --k) p = fma_d(p, r2, COS_C[k]);
+    double p = C[kCosC + 8];
+    for (int k = 7; k >= 1; --k) p = fma_d(p, r2, C[kCosC + k]);
     return fma_d(r2, p, 1.0);
 }
 
 // sin x (want_cos = 0) or cos x (want_cos = 1) from the reduced argument.
 // Both series are evaluated and the result selected per lane: the quadrant
 // differs from lane to lane, and a branch on it would run both anyway.
-GM_HD double sincos_from_reduced(double r, int q, int want_cos) {
-    const double s = sin_poly(r);
-    const double c = cos_poly(r);
+GM_HD double sincos_from_reduced(double r, int q, int want_cos, CoefPtr C) {
+    const double s = sin_poly(r, C);
+    const double c = cos_poly(r, C);
     const int qq = q + want_cos;  // cos x = sin(x + pi/2)
     const double v = (qq & 1) ? c : s;
     return (qq & 2) ? -v : v;
 }
 
-GM_HD float sinf_r(float x) {
+GM_HD float sinf_r(float x, CoefPtr C) {
     if (x == 0.0f) return x;  // keeps the sign of zero
     const float ax = u2f(f2u(x) & 0x7fffffffu);
     if (!(ax < u2f(0x7f800000u))) return x - x;  // inf, NaN -> NaN
     int q;
-    const double r = reduce_pio2(x, q);
-    return (float)sincos_from_reduced(r, q, 0);
+    const double r = reduce_pio2(x, q, C);
+    return (float)sincos_from_reduced(r, q, 0, C);
 }
 
-GM_HD float cosf_r(float x) {
+GM_HD float cosf_r(float x, CoefPtr C) {
     const float ax = u2f(f2u(x) & 0x7fffffffu);
     if (!(ax < u2f(0x7f800000u))) return x - x;
     int q;
-    const double r = reduce_pio2(x, q);
-    return (float)sincos_from_reduced(r, q, 1);
+    const double r = reduce_pio2(x, q, C);
+    return (float)sincos_from_reduced(r, q, 1, C);
 }
 
 }  // namespace galop_math

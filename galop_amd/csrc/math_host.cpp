@@ -4,17 +4,18 @@
 #include "lgp_math.h"
 
 extern "C" {
-float gm_sinf(float x) { return galop_math::sinf_r(x); }
-float gm_cosf(float x) { return galop_math::cosf_r(x); }
-float gm_expf(float x) { return galop_math::expf_r(x); }
-float gm_plogf(float x) { return galop_math::plogf_r(x); }
+float gm_sinf(float x) { return galop_math::sinf_r(x, galop_math::coef_table()); }
+float gm_cosf(float x) { return galop_math::cosf_r(x, galop_math::coef_table()); }
+float gm_expf(float x) { return galop_math::expf_r(x, galop_math::coef_table()); }
+float gm_plogf(float x) { return galop_math::plogf_r(x, galop_math::coef_table()); }
 void gm_map(int op, const float* x, float* y, long n) {
+    const galop_math::CoefPtr C = galop_math::coef_table();
     for (long i = 0; i < n; ++i) {
         switch (op) {
-        case 4: y[i] = galop_math::sinf_r(x[i]); break;
-        case 5: y[i] = galop_math::cosf_r(x[i]); break;
-        case 6: y[i] = galop_math::expf_r(x[i]); break;
-        default: y[i] = galop_math::plogf_r(x[i]); break;
+        case 4: y[i] = galop_math::sinf_r(x[i], C); break;
+        case 5: y[i] = galop_math::cosf_r(x[i], C); break;
+        case 6: y[i] = galop_math::expf_r(x[i], C); break;
+        default: y[i] = galop_math::plogf_r(x[i], C); break;
         }
     }
 }

@@ -252,10 +252,12 @@ class BatchPlan:
         k = ctypes.c_int32()
         w = ctypes.c_int32()
         c = ctypes.c_int32()
+        ch = ctypes.c_int32()
         check(self.lib.galop_batch_autotune(self.handle, _dptr(d_words), _dptr(d_offsets), int(n),
                                             _dptr(d_fitness), ctypes.byref(k), ctypes.byref(w),
-                                            ctypes.byref(c)), self.ds.ctx.handle)
-        return {"cases_per_lane": k.value, "waves_per_block": w.value, "workers_per_cu": c.value}
+                                            ctypes.byref(c), ctypes.byref(ch)), self.ds.ctx.handle)
+        return {"cases_per_lane": k.value, "waves_per_block": w.value, "workers_per_cu": c.value,
+                "chunk": ch.value}
 
     def close(self):
         if self.handle:

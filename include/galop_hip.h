@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define GALOP_ABI_VERSION 1
+#define GALOP_ABI_VERSION 2
 
 /* ---- status codes ---- */
 #define GALOP_OK            0
@@ -156,12 +156,14 @@ int  galop_batch_eval_device(galop_batch* b, const uint32_t* d_words,
                              const int64_t* d_offsets, int64_t n_programs,
                              float* d_fitness, double* d_sse, int32_t use_graph,
                              void* stream);
-/* Autotune (cases-per-lane x waves-per-block x program staging) on the given
- * population; the winner is kept in the plan.  Writes the chosen values. */
+/* Autotune (cases per lane x waves per block x persistent waves per CU x
+ * programs per queue grab) on a prefix of the given population; the winner is
+ * kept in the plan.  Writes the chosen values (any output may be NULL). */
 int  galop_batch_autotune(galop_batch* b, const uint32_t* d_words,
                           const int64_t* d_offsets, int64_t n_programs,
                           float* d_fitness, int32_t* cases_per_lane,
-                          int32_t* waves_per_block, int32_t* workers_per_cu);
+                          int32_t* waves_per_block, int32_t* workers_per_cu,
+                          int32_t* chunk);
 
 /* ---- serving: many evolution runs, continuous batching -----------------
  * A server keeps datasets and populations resident, admits requests while
