@@ -28,6 +28,22 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 FP32_VALU_PEAK_TFLOPS = 157.3  # MI355X published dense FP32 (vector) peak
+CLOCK_HZ = 2.4e9               # MI355X peak engine clock
+SIMDS_PER_CU = 4               # one wave64 VALU instruction per SIMD per 4 clocks
+
+# One plain_kernel<8,8,EXT8> launch on the default plain workload (1,048,576
+# programs, seed 0), measured with rocprofv3 --pmc; the commands and raw
+# figures are in profiles/plain_kernel_plain_workload_counters.md.  FETCH_SIZE
+# and WRITE_SIZE are scaled by what each reads for a 1 GiB device copy
+# (1,048,576 KiB copied: FETCH_SIZE 524,307 KiB, WRITE_SIZE 1,048,576 KiB).
+# They describe that exact population only.
+PLAIN_WORKLOAD_COUNTERS = {
+    "fetch_bytes": 319367.1 * 1024 * (1048576 / 524307.38),
+    "write_bytes": 16384.0 * 1024 * 1.0,
+    "valu_insts": 32691707433,
+    "salu_insts": 11223631956,
+    "valu_busy_pct": 83.2,
+}
 
 
 def log(*a):
@@ -232,6 +248,13 @@ def main():
     props = torch.cuda.get_device_properties(0)
     n_cu = props.multi_processor_count
     achieved_tflops = ops / (ms / 1e3) / 1e12
+    # algorithmic bytes of one launch: every word, offset and output once,
+    # the dataset once
+    dset = wl.dds.dataset
+    alg_bytes = (4 * wl.raw_words + 8 * (P + 1) + 4 * P + 8 * P +
+                 4 * dset.N * (dset.num_inputs + 1))
+    default_plain = (args.workload == "plain" and not args.programs and args.kernel == "plain"
+                     and args.cases_per_lane in (0, 8))
     line = {
         "metric": "GPop/s fitness eval (batched vs plain kernel, 1e-6 rel); p50/p99 eval latency",
         "value": round(gpops, 3),
@@ -257,11 +280,27 @@ def main():
             "unit": "TFLOP/s",
             "frac": round(achieved_tflops / FP32_VALU_PEAK_TFLOPS, 5),
             "traffic": None,
+            "algorithmic_bytes": alg_bytes,
             "note": "1 GP-op = 1 algorithmic FP32 op; kernel time from HIP events on the launch "
                     f"stream; {n_cu} CUs",
-            "frac_of_lane_ops_peak": round(ops / (ms / 1e3) / (n_cu * 64 * 2.4e9), 5),
+            "lane_ops_peak_T": round(n_cu * 64 * CLOCK_HZ / 1e12, 2),
+            "frac_of_lane_ops_peak": round(ops / (ms / 1e3) / (n_cu * 64 * CLOCK_HZ), 5),
         },
     }
+    if default_plain:
+        c = PLAIN_WORKLOAD_COUNTERS
+        traffic = c["fetch_bytes"] + c["write_bytes"]
+        issue_peak = n_cu * SIMDS_PER_CU * CLOCK_HZ / 4
+        issue = c["valu_insts"] / (ms / 1e3)
+        line["roofline"]["traffic"] = round(traffic)
+        line["roofline"]["traffic_over_algorithmic"] = round(traffic / alg_bytes, 3)
+        line["roofline"]["hbm_GBps"] = round(traffic / (ms / 1e3) / 1e9, 1)
+        line["roofline"]["valu_issue"] = {
+            "insts_per_launch": c["valu_insts"], "achieved_per_s": round(issue / 1e9, 1),
+            "peak_per_s": round(issue_peak / 1e9, 1), "unit": "G wave-instructions/s",
+            "frac": round(issue / issue_peak, 4), "valu_busy_pct_counter": c["valu_busy_pct"],
+            "valu_insts_per_gp_op": round(c["valu_insts"] * 64 / ops, 2),
+        }
     if not args.no_cpu_baseline:
         g = fitness.cpu().numpy()
         threads = os.cpu_count() or 1

@@ -546,8 +546,12 @@ static hipError_t launch_batched_t(const DevData& ds, const BatchTuning& t, int 
     const int wpb = t.waves_per_block;
     const int64_t waves = (int64_t)n_cu * t.workers_per_cu;
     const unsigned grid = (unsigned)((waves + wpb - 1) / wpb);
+    // at least ~4 grabs per wave: a small population in large chunks would
+    // leave most waves without work
+    int64_t chunk = n / (waves * 4);
+    chunk = chunk < 1 ? 1 : (chunk > t.chunk ? t.chunk : chunk);
     hipLaunchKernelGGL((batched_kernel<NR, K, EXT>), dim3(grid), dim3(wpb * kWave), 0, stream,
-                       words, offs, s.order, s.queue_head, n, t.chunk, by_position, ds, fitness,
+                       words, offs, s.order, s.queue_head, n, (int)chunk, by_position, ds, fitness,
                        sse);
     return hipGetLastError();
 }
