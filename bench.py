@@ -150,6 +150,64 @@ def bench_uneven(ctx, steps, warmup, programs=0):
     return res
 
 
+def bench_casebound(ctx, programs, steps, warmup, threads):
+    """The case-bound population at full shape (1,048,576 cases of 5 inputs,
+    16 registers, 128-instruction programs), timed on its first `programs`
+    programs, with the FP64 oracle's SSE and fitness computed for every one of
+    them: the device-math oracle (only the summation differs) for the SSE error,
+    the host-libm oracle for the fitness bar."""
+    import torch
+    from galop_amd import eval_csr_device, workloads
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle_py as O
+    wl = workloads.make_casebound(ctx, seed=0, n_programs=programs)
+    P = wl.n_programs
+    N = wl.n_cases
+    stream = torch.cuda.current_stream().cuda_stream
+    fit = torch.empty(P, dtype=torch.float32, device="cuda")
+    sse = torch.empty(P, dtype=torch.float64, device="cuda")
+    ms = time_steps(lambda: eval_csr_device(wl.dds, wl.words_d, wl.offsets_d, P, fit, sse,
+                                            stream=stream), steps, warmup)
+    ops = float(wl.raw_words) * N
+    g_fit = fit.cpu().numpy()
+    g_sse = sse.cpu().numpy()
+    words, offs = wl.host_prefix(P)
+    isa = O.isa_from(wl.isa)
+    x = wl.dataset.inputs[:N]
+    y = wl.dataset.targets[:N]
+    t0 = time.perf_counter()
+    with O.unary_math(os.path.join(REPO, "galop_amd", "libgalop_math_host.so")):
+        m_fit, m_sse = O.eval_csr(isa, words, offs, x, y, threads=threads)
+    t_oracle = time.perf_counter() - t0
+    h_fit, _ = O.eval_csr(isa, words, offs, x, y, threads=threads)
+
+    def rel(a, b):
+        a = np.asarray(a, np.float64)
+        b = np.asarray(b, np.float64)
+        with np.errstate(all="ignore"):
+            return np.where(a == b, 0.0, np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+    fin = np.isfinite(m_sse)
+    r_sse = rel(g_sse[fin], m_sse[fin])
+    r_fit = rel(g_fit, m_fit)
+    r_host = rel(g_fit, h_fit)
+    over = np.nonzero(r_host > 1e-6)[0]
+    attributed = sum(O.attributed(isa, words[offs[i]:offs[i + 1]], x, y, 1e-6) for i in over[:64])
+    return {
+        "workload": f"first {P} of the 65536 x 128-instruction population; {wl.notes}",
+        "programs": P, "cases": N, "ms": round(ms, 2),
+        "gpops": round(ops / ms / 1e6, 1),
+        "finite_programs": int(fin.sum()),
+        "sse_rel_err_vs_fp64_oracle": {"max": float(r_sse.max()) if r_sse.size else 0.0,
+                                       "median": float(np.median(r_sse)) if r_sse.size else 0.0},
+        "fitness_rel_err_vs_fp64_oracle_max": float(r_fit.max()),
+        "fitness_vs_host_libm_oracle": {"max_rel_err": float(r_host.max()),
+                                        "over_1e-6": int(over.size),
+                                        "over_attributed": int(attributed)},
+        "oracle_seconds": round(t_oracle, 1), "oracle_threads": threads,
+        "per_program_fitness_errors": "every sampled program is compared; see the max/median",
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -172,6 +230,11 @@ def main():
     ap.add_argument("--no-uneven", action="store_true",
                     help="skip the uneven-length batched vs plain comparison")
     ap.add_argument("--uneven-programs", type=int, default=0)
+    ap.add_argument("--no-casebound", action="store_true",
+                    help="skip the case-bound sample (1,048,576 cases) with its FP64 oracle check")
+    ap.add_argument("--casebound-programs", type=int, default=128)
+    ap.add_argument("--no-host-entry", action="store_true",
+                    help="skip timing the host-pointer entry point (pinned staging + H2D + D2H)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -330,12 +393,41 @@ def main():
             "vs_device_math_oracle": {"max_rel_err": cbn["device_math_max_rel_err"],
                                       "within_1e-6": cbn["device_math_within_1e-6"]},
         }
+    if not args.no_host_entry and args.workload == "plain":
+        # The same population through the synchronous host-pointer entry point
+        # (galop_eval_csr): host memcpy into pinned staging, H2D, kernel, D2H.
+        from galop_amd import eval_csr
+        words_h, offs_h = wl.host_prefix(P)
+        eval_csr(wl.dds, words_h, offs_h)  # sizes the staging buffers
+        times = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            fit_h = eval_csr(wl.dds, words_h, offs_h)
+            times.append(time.perf_counter() - t0)
+        same = bool(np.array_equal(fit_h.view(np.int32), fitness.cpu().numpy().view(np.int32)))
+        t_host = float(np.median(times))
+        line["host_entry"] = {
+            "ms": round(t_host * 1e3, 2), "gpops": round(ops / t_host / 1e9, 1),
+            "h2d_bytes": int(4 * wl.raw_words + 8 * (P + 1)),
+            "extra_ms_over_kernel": round(t_host * 1e3 - ms, 2),
+            "fitness_bitwise_equal_device_entry": same,
+            "note": "galop_eval_csr on host arrays: memcpy to pinned staging + H2D + kernel + D2H; "
+                    "median of 3 calls; not the headline value",
+        }
+        log(f"[bench] host entry: {line['host_entry']}")
+        del words_h, offs_h
     del fitness, sse
     if not args.no_uneven and args.workload == "plain":
         log("[bench] uneven lengths: plain vs batched")
         res = bench_uneven(ctx, max(3, args.steps // 4), 1, args.uneven_programs)
         log(f"[bench] uneven: {res}")
         line["uneven"] = res
+    if not args.no_casebound and args.workload == "plain":
+        log("[bench] case-bound sample")
+        threads = min(len(os.sched_getaffinity(0)), 16)
+        res = bench_casebound(ctx, args.casebound_programs, 1, 1, threads)
+        log(f"[bench] casebound: {res}")
+        line["casebound"] = res
     if not args.no_serving and args.workload == "plain":
         from galop_amd.serving_mix import run_mix
         log("[bench] serving mix")

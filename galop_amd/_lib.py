@@ -21,6 +21,8 @@ GALOP_ERR_CANCELLED = 7
 
 OPS_REF8 = 0
 OPS_EXT8 = 1
+FITNESS_MSE = 0
+FITNESS_R2 = 1
 KERNEL_PLAIN = 0
 KERNEL_BATCHED = 1
 LAYOUT_PACKED = 1
@@ -39,7 +41,7 @@ class GalopError(RuntimeError):
 
 class galop_isa(ctypes.Structure):
     _fields_ = [("num_registers", ctypes.c_int32), ("op_set", ctypes.c_int32),
-                ("input_protect", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("input_protect", ctypes.c_int32), ("fitness", ctypes.c_int32),
                 ("constants", ctypes.c_float * 16)]
 
 

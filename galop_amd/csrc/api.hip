@@ -138,6 +138,21 @@ void* galop_ctx_stream(galop_ctx* ctx) { return ctx ? (void*)ctx->stream : nullp
 // ---------------------------------------------------------------------------
 // datasets
 // ---------------------------------------------------------------------------
+// SS_tot = sum (y - mean)^2 over the n real targets: mean first, then the
+// squared deviations, both in FP64 and in case order (src/Dataset.cpp:35-47).
+static double ss_tot_of(const float* targets, int64_t n) {
+    if (n <= 0) return 0.0;
+    double mean = 0.0;
+    for (int64_t i = 0; i < n; ++i) mean += (double)targets[i];
+    mean /= (double)n;
+    double ss = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double d = (double)targets[i] - mean;
+        ss += d * d;
+    }
+    return ss;
+}
+
 int galop_dataset_create(galop_ctx* ctx, const galop_isa* isa, const float* inputs,
                          const float* targets, int64_t n_cases, int32_t n_inputs,
                          galop_dataset** out) {
@@ -148,8 +163,10 @@ int galop_dataset_create(galop_ctx* ctx, const galop_isa* isa, const float* inpu
         return set_error(ctx, GALOP_ERR_INVALID, "galop_dataset_create: num_registers must be 8 or 16");
     if (isa->op_set != GALOP_OPS_REF8 && isa->op_set != GALOP_OPS_EXT8)
         return set_error(ctx, GALOP_ERR_INVALID, "galop_dataset_create: unknown op_set");
-    if ((isa->input_protect != 0 && isa->input_protect != 1) || isa->reserved != 0)
-        return set_error(ctx, GALOP_ERR_INVALID, "galop_dataset_create: input_protect must be 0/1, reserved 0");
+    if (isa->input_protect != 0 && isa->input_protect != 1)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_dataset_create: input_protect must be 0/1");
+    if (isa->fitness != GALOP_FITNESS_MSE && isa->fitness != GALOP_FITNESS_R2)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_dataset_create: unknown fitness kind");
     if (n_cases < 0 || n_cases > (int64_t(1) << 31))
         return set_error(ctx, GALOP_ERR_INVALID, "galop_dataset_create: n_cases out of range [0, 2^31]");
     // src/Interpreter.cpp:39: inputs occupy r0..k-1, so k <= NUM_REGISTERS
@@ -202,6 +219,8 @@ int galop_dataset_create(galop_ctx* ctx, const galop_isa* isa, const float* inpu
     d.op_set = isa->op_set;
     d.input_protect = isa->input_protect;
     d.out_reg = isa->input_protect ? isa->num_registers / 2 : 0;
+    d.fitness_kind = isa->fitness;
+    d.ss_tot = ss_tot_of(targets, n_cases);
     *out = ds;
     return GALOP_OK;
 }
@@ -267,6 +286,17 @@ static int strip_into_staging(galop_ctx* ctx, const galop_dataset* ds, const uin
 }
 
 // Upload host CSR, evaluate, download.  Shared by the dense and CSR calls.
+// Block-partial scratch for a case-split plain launch (nullptr when the
+// launch will not split).
+static hipError_t split_scratch(galop_ctx* ctx, const galop_dataset* ds, int64_t n, double** out) {
+    *out = nullptr;
+    const int64_t need = plain_split_partials(ds->dev, n, ctx->n_cu);
+    if (need <= 0) return hipSuccess;
+    const hipError_t e = ctx->staging.d_partials.reserve((size_t)need);
+    if (e == hipSuccess) *out = ctx->staging.d_partials.ptr;
+    return e;
+}
+
 static int eval_host_csr(galop_ctx* ctx, const galop_dataset* ds, int64_t n, int64_t total_words,
                          bool strip, float* fitness_out, double* sse_out) {
     Staging& st = ctx->staging;
@@ -289,8 +319,10 @@ static int eval_host_csr(galop_ctx* ctx, const galop_dataset* ds, int64_t n, int
         const int rc = strip_into_staging(ctx, ds, w, o, n, total_words, s, &w, &o);
         if (rc != GALOP_OK) return rc;
     }
+    double* partials = nullptr;
+    GALOP_HIP(ctx, split_scratch(ctx, ds, n, &partials));
     GALOP_HIP(ctx, launch_plain(ds->dev, ds->isa.num_registers, ctx->tuning, w, o, n, st.d_fitness.ptr,
-                                sse_out ? st.d_sse.ptr : nullptr, s));
+                                sse_out ? st.d_sse.ptr : nullptr, partials, ctx->n_cu, s));
     GALOP_HIP(ctx, hipMemcpyAsync(st.h_fitness.ptr, st.d_fitness.ptr, (size_t)n * sizeof(float),
                                   hipMemcpyDeviceToHost, s));
     if (sse_out)
@@ -369,8 +401,10 @@ int galop_eval_csr_device(galop_ctx* ctx, const galop_dataset* ds, const uint32_
     GALOP_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     if (kernel == GALOP_KERNEL_PLAIN) {
+        double* partials = nullptr;
+        GALOP_HIP(ctx, split_scratch(ctx, ds, n_programs, &partials));
         GALOP_HIP(ctx, launch_plain(ds->dev, ds->isa.num_registers, ctx->tuning, d_words, d_offsets,
-                                    n_programs, d_fitness, d_sse, s));
+                                    n_programs, d_fitness, d_sse, partials, ctx->n_cu, s));
         return GALOP_OK;
     }
     if (kernel == GALOP_KERNEL_BATCHED) {

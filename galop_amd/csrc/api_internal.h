@@ -98,6 +98,7 @@ struct Staging {
     DeviceBuf<int64_t> d_scan_tmp;
     DeviceBuf<int64_t> d_eff_offsets;
     DeviceBuf<uint32_t> d_eff_words;
+    DeviceBuf<double> d_partials;  // case-split block partials
     BatchBufs batch;
     void release() {
         d_words.release();
@@ -112,6 +113,7 @@ struct Staging {
         d_scan_tmp.release();
         d_eff_offsets.release();
         d_eff_words.release();
+        d_partials.release();
         batch.release();
     }
 };

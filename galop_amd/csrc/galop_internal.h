@@ -23,9 +23,14 @@ struct PlainTuning {
 PlainTuning resolve_plain_tuning(const DevData& ds, int num_registers, PlainTuning t);
 
 // Plain kernel: one program per wave over programs [0, n) of a CSR buffer.
+// With `partials` (plain_split_partials(ds, n, n_cu) doubles), a population
+// too small to fill the GPU is evaluated case-split instead (same bits).
 hipError_t launch_plain(const DevData& ds, int num_registers, PlainTuning t,
                         const uint32_t* words, const int64_t* offsets, int64_t n,
-                        float* fitness, double* sse, hipStream_t stream);
+                        float* fitness, double* sse, double* partials, int n_cu,
+                        hipStream_t stream);
+// Block partials launch_plain needs for a case-split launch (0: no split).
+int64_t plain_split_partials(const DevData& ds, int64_t n, int n_cu);
 
 // Per-case outputs (same interpreter, writes r[OUTPUT_REG] per case).
 hipError_t launch_run_cases(const DevData& ds, int num_registers, const uint32_t* words,

@@ -31,9 +31,48 @@ plain_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ off
     const int64_t hi = offsets[p + 1];
     const double sse = eval_program_wave<NR, K, EXT>(words + lo, hi - lo, ds, lane, cr, nullptr);
     if (lane == 0) {
-        fitness[p] = sse_to_fitness(sse, ds.n_cases);
+        fitness[p] = fitness_of_sse(sse, ds);
         if (sse_out) sse_out[p] = sse;
     }
+}
+
+// Case-split evaluation, for populations too small to fill the GPU with one
+// wave per program: wave w takes program w / splits and, with s = w % splits,
+// the case blocks [s*bpw, min((s+1)*bpw, n_blocks)), writing one FP64 partial
+// per block.  split_finish_kernel then sums each program's partials in block
+// order, the same FP64 additions eval_program_wave makes, so the SSE and
+// fitness are the bits the plain kernel gives.
+template <int NR, int K, int EXT>
+__global__ void __launch_bounds__(256)
+split_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ offsets, int64_t n,
+             int splits, int64_t bpw, DevData ds, double* __restrict__ partials) {
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    const int wave_in_block = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t w = (int64_t)blockIdx.x * (int64_t)(blockDim.x >> 6) + wave_in_block;
+    const int64_t p = w / splits;
+    if (p >= n) return;  // wave-uniform
+    const int64_t n_blocks = (ds.n_cases + kCaseBlock - 1) / kCaseBlock;
+    const int64_t b0 = (w % splits) * bpw;
+    const int64_t b1 = (b0 + bpw) < n_blocks ? (b0 + bpw) : n_blocks;
+    const ConstRegs cr = load_const_regs(ds.consts, lane);
+    const DestDecode dd = dest_decode(NR, ds.input_protect);
+    const int64_t lo = offsets[p], hi = offsets[p + 1];
+    for (int64_t b = b0; b < b1; ++b) {
+        const double part = block_sse<NR, K, EXT>(words + lo, hi - lo, ds, lane, cr, dd,
+                                                  b * kCaseBlock, nullptr);
+        if (lane == 0) partials[p * n_blocks + b] = part;
+    }
+}
+
+__global__ void split_finish_kernel(const double* __restrict__ partials, int64_t n,
+                                    int64_t n_blocks, DevData ds,
+                                    float* __restrict__ fitness, double* __restrict__ sse_out) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    double sse = 0.0;
+    for (int64_t b = 0; b < n_blocks; ++b) sse += partials[p * n_blocks + b];
+    fitness[p] = fitness_of_sse(sse, ds);
+    if (sse_out) sse_out[p] = sse;
 }
 
 template <int NR, int EXT>
@@ -82,10 +121,42 @@ PlainTuning resolve_plain_tuning(const DevData& ds, int num_registers, PlainTuni
     return t;
 }
 
+// Case splitting pays when the population leaves the GPU short of waves and
+// each program has several case blocks.  Returns the waves per program (1 =
+// no split) and sets *bpw to the blocks each split wave takes.
+static int plain_splits(const DevData& ds, int64_t n, int n_cu, int64_t* bpw) {
+    const int64_t n_blocks = (ds.n_cases + kCaseBlock - 1) / kCaseBlock;
+    const int64_t target = (int64_t)n_cu * 16;  // 4 waves on every SIMD
+    *bpw = n_blocks;
+    if (n_blocks <= 1 || n <= 0 || n >= target) return 1;
+    int64_t splits = (target + n - 1) / n;
+    if (splits > n_blocks) splits = n_blocks;
+    *bpw = (n_blocks + splits - 1) / splits;
+    return (int)((n_blocks + *bpw - 1) / *bpw);
+}
+
+int64_t plain_split_partials(const DevData& ds, int64_t n, int n_cu) {
+    int64_t bpw;
+    if (plain_splits(ds, n, n_cu, &bpw) <= 1) return 0;
+    return n * ((ds.n_cases + kCaseBlock - 1) / kCaseBlock);
+}
+
 template <int NR, int K, int EXT>
 static hipError_t launch_plain_t(const DevData& ds, int wpb, const uint32_t* words,
                                  const int64_t* offsets, int64_t n, float* fitness,
-                                 double* sse, hipStream_t stream) {
+                                 double* sse, double* partials, int n_cu, hipStream_t stream) {
+    int64_t bpw = 0;
+    const int splits = partials ? plain_splits(ds, n, n_cu, &bpw) : 1;
+    if (splits > 1) {
+        const int64_t waves = n * splits;
+        hipLaunchKernelGGL((split_kernel<NR, K, EXT>), dim3((unsigned)((waves + wpb - 1) / wpb)),
+                           dim3(wpb * kWave), 0, stream, words, offsets, n, splits, bpw, ds,
+                           partials);
+        hipLaunchKernelGGL(split_finish_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                           stream, partials, n, (ds.n_cases + kCaseBlock - 1) / kCaseBlock,
+                           ds, fitness, sse);
+        return hipGetLastError();
+    }
     for (int64_t first = 0; first < n; first += kLaunchWindow) {
         const int64_t cnt = (n - first) < kLaunchWindow ? (n - first) : kLaunchWindow;
         const unsigned grid = (unsigned)((cnt + wpb - 1) / wpb);
@@ -99,14 +170,15 @@ static hipError_t launch_plain_t(const DevData& ds, int wpb, const uint32_t* wor
 
 hipError_t launch_plain(const DevData& ds, int num_registers, PlainTuning t,
                         const uint32_t* words, const int64_t* offsets, int64_t n,
-                        float* fitness, double* sse, hipStream_t stream) {
+                        float* fitness, double* sse, double* partials, int n_cu,
+                        hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     t = resolve_plain_tuning(ds, num_registers, t);
     const int wpb = t.waves_per_block;
 #define GALOP_CASE(NRV, KV)                                                            \
     if (num_registers == NRV && t.cases_per_lane == KV)                                \
-        return ds.op_set ? launch_plain_t<NRV, KV, 1>(ds, wpb, words, offsets, n, fitness, sse, stream) \
-                         : launch_plain_t<NRV, KV, 0>(ds, wpb, words, offsets, n, fitness, sse, stream);
+        return ds.op_set ? launch_plain_t<NRV, KV, 1>(ds, wpb, words, offsets, n, fitness, sse, partials, n_cu, stream) \
+                         : launch_plain_t<NRV, KV, 0>(ds, wpb, words, offsets, n, fitness, sse, partials, n_cu, stream);
     GALOP_CASE(8, 1) GALOP_CASE(8, 2) GALOP_CASE(8, 4) GALOP_CASE(8, 8)
     GALOP_CASE(16, 1) GALOP_CASE(16, 2) GALOP_CASE(16, 4)
 #undef GALOP_CASE
@@ -516,7 +588,7 @@ batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ o
             const int64_t hi = readlane_i64(hi_l, j);
             const double sse = eval_program_wave<NR, K, EXT>(words + lo, hi - lo, ds, lane, cr, nullptr);
             if (lane == 0) {
-                fitness[p] = sse_to_fitness(sse, ds.n_cases);
+                fitness[p] = fitness_of_sse(sse, ds);
                 if (sse_out) sse_out[p] = sse;
             }
         }

@@ -46,6 +46,8 @@ struct DevData {
     int32_t op_set;        // 0 REF8, 1 EXT8
     int32_t input_protect;
     int32_t out_reg;
+    int32_t fitness_kind;  // 0 MSE, 1 R^2 (GALOP_FITNESS_*)
+    double ss_tot;         // sum (y - mean)^2 of the real targets
 };
 
 // DevData::consts: entries [0,16) are the constant pool; entry 64 + u*16 + idx
@@ -228,14 +230,13 @@ __device__ __forceinline__ void run_program(const uint32_t* __restrict__ prog,
     }
 }
 
-// Load the K cases of tile `tile` for this lane into registers
-// (src/Interpreter.cpp:43-52).
+// Load the K cases of the tile starting at case `first` (a multiple of
+// 64*K) for this lane into registers (src/Interpreter.cpp:43-52).
 // Addresses are a wave-uniform column pointer (SGPRs) plus the lane offset,
 // so no per-lane 64-bit pointers stay live across the program loop.
 template <int NR, int K>
-__device__ __forceinline__ void load_cases(const DevData& ds, int64_t tile,
+__device__ __forceinline__ void load_cases(const DevData& ds, int64_t first,
                                            int lane, RegFile<NR, K>& r) {
-    const int64_t first = tile * (int64_t)(kWave * K);
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
         if (i < ds.n_inputs) {
@@ -295,6 +296,60 @@ __device__ __forceinline__ float sse_to_fitness(double sse, int64_t n_cases) {
     return 1.0f / (1.0f + mse);
 }
 
+// The R^2 variant, src/Evaluator.cpp:56-58,80-84 then :93-98:
+// r2 = ss_tot > 0 and sse finite ? float(1 - sse/ss_tot) : 0;
+// fitness = finite and >= 0 ? r2 : 0.
+__device__ __forceinline__ float sse_to_r2_fitness(double sse, double ss_tot) {
+    float r2 = 0.0f;
+    if (ss_tot > 0.0 && __builtin_isfinite(sse)) r2 = (float)(1.0 - sse / ss_tot);
+    if (!__builtin_isfinite(r2)) return 0.0f;
+    return r2 < 0.0f ? 0.0f : r2;
+}
+
+__device__ __forceinline__ float fitness_of_sse(double sse, const DevData& ds) {
+    return ds.fitness_kind == 1 ? sse_to_r2_fitness(sse, ds.ss_tot)
+                                : sse_to_fitness(sse, ds.n_cases);
+}
+
+// Canonical case blocks.  A program's SSE is the FP64 sum, in block order, of
+// one partial per block of kCaseBlock consecutive cases; a block's partial is
+// the fixed-order FP64 wave sum of the lanes' compensated FP32 sums over the
+// block (lane l takes the block's cases l, l+64, ... in increasing order).
+// The definition does not depend on K or on which wave evaluates which
+// block, so a program's SSE is the same bits whether one wave walks all its
+// blocks or many waves share them (split_kernel).  kCaseBlock is a multiple
+// of every tile size 64*K.
+constexpr int64_t kCaseBlock = 16384;
+
+// Partial of the block of cases [c0, min(c0 + kCaseBlock, n_cases)); valid in
+// lane 0.  When case_out is non-null the per-case outputs are written there
+// instead of being accumulated.
+template <int NR, int K, int EXT>
+__device__ __forceinline__ double block_sse(const uint32_t* __restrict__ prog, int64_t len,
+                                            const DevData& ds, int lane, ConstRegs cr,
+                                            DestDecode dd, int64_t c0,
+                                            float* __restrict__ case_out) {
+    const int64_t tile_cases = (int64_t)kWave * K;
+    const int64_t c1 = (c0 + kCaseBlock) < ds.n_cases ? (c0 + kCaseBlock) : ds.n_cases;
+    const int out_reg = ds.out_reg;
+    LaneAcc acc;
+    for (int64_t first = c0; first < c1; first += tile_cases) {
+        RegFile<NR, K> r;
+        load_cases<NR, K>(ds, first, lane, r);
+        run_program<NR, K, EXT>(prog, len, lane, r, cr, dd);
+        const float* __restrict__ tgt = ds.targets + first;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (first + k * kWave + lane < c1) {
+                const float out = r.get(k, out_reg);
+                if (case_out) case_out[first + k * kWave + lane] = out;
+                else acc.add(out, tgt[k * kWave + lane]);
+            }
+        }
+    }
+    return wave_sum_f64(acc.total());
+}
+
 // Evaluate one program on the whole dataset with one wave: returns the FP64
 // SSE in lane 0 (other lanes hold partial sums).  When case_out is non-null
 // the per-case outputs are written there instead of being accumulated.
@@ -304,26 +359,10 @@ __device__ __forceinline__ double eval_program_wave(const uint32_t* __restrict__
                                                     int lane, ConstRegs cr,
                                                     float* __restrict__ case_out) {
     const DestDecode dd = dest_decode(NR, ds.input_protect);
-    const int out_reg = ds.out_reg;
-    const int64_t tile_cases = (int64_t)kWave * K;
-    const int64_t n_tiles = (ds.n_cases + tile_cases - 1) / tile_cases;
-    LaneAcc acc;
-    for (int64_t t = 0; t < n_tiles; ++t) {
-        RegFile<NR, K> r;
-        load_cases<NR, K>(ds, t, lane, r);
-        run_program<NR, K, EXT>(prog, len, lane, r, cr, dd);
-        const int64_t first = t * tile_cases;
-        const float* __restrict__ tgt = ds.targets + first;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (first + k * kWave + lane < ds.n_cases) {
-                const float out = r.get(k, out_reg);
-                if (case_out) case_out[first + k * kWave + lane] = out;
-                else acc.add(out, tgt[k * kWave + lane]);
-            }
-        }
-    }
-    return wave_sum_f64(acc.total());
+    double sse = 0.0;
+    for (int64_t c0 = 0; c0 < ds.n_cases; c0 += kCaseBlock)
+        sse += block_sse<NR, K, EXT>(prog, len, ds, lane, cr, dd, c0, case_out);
+    return sse;
 }
 
 }  // namespace galop

@@ -17,11 +17,13 @@
 //   log  x = m 2^e with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s),
 //        s = (m-1)/(m+1), series to s^17; plus e ln2.
 //   sin, cos
-//        |x| < 2^25: x = k pi/2 + r with pi/2 split in three (Cody-Waite);
-//        |x| >= 2^25: Payne-Hanek -- the 24-bit mantissa times a 128-bit
-//        window of the binary expansion of 2/pi gives the quadrant and a
-//        62-bit fraction of a quarter turn; then Taylor series of sin r and
-//        cos r to r^15 / r^16 on |r| <= pi/4.
+//        x = k pi/32 + t, |t| <= pi/64: |x| < 2^25 by Cody-Waite with pi/32
+//        split in three; |x| >= 2^25 by Payne-Hanek -- the 24-bit mantissa
+//        times a 128-bit window of the binary expansion of 2/pi gives k mod 64
+//        and a 122-bit fraction.  Then sin x = S_j cos t + C_j sin t with
+//        (S_j, C_j) = (sin, cos)(j pi/32) from a 64-entry table, j = k mod 64
+//        (cos x is j + 16), and Taylor series of sin t to t^7 and cos t - 1
+//        to t^8.
 // Everything is IEEE double add/mul/fma/div, rint, ldexp, frexp and
 // conversions, so the same source compiled for the host gives the same bits.
 #pragma once
@@ -39,13 +41,13 @@
 
 namespace galop_math {
 
-// pi/2 = PIO2_0 + PIO2_1 + PIO2_2; the first two carry 28 bits each, so
-// k * PIO2_{0,1} is exact for |k| < 2^25.
-constexpr double PIO2_0 = 0x1.921fb54000000p+0;
-constexpr double PIO2_1 = 0x1.10b4612000000p-30;
-constexpr double PIO2_2 = -0x1.676733ae8fe48p-60;
-constexpr double TWO_OVER_PI = 0x1.45f306dc9c883p-1;
-constexpr double PIO2_SCALED62 = 0x1.921fb54442d18p-62;  // pi/2 * 2^-62
+// pi/32 = PIO32_0 + PIO32_1 + PIO32_2; the first two carry 24 bits each, so
+// k * PIO32_{0,1} is exact for |k| < 2^29 (|x| < 2^25 gives |k| < 2^28.4).
+constexpr double PIO32_0 = 0x1.921fb60000000p-4;
+constexpr double PIO32_1 = -0x1.777a5c0000000p-29;
+constexpr double PIO32_2 = -0x1.ee59d9cceba40p-54;
+constexpr double INV_PIO32 = 0x1.45f306dc9c883p+3;       // 32/pi
+constexpr double PIO32_SCALED64 = 0x1.921fb54442d18p-68;  // pi/32 * 2^-64
 // ln 2 = LN2_0 + LN2_1 + LN2_2; LN2_{0,1} carry 32 bits each.
 constexpr double LN2_0 = 0x1.62e42ff000000p-1;
 constexpr double LN2_1 = -0x1.718432a200000p-35;
@@ -59,15 +61,10 @@ constexpr double EXP_C[13] = {
     0x1.5555555555555p-5, 0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13,
     0x1.a01a01a01a01ap-16, 0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22, 0x1.ae64567f544e4p-26,
     0x1.1eed8eff8d898p-29};
-// (-1)^k / (2k+1)!, k = 0..7
-constexpr double SIN_C[8] = {
-    0x1.0000000000000p+0, -0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13,
-    0x1.71de3a556c734p-19, -0x1.ae64567f544e4p-26, 0x1.6124613a86d09p-33, -0x1.ae7f3e733b81fp-41};
-// (-1)^k / (2k)!, k = 0..8
-constexpr double COS_C[9] = {
-    0x1.0000000000000p+0, -0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10,
-    0x1.a01a01a01a01ap-16, -0x1.27e4fb7789f5cp-22, 0x1.1eed8eff8d898p-29, -0x1.93974a8c07c9dp-37,
-    0x1.ae7f3e733b81fp-45};
+// (-1)^k / (2k+1)!, k = 1..3
+constexpr double SINT_C[3] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13};
+// (-1)^k / (2k)!, k = 1..4
+constexpr double COST_C[4] = {-0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-16};
 // 2/(2k+1), k = 0..8  (log m = s * sum_k 2/(2k+1) s^(2k))
 constexpr double LOG_C[9] = {
     0x1.0000000000000p+1, 0x1.5555555555555p-1, 0x1.999999999999ap-2, 0x1.2492492492492p-2,
@@ -76,6 +73,73 @@ constexpr double LOG_C[9] = {
 // bits 2^-1 .. 2^-256 of 2/pi (word 0 first)
 GM_CONST uint32_t TWO_OVER_PI_W[8] = {0xA2F9836Eu, 0x4E441529u, 0xFC2757D1u, 0xF534DDC0u,
                                       0xDB629599u, 0x3C439041u, 0xFE5163ABu, 0xDEBBC561u};
+// (sin, cos)(j pi/32), j = 0..63, interleaved
+GM_CONST double SINCOS_T[128] = {
+    0x0.0p+0, 0x1.0000000000000p+0,
+    0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1,
+    0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1,
+    0x1.294062ed59f06p-2, 0x1.e9f4156c62ddap-1,
+    0x1.87de2a6aea963p-2, 0x1.d906bcf328d46p-1,
+    0x1.e2b5d3806f63bp-2, 0x1.c38b2f180bdb1p-1,
+    0x1.1c73b39ae68c8p-1, 0x1.a9b66290ea1a3p-1,
+    0x1.44cf325091dd6p-1, 0x1.8bc806b151741p-1,
+    0x1.6a09e667f3bcdp-1, 0x1.6a09e667f3bcdp-1,
+    0x1.8bc806b151741p-1, 0x1.44cf325091dd6p-1,
+    0x1.a9b66290ea1a3p-1, 0x1.1c73b39ae68c8p-1,
+    0x1.c38b2f180bdb1p-1, 0x1.e2b5d3806f63bp-2,
+    0x1.d906bcf328d46p-1, 0x1.87de2a6aea963p-2,
+    0x1.e9f4156c62ddap-1, 0x1.294062ed59f06p-2,
+    0x1.f6297cff75cb0p-1, 0x1.8f8b83c69a60bp-3,
+    0x1.fd88da3d12526p-1, 0x1.917a6bc29b42cp-4,
+    0x1.0000000000000p+0, 0x0.0p+0,
+    0x1.fd88da3d12526p-1, -0x1.917a6bc29b42cp-4,
+    0x1.f6297cff75cb0p-1, -0x1.8f8b83c69a60bp-3,
+    0x1.e9f4156c62ddap-1, -0x1.294062ed59f06p-2,
+    0x1.d906bcf328d46p-1, -0x1.87de2a6aea963p-2,
+    0x1.c38b2f180bdb1p-1, -0x1.e2b5d3806f63bp-2,
+    0x1.a9b66290ea1a3p-1, -0x1.1c73b39ae68c8p-1,
+    0x1.8bc806b151741p-1, -0x1.44cf325091dd6p-1,
+    0x1.6a09e667f3bcdp-1, -0x1.6a09e667f3bcdp-1,
+    0x1.44cf325091dd6p-1, -0x1.8bc806b151741p-1,
+    0x1.1c73b39ae68c8p-1, -0x1.a9b66290ea1a3p-1,
+    0x1.e2b5d3806f63bp-2, -0x1.c38b2f180bdb1p-1,
+    0x1.87de2a6aea963p-2, -0x1.d906bcf328d46p-1,
+    0x1.294062ed59f06p-2, -0x1.e9f4156c62ddap-1,
+    0x1.8f8b83c69a60bp-3, -0x1.f6297cff75cb0p-1,
+    0x1.917a6bc29b42cp-4, -0x1.fd88da3d12526p-1,
+    0x0.0p+0, -0x1.0000000000000p+0,
+    -0x1.917a6bc29b42cp-4, -0x1.fd88da3d12526p-1,
+    -0x1.8f8b83c69a60bp-3, -0x1.f6297cff75cb0p-1,
+    -0x1.294062ed59f06p-2, -0x1.e9f4156c62ddap-1,
+    -0x1.87de2a6aea963p-2, -0x1.d906bcf328d46p-1,
+    -0x1.e2b5d3806f63bp-2, -0x1.c38b2f180bdb1p-1,
+    -0x1.1c73b39ae68c8p-1, -0x1.a9b66290ea1a3p-1,
+    -0x1.44cf325091dd6p-1, -0x1.8bc806b151741p-1,
+    -0x1.6a09e667f3bcdp-1, -0x1.6a09e667f3bcdp-1,
+    -0x1.8bc806b151741p-1, -0x1.44cf325091dd6p-1,
+    -0x1.a9b66290ea1a3p-1, -0x1.1c73b39ae68c8p-1,
+    -0x1.c38b2f180bdb1p-1, -0x1.e2b5d3806f63bp-2,
+    -0x1.d906bcf328d46p-1, -0x1.87de2a6aea963p-2,
+    -0x1.e9f4156c62ddap-1, -0x1.294062ed59f06p-2,
+    -0x1.f6297cff75cb0p-1, -0x1.8f8b83c69a60bp-3,
+    -0x1.fd88da3d12526p-1, -0x1.917a6bc29b42cp-4,
+    -0x1.0000000000000p+0, 0x0.0p+0,
+    -0x1.fd88da3d12526p-1, 0x1.917a6bc29b42cp-4,
+    -0x1.f6297cff75cb0p-1, 0x1.8f8b83c69a60bp-3,
+    -0x1.e9f4156c62ddap-1, 0x1.294062ed59f06p-2,
+    -0x1.d906bcf328d46p-1, 0x1.87de2a6aea963p-2,
+    -0x1.c38b2f180bdb1p-1, 0x1.e2b5d3806f63bp-2,
+    -0x1.a9b66290ea1a3p-1, 0x1.1c73b39ae68c8p-1,
+    -0x1.8bc806b151741p-1, 0x1.44cf325091dd6p-1,
+    -0x1.6a09e667f3bcdp-1, 0x1.6a09e667f3bcdp-1,
+    -0x1.44cf325091dd6p-1, 0x1.8bc806b151741p-1,
+    -0x1.1c73b39ae68c8p-1, 0x1.a9b66290ea1a3p-1,
+    -0x1.e2b5d3806f63bp-2, 0x1.c38b2f180bdb1p-1,
+    -0x1.87de2a6aea963p-2, 0x1.d906bcf328d46p-1,
+    -0x1.294062ed59f06p-2, 0x1.e9f4156c62ddap-1,
+    -0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1,
+    -0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1,
+};
 
 // Every FP64 constant above in one table; the functions below take it as an
 // argument.  On the device the table sits in constant memory and is reached
@@ -86,26 +150,24 @@ GM_CONST uint32_t TWO_OVER_PI_W[8] = {0xA2F9836Eu, 0x4E441529u, 0xFC2757D1u, 0xF
 // gfx950 takes no 64-bit literal), about 60 VGPRs that cost occupancy.
 enum CoefSlot : int {
     kExpC = 0,        // EXP_C[0..12]
-    kSinC = 13,       // SIN_C[0..7]
-    kCosC = 21,       // COS_C[0..8]
-    kLogC = 30,       // LOG_C[0..8]
-    kPio2 = 39,       // PIO2_0, PIO2_1, PIO2_2
-    kTwoOverPi = 42,
-    kPio2Scaled62 = 43,
-    kLn2 = 44,        // LN2_0, LN2_1, LN2_2
-    kInvLn2 = 47,
-    kSqrtHalf = 48,
-    kCoefCount = 49
+    kLogC = 13,       // LOG_C[0..8]
+    kSinT = 22,       // SINT_C[0..2]
+    kCosT = 25,       // COST_C[0..3]
+    kPio32 = 29,      // PIO32_0, PIO32_1, PIO32_2
+    kInvPio32 = 32,
+    kPio32Scaled64 = 33,
+    kLn2 = 34,        // LN2_0, LN2_1, LN2_2
+    kInvLn2 = 37,
+    kSqrtHalf = 38,
+    kCoefCount = 39
 };
 #define GM_COEF_INIT                                                              \
     {EXP_C[0], EXP_C[1], EXP_C[2], EXP_C[3], EXP_C[4], EXP_C[5], EXP_C[6],        \
      EXP_C[7], EXP_C[8], EXP_C[9], EXP_C[10], EXP_C[11], EXP_C[12],               \
-     SIN_C[0], SIN_C[1], SIN_C[2], SIN_C[3], SIN_C[4], SIN_C[5], SIN_C[6], SIN_C[7], \
-     COS_C[0], COS_C[1], COS_C[2], COS_C[3], COS_C[4], COS_C[5], COS_C[6], COS_C[7], \
-     COS_C[8],                                                                    \
      LOG_C[0], LOG_C[1], LOG_C[2], LOG_C[3], LOG_C[4], LOG_C[5], LOG_C[6], LOG_C[7], \
      LOG_C[8],                                                                    \
-     PIO2_0, PIO2_1, PIO2_2, TWO_OVER_PI, PIO2_SCALED62,                          \
+     SINT_C[0], SINT_C[1], SINT_C[2], COST_C[0], COST_C[1], COST_C[2], COST_C[3],  \
+     PIO32_0, PIO32_1, PIO32_2, INV_PIO32, PIO32_SCALED64,                        \
      LN2_0, LN2_1, LN2_2, INV_LN2, SQRT_HALF}
 
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -178,32 +240,35 @@ GM_HD float plogf_r(float b, CoefPtr C) {
 }
 
 // ---- sin / cos ---------------------------------------------------------------
-// Reduce a finite x to r in [-pi/4, pi/4] and a quadrant q with x = q pi/2 + r (mod 2 pi).
-GM_HD double reduce_pio2(float x, int& q, CoefPtr C) {
+// Reduce a finite x to x = k pi/32 + t, |t| <= pi/64 (plus rounding), and
+// j = k mod 64.
+GM_HD double reduce_pio32(float x, int& j, CoefPtr C) {
     const double xd = (double)x;
     const uint32_t u = f2u(x);
     const uint32_t ex = (u >> 23) & 0xffu;
     if (ex < 127u + 25u) {  // |x| < 2^25: Cody-Waite
-        const double k = rint_d(xd * C[kTwoOverPi]);
-        double r = fma_d(-k, C[kPio2], xd);
-        r = fma_d(-k, C[kPio2 + 1], r);
-        r = fma_d(-k, C[kPio2 + 2], r);
-        q = ((int)k) & 3;
-        return r;
+        const double k = rint_d(xd * C[kInvPio32]);
+        double t = fma_d(-k, C[kPio32], xd);
+        t = fma_d(-k, C[kPio32 + 1], t);
+        t = fma_d(-k, C[kPio32 + 2], t);
+        j = ((int)k) & 63;
+        return t;
     }
-    // |x| >= 2^25: x = M 2^E with E = ex - 150 >= 2.  Only bits 2^-i of 2/pi with
-    // i >= E-1 matter modulo 4; take 128 of them starting there (W), then
-    // x * 2/pi = M * W * 2^-126 (mod 4).
+    // |x| >= 2^25: |x| = M 2^E with E = ex - 150 >= 2.  Only bits 2^-i of 2/pi
+    // with i >= E-1 matter modulo 4 quarter turns; take 128 of them starting
+    // there (W): then |x| * 2/pi = M * W * 2^-126 (mod 4).  In P = M * W, bits
+    // 127..126 are the quarter turn, so bits 127..122 are k mod 64 for
+    // |x| * 32/pi = 16 |x| * 2/pi, and bits 121..0 its fraction.
     const uint32_t M = (u & 0x7fffffu) | 0x800000u;
     const int i0 = (int)ex - 150 - 1;  // first bit index needed (>= 1)
-    const int j = (i0 - 1) >> 5;
+    const int jw = (i0 - 1) >> 5;
     const int sh = (i0 - 1) & 31;
     uint32_t w[5];
-    for (int t = 0; t < 5; ++t) w[t] = TWO_OVER_PI_W[j + t];
+    for (int t = 0; t < 5; ++t) w[t] = TWO_OVER_PI_W[jw + t];
     uint32_t W[4];
     for (int t = 0; t < 4; ++t)
         W[t] = sh ? ((w[t] << sh) | (w[t + 1] >> (32 - sh))) : w[t];
-    // P = M * (W0 W1 W2 W3); keep bits 64..127 (mid) with the carries below.
+    // P = M * (W0 W1 W2 W3) mod 2^128: hi64 = bits 64..127, lo64 = bits 0..63
     const uint64_t a0 = (uint64_t)M * W[0];
     const uint64_t a1 = (uint64_t)M * W[1];
     const uint64_t a2 = (uint64_t)M * W[2];
@@ -211,61 +276,56 @@ GM_HD double reduce_pio2(float x, int& q, CoefPtr C) {
     const uint64_t lo = a3 + (a2 << 32);
     const uint64_t c0 = lo < a3 ? 1u : 0u;
     const uint64_t mid = (a2 >> 32) + a1 + (a0 << 32) + c0;
-    q = (int)(mid >> 62);
-    int64_t F = (int64_t)(mid & 0x3fffffffffffffffULL);
-    if (F >= (int64_t(1) << 61)) {  // fraction >= 1/2: round to the next quadrant
-        F -= int64_t(1) << 62;
-        q += 1;
-    }
-    double r = (double)F * C[kPio2Scaled62];
+    int k = (int)(mid >> 58);
+    const uint64_t fh = (mid << 6) | (lo >> 58);  // fraction bits 2^-1 .. 2^-64
+    const uint64_t fl = lo << 6;                   // fraction bits 2^-65 .. 2^-122
+    // A fraction >= 1/2 rounds k up; fh:fl read as a signed 128-bit number is
+    // then the fraction minus one.
+    if (fh >> 63) k += 1;
+    const double f = (double)(int64_t)fh + (double)fl * 0x1p-64;  // fraction * 2^64
+    double t = f * C[kPio32Scaled64];
     if (xd < 0.0) {
-        r = -r;
-        q = -q;
+        t = -t;
+        k = -k;
     }
-    q &= 3;
-    return r;
+    j = k & 63;
+    return t;
 }
 
-GM_HD double sin_poly(double r, CoefPtr C) {
-    const double r2 = r * r;
-    double p = C[kSinC + 7];
-    for (int k = 6; k >= 1; --k) p = fma_d(p, r2, C[kSinC + k]);
-    return fma_d(r * r2, p, r);
-}
-
-GM_HD double cos_poly(double r, CoefPtr C) {
-    const double r2 = r * r;
-    double p = C[kCosC + 8];
-    for (int k = 7; k >= 1; --k) p = fma_d(p, r2, C[kCosC + k]);
-    return fma_d(r2, p, 1.0);
-}
-
-// sin x (want_cos = 0) or cos x (want_cos = 1) from the reduced argument.
-// Both series are evaluated and the result selected per lane: the quadrant
-// differs from lane to lane, and a branch on it would run both anyway.
-GM_HD double sincos_from_reduced(double r, int q, int want_cos, CoefPtr C) {
-    const double s = sin_poly(r, C);
-    const double c = cos_poly(r, C);
-    const int qq = q + want_cos;  // cos x = sin(x + pi/2)
-    const double v = (qq & 1) ? c : s;
-    return (qq & 2) ? -v : v;
+// sin(j pi/32 + t) = S_j cos t + C_j sin t = S_j + (S_j (cos t - 1) + C_j sin t)
+GM_HD double sin_from_reduced(double t, int j, CoefPtr C) {
+    const double s = SINCOS_T[2 * j];
+    const double c = SINCOS_T[2 * j + 1];
+    const double t2 = t * t;
+    // sin t = t + t^3 (s3 + t^2 (s5 + t^2 s7)); the innermost step is a
+    // multiply and an add, since one VOP3 instruction cannot read two SGPR
+    // pairs on gfx950
+    double ps = t2 * C[kSinT + 2] + C[kSinT + 1];
+    ps = fma_d(ps, t2, C[kSinT]);
+    const double sin_t = fma_d(t * t2, ps, t);
+    // cos t - 1 = t^2 (c2 + t^2 (c4 + t^2 (c6 + t^2 c8)))
+    double pc = t2 * C[kCosT + 3] + C[kCosT + 2];
+    pc = fma_d(pc, t2, C[kCosT + 1]);
+    pc = fma_d(pc, t2, C[kCosT]);
+    const double cm1 = pc * t2;
+    return s + fma_d(c, sin_t, s * cm1);
 }
 
 GM_HD float sinf_r(float x, CoefPtr C) {
     if (x == 0.0f) return x;  // keeps the sign of zero
     const float ax = u2f(f2u(x) & 0x7fffffffu);
     if (!(ax < u2f(0x7f800000u))) return x - x;  // inf, NaN -> NaN
-    int q;
-    const double r = reduce_pio2(x, q, C);
-    return (float)sincos_from_reduced(r, q, 0, C);
+    int j;
+    const double t = reduce_pio32(x, j, C);
+    return (float)sin_from_reduced(t, j, C);
 }
 
 GM_HD float cosf_r(float x, CoefPtr C) {
     const float ax = u2f(f2u(x) & 0x7fffffffu);
     if (!(ax < u2f(0x7f800000u))) return x - x;
-    int q;
-    const double r = reduce_pio2(x, q, C);
-    return (float)sincos_from_reduced(r, q, 1, C);
+    int j;
+    const double t = reduce_pio32(x, j, C);
+    return (float)sin_from_reduced(t, (j + 16) & 63, C);  // cos x = sin(x + pi/2)
 }
 
 }  // namespace galop_math

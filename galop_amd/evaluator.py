@@ -30,10 +30,11 @@ class Isa:
     the constant pool (default: the reference's pool for the register count)."""
 
     def __init__(self, num_registers=8, op_set=_lib.OPS_REF8, input_protect=False,
-                 constants=None):
+                 constants=None, fitness=_lib.FITNESS_MSE):
         self.num_registers = int(num_registers)
         self.op_set = int(op_set)
         self.input_protect = bool(input_protect)
+        self.fitness = int(fitness)
         if constants is None:
             constants = REF_CONSTANTS_8 if self.num_registers == 8 else REF_CONSTANTS_16
         c = np.zeros(16, np.float32)
@@ -49,7 +50,7 @@ class Isa:
         s.num_registers = self.num_registers
         s.op_set = self.op_set
         s.input_protect = 1 if self.input_protect else 0
-        s.reserved = 0
+        s.fitness = self.fitness
         for i in range(16):
             s.constants[i] = float(self.constants[i])
         return s

@@ -28,6 +28,7 @@ galop_isa HipEvaluator::engine_isa() {
     for (int i = 0; i < 8; ++i) isa.constants[i] = pool[i];
 #endif
     isa.op_set = GALOP_OPS_REF8;
+    isa.fitness = GALOP_FITNESS_MSE;  // CpuEvaluator's 1/(1+mse)
     return isa;
 }
 

@@ -95,21 +95,63 @@ def taylor(kind, n_terms):
     raise ValueError(kind)
 
 
+def sin_cos_fixed(r, prec=PREC):
+    """(sin, cos) of r*pi/32 for 0 <= r <= 16, as integers scaled by 2^prec,
+    by their Taylor series in fixed point (a few units in the last place)."""
+    guard = 64
+    p = prec + guard
+    one = 1 << p
+    theta = r * pi_fixed(p) // 32
+    th2 = theta * theta >> p
+    out = []
+    for start, first in ((theta, 1), (one, 0)):
+        total = 0
+        term = start
+        k = first  # term = theta^k / k!
+        sign = 1
+        while term:
+            total += sign * term
+            term = term * th2 >> p
+            term //= (k + 1) * (k + 2)
+            k += 2
+            sign = -sign
+        out.append(total >> guard)
+    return tuple(out)
+
+
+def sincos_table():
+    """64 pairs (sin(j pi/32), cos(j pi/32)), j = 0..63, rounded to double;
+    the multiples of pi/2 hold exact 0 and +-1."""
+    first = []
+    for r in range(17):
+        sf, cf = sin_cos_fixed(r)
+        first.append((float(Fraction(sf, 1 << PREC)), float(Fraction(cf, 1 << PREC))))
+    first[0] = (0.0, 1.0)
+    first[16] = (1.0, 0.0)
+    out = []
+    for j in range(64):
+        q, r = divmod(j, 16)
+        s, c = first[r]
+        s, c = [(s, c), (c, -s), (-s, -c), (-c, s)][q]
+        out.extend([s + 0.0, c + 0.0])  # no negative zeros
+    return out
+
+
 def constants():
     pi = pi_fixed()
     ln2 = ln2_fixed()
-    half_pi = pi  # pi/2 = pi * 2^-1
     c = {
-        "PIO2": split(half_pi, PREC + 1, 28),
-        "TWO_OVER_PI": float(Fraction(2 << PREC, pi)),
-        "PIO2_SCALED62": float(Fraction(pi, 1 << (PREC + 1)) / (1 << 62)),
+        "PIO32": split(pi, PREC + 5, 24),
+        "INV_PIO32": float(Fraction(32 << PREC, pi)),
+        "PIO32_SCALED64": float(Fraction(pi, 1 << (PREC + 5)) / (1 << 64)),
         "LN2": split(ln2, PREC, 32),
         "INV_LN2": float(Fraction(1 << PREC, ln2)),
         "TWO_OVER_PI_WORDS": two_over_pi_words(12),
         "EXP_POLY": taylor("exp", 13),
-        "SIN_POLY": taylor("sin", 8),
-        "COS_POLY": taylor("cos", 9),
+        "SINT_POLY": taylor("sin", 4)[1:],   # -1/3!, 1/5!, -1/7!
+        "COST_POLY": taylor("cos", 5)[1:],   # -1/2!, 1/4!, -1/6!, 1/8!
         "LOG_POLY": taylor("atanh", 9),
+        "SINCOS_T": sincos_table(),
     }
     return c
 

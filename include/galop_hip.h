@@ -57,6 +57,16 @@ extern "C" {
 #define GALOP_LAYOUT_PACKED    1 /* CSR of the raw programs                 */
 #define GALOP_LAYOUT_EFFECTIVE 2 /* CSR of intron-stripped programs         */
 
+/* Fitness of a program from its FP64 SSE over N cases:
+ *   GALOP_FITNESS_MSE  mse = float(sse/N); finite ? 1/(1+mse) : 0
+ *                      (CpuEvaluator, src/Evaluator.cpp:44,88-92)
+ *   GALOP_FITNESS_R2   r2 = float(1 - sse/ss_tot), 0 when ss_tot <= 0 or sse
+ *                      is not finite; fitness = finite and >= 0 ? r2 : 0
+ *                      (src/Evaluator.cpp:49-85,93-98).  ss_tot is computed
+ *                      at dataset creation as src/Dataset.cpp:35-47 does. */
+#define GALOP_FITNESS_MSE 0
+#define GALOP_FITNESS_R2  1
+
 /* Per-problem ISA configuration: the reference's compile-time LGPConfig
  * (NUM_REGISTERS src/LGPConfig.h:132-140, CONSTANTS :151-170, INPUT_PROTECT /
  * OUTPUT_REG :289-300) as run-time data. */
@@ -64,7 +74,7 @@ typedef struct galop_isa {
     int32_t num_registers;  /* 8 or 16                                     */
     int32_t op_set;         /* GALOP_OPS_REF8 or GALOP_OPS_EXT8             */
     int32_t input_protect;  /* 1: dest folded into upper half, out=NR/2     */
-    int32_t reserved;       /* must be 0                                   */
+    int32_t fitness;        /* GALOP_FITNESS_MSE (0) or GALOP_FITNESS_R2    */
     float   constants[16];  /* constant pool, first num_registers used     */
 } galop_isa;
 

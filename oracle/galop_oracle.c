@@ -150,6 +150,35 @@ float go_sse_to_fitness(double sse, int64_t n_cases) {
     return go_mse_to_fitness(mse);
 }
 
+/* src/Dataset.cpp:35-47: mean first, then squared deviations, in FP64 */
+double go_ss_tot(const float* targets, int64_t n) {
+    if (n <= 0) return 0.0;
+    double mean = 0.0;
+    for (int64_t i = 0; i < n; ++i) mean += (double)targets[i];
+    mean /= (double)n;
+    double ss = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+        const double d = (double)targets[i] - mean;
+        ss += d * d;
+    }
+    return ss;
+}
+
+/* src/Evaluator.cpp:56-58,80-84 (evaluate_sr_r2) and :93-98 (r2_to_fitness) */
+float go_r2_to_fitness(double sse, double ss_tot) {
+    float r2;
+    if (ss_tot <= 0.0) r2 = 0.0f;
+    else if (!isfinite(sse)) r2 = 0.0f;
+    else r2 = (float)(1.0 - sse / ss_tot);
+    if (!isfinite(r2)) return 0.0f;
+    return r2 < 0.0f ? 0.0f : r2;
+}
+
+float go_fitness_of_sse(const go_isa* isa, double sse, int64_t n_cases, double ss_tot) {
+    return isa->fitness == GO_FITNESS_R2 ? go_r2_to_fitness(sse, ss_tot)
+                                         : go_sse_to_fitness(sse, n_cases);
+}
+
 static int pick_threads(int32_t threads) {
 #ifdef _OPENMP
     return threads > 0 ? threads : omp_get_num_procs();
@@ -165,12 +194,13 @@ void go_eval_csr(const go_isa* isa, const uint32_t* words, const int64_t* offset
                  double* sse_out, int32_t threads) {
     const int nt = pick_threads(threads);
     (void)nt;
+    const double ss_tot = isa->fitness == GO_FITNESS_R2 ? go_ss_tot(targets, n_cases) : 0.0;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
     for (int64_t i = 0; i < n_programs; ++i) {
         const double sse = go_program_sse(isa, words + offsets[i], offsets[i + 1] - offsets[i],
                                           inputs, targets, n_inputs, n_cases);
         if (sse_out) sse_out[i] = sse;
-        fitness_out[i] = go_sse_to_fitness(sse, n_cases);
+        fitness_out[i] = go_fitness_of_sse(isa, sse, n_cases, ss_tot);
     }
 }
 
@@ -181,12 +211,13 @@ void go_eval_dense(const go_isa* isa, const uint32_t* instructions,
                    int64_t n_cases, float* fitness_out, int32_t threads) {
     const int nt = threads > 1 ? pick_threads(threads) : 1;
     (void)nt;
+    const double ss_tot = isa->fitness == GO_FITNESS_R2 ? go_ss_tot(targets, n_cases) : 0.0;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
     for (int64_t i = 0; i < n_programs; ++i) {
         const double sse = go_program_sse(isa, instructions + (size_t)i * (size_t)stride,
                                           (int64_t)lengths[i], inputs, targets,
                                           n_inputs, n_cases);
-        fitness_out[i] = go_sse_to_fitness(sse, n_cases);
+        fitness_out[i] = go_fitness_of_sse(isa, sse, n_cases, ss_tot);
     }
 }
 
@@ -271,10 +302,9 @@ void go_dual_precision_fitness(const go_isa* isa, const uint32_t* prog,
         sse_f += ef * ef;
         sse_d += ed * ed;
     }
-    const float mse_f = (float)(sse_f / (double)n_cases);
-    const float mse_d = (float)(sse_d / (double)n_cases);
-    *fit_f = go_mse_to_fitness(mse_f);
-    *fit_d = go_mse_to_fitness(mse_d);
+    const double ss_tot = isa->fitness == GO_FITNESS_R2 ? go_ss_tot(targets, n_cases) : 0.0;
+    *fit_f = go_fitness_of_sse(isa, sse_f, n_cases, ss_tot);
+    *fit_d = go_fitness_of_sse(isa, sse_d, n_cases, ss_tot);
 }
 
 /* ---- libm-sensitivity probe -------------------------------------------
@@ -329,5 +359,6 @@ float go_perturbed_fitness(const go_isa* isa, const uint32_t* prog, int64_t len,
         const double err = (double)r[o] - (double)targets[c];
         sse += err * err;
     }
-    return go_sse_to_fitness(sse, n_cases);
+    const double ss_tot = isa->fitness == GO_FITNESS_R2 ? go_ss_tot(targets, n_cases) : 0.0;
+    return go_fitness_of_sse(isa, sse, n_cases, ss_tot);
 }

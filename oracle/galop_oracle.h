@@ -12,6 +12,8 @@
  *   context-chunked execution ..... src/Interpreter.cpp:11-33
  *   FP64 SSE, float(sse/N) ........ src/Evaluator.cpp:11-45
  *   mse -> fitness ................ src/Evaluator.cpp:88-92
+ *   R^2 fitness (variant) ......... src/Evaluator.cpp:49-85,93-98, ss_tot from
+ *                                   src/Dataset.cpp:35-47
  *   population loop ............... src/EvaluatorCpu.cpp:7-16
  *   intron analysis ............... src/Introns.h:28-57
  *   float/double chaos probe ...... tests/cuda_diff.cu:92-144 (with protected
@@ -35,13 +37,15 @@ extern "C" {
 
 #define GO_OPS_REF8 0
 #define GO_OPS_EXT8 1
+#define GO_FITNESS_MSE 0  /* 1/(1+mse): CpuEvaluator's fitness */
+#define GO_FITNESS_R2 1   /* max(0, R^2): the reference's R^2 variant */
 
 /* Same field layout as galop_isa in include/galop_hip.h. */
 typedef struct go_isa {
     int32_t num_registers;  /* 8 or 16 */
     int32_t op_set;         /* GO_OPS_REF8 / GO_OPS_EXT8 */
     int32_t input_protect;  /* 0/1 (LGPConfig INPUT_PROTECT) */
-    int32_t reserved;
+    int32_t fitness;        /* GO_FITNESS_MSE / GO_FITNESS_R2 */
     float   constants[16];  /* first num_registers entries used */
 } go_isa;
 
@@ -75,6 +79,13 @@ double go_program_sse(const go_isa* isa, const uint32_t* prog, int64_t len,
 /* mse = float(sse/N); fitness = finite ? 1/(1+mse) : 0 */
 float go_sse_to_fitness(double sse, int64_t n_cases);
 float go_mse_to_fitness(float mse);
+/* SS_tot = sum (y - mean)^2 over n targets, two passes in FP64. */
+double go_ss_tot(const float* targets, int64_t n);
+/* R^2 = float(1 - sse/ss_tot) (0 if ss_tot <= 0 or sse not finite), then
+ * fitness = finite and >= 0 ? R^2 : 0. */
+float go_r2_to_fitness(double sse, double ss_tot);
+/* The fitness the ISA's fitness kind gives for an FP64 SSE. */
+float go_fitness_of_sse(const go_isa* isa, double sse, int64_t n_cases, double ss_tot);
 
 /* Population over CSR programs: program i = words[offsets[i]:offsets[i+1]].
  * fitness_out[i] (and sse_out[i] if non-NULL).  threads<=0: all cores. */

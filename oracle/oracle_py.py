@@ -17,7 +17,7 @@ OPS_EXT8 = 1
 
 class go_isa(ctypes.Structure):
     _fields_ = [("num_registers", ctypes.c_int32), ("op_set", ctypes.c_int32),
-                ("input_protect", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("input_protect", ctypes.c_int32), ("fitness", ctypes.c_int32),
                 ("constants", ctypes.c_float * 16)]
 
 
@@ -40,6 +40,10 @@ def lib():
         L.go_program_sse.argtypes = [pisa, vp, i64, vp, vp, i32, i64]
         L.go_sse_to_fitness.restype = ctypes.c_float
         L.go_sse_to_fitness.argtypes = [ctypes.c_double, i64]
+        L.go_ss_tot.restype = ctypes.c_double
+        L.go_ss_tot.argtypes = [vp, i64]
+        L.go_r2_to_fitness.restype = ctypes.c_float
+        L.go_r2_to_fitness.argtypes = [ctypes.c_double, ctypes.c_double]
         L.go_mse_to_fitness.restype = ctypes.c_float
         L.go_mse_to_fitness.argtypes = [ctypes.c_float]
         L.go_eval_csr.restype = None
@@ -70,12 +74,17 @@ def lib():
     return _lib
 
 
-def make_isa(num_registers=8, op_set=OPS_REF8, input_protect=False, constants=None):
+FITNESS_MSE = 0
+FITNESS_R2 = 1
+
+
+def make_isa(num_registers=8, op_set=OPS_REF8, input_protect=False, constants=None,
+             fitness=FITNESS_MSE):
     s = go_isa()
     s.num_registers = num_registers
     s.op_set = op_set
     s.input_protect = 1 if input_protect else 0
-    s.reserved = 0
+    s.fitness = int(fitness)
     if constants is None:
         from galop_amd.evaluator import REF_CONSTANTS_8, REF_CONSTANTS_16
         constants = REF_CONSTANTS_8 if num_registers == 8 else REF_CONSTANTS_16
@@ -86,7 +95,8 @@ def make_isa(num_registers=8, op_set=OPS_REF8, input_protect=False, constants=No
 
 def isa_from(isa):
     """Oracle ISA from a galop_amd.Isa."""
-    return make_isa(isa.num_registers, isa.op_set, isa.input_protect, list(isa.constants))
+    return make_isa(isa.num_registers, isa.op_set, isa.input_protect, list(isa.constants),
+                    getattr(isa, "fitness", FITNESS_MSE))
 
 
 def _p(a):
@@ -225,3 +235,14 @@ class unary_math:
 
     def __exit__(self, *a):
         lib().go_set_unary_math(None, None, None, None)
+
+
+def ss_tot(targets):
+    """SS_tot of the targets, two passes in FP64 (src/Dataset.cpp:35-47)."""
+    y = np.ascontiguousarray(targets, np.float32)
+    return float(lib().go_ss_tot(_p(y), y.size))
+
+
+def r2_to_fitness(sse, sst):
+    """max(0, float(1 - sse/ss_tot)) with the reference's guards."""
+    return float(lib().go_r2_to_fitness(float(sse), float(sst)))

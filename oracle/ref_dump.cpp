@@ -143,6 +143,12 @@ int dump_engine(uint32_t seed, const std::string& csv, const std::string& out) {
         std::vector<uint8_t> lens(lengths, lengths + n);
         std::vector<float> fit(n);
         CpuEvaluator{}.evaluate_population(instr, lengths, fit.data(), n, ds);
+        // the reference's R^2 fitness of the same programs (src/Evaluator.cpp:49-98)
+        std::vector<float> r2fit(n);
+        for (int i = 0; i < n; ++i) {
+            ProgramView view{instr + (size_t)i * S, (int)lengths[i]};
+            r2fit[i] = Fitness::r2_to_fitness(Evaluator::evaluate_sr_r2(view, ds));
+        }
         std::vector<uint8_t> mask((size_t)n * S, 0);
         std::vector<int64_t> cnt(n);
         for (int i = 0; i < n; ++i) {
@@ -153,6 +159,7 @@ int dump_engine(uint32_t seed, const std::string& csv, const std::string& out) {
         write_vec(out + "." + tag + ".dense.u32", dense);
         write_vec(out + "." + tag + ".lengths.u8", lens);
         write_vec(out + "." + tag + ".fitness.f32", fit);
+        write_vec(out + "." + tag + ".r2fitness.f32", r2fit);
         write_vec(out + "." + tag + ".effmask.u8", mask);
         write_vec(out + "." + tag + ".effcount.i64", cnt);
         long long sumL = 0, sumE = 0;
@@ -168,6 +175,7 @@ int dump_engine(uint32_t seed, const std::string& csv, const std::string& out) {
     std::vector<float> targets(ds.targets.begin(), ds.targets.begin() + ds.N);
     write_vec(out + ".inputs.f32", inputs);
     write_vec(out + ".targets.f32", targets);
+    write_vec(out + ".ss_tot.f64", std::vector<double>{ds.ss_tot});
     return 0;
 }
 

@@ -16,7 +16,9 @@ Fixtures:
   engine_seed42.npz  LGPEngine(42).init_evolution() population and the
                      population after evolve_sr on datasets/nguyen1_train.csv
                      (the cuda_diff populations, tests/cuda_diff.cu:57-73,
-                     205-222) with CpuEvaluator fitness and intron masks
+                     205-222) with CpuEvaluator fitness, the R^2 fitness of
+                     Evaluator::evaluate_sr_r2 / Fitness::r2_to_fitness,
+                     the dataset's ss_tot and intron masks
 
 usage: python tests/golden/make_fixtures.py [--ref /root/reference]
 """
@@ -79,10 +81,12 @@ def engine(seed, csv, out_name):
             arrays[f"{tag}_words"] = words
             arrays[f"{tag}_lengths"] = lengths
             arrays[f"{tag}_fitness"] = _read(prefix, f"{tag}.fitness.f32", np.float32)
+            arrays[f"{tag}_r2_fitness"] = _read(prefix, f"{tag}.r2fitness.f32", np.float32)
             arrays[f"{tag}_effmask"] = effmask
             arrays[f"{tag}_effcount"] = _read(prefix, f"{tag}.effcount.i64", np.int64)
         arrays["inputs"] = _read(prefix, "inputs.f32", np.float32)
         arrays["targets"] = _read(prefix, "targets.f32", np.float32)
+        arrays["ss_tot"] = _read(prefix, "ss_tot.f64", np.float64)
         arrays["stride"] = np.array(stride)
     np.savez_compressed(os.path.join(HERE, out_name), **arrays)
     print("wrote", out_name)

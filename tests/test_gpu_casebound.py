@@ -8,7 +8,12 @@ sum in case order (src/Evaluator.cpp:36-44) to 1e-8 relative -- 100x tighter
 than the 1e-6 fitness bar -- where a plain per-lane FP32 sum (the reference
 kernel's, src/EvaluatorCuda.cu:120-158) is typically off by ~1e-5 at this N.
 The oracle runs with the device's own unary functions, so only the summation
-differs."""
+differs.
+
+With 12 programs the plain entry point evaluates case-split (several waves per
+program, one FP64 partial per 16,384-case block, summed in block order); the
+batched kernel walks each program's blocks with one wave.  The two must give
+the same bits."""
 import numpy as np
 import pytest
 
@@ -31,6 +36,10 @@ def test_million_case_sse_precision(gpu_ctx):
     words, offsets = random_population(rng, P, L, L, 16)
     words = words & np.uint32(0xFFFFFF03)
     fit, sse = gpu_fitness(dds, words, offsets)
+    from galop_amd._lib import KERNEL_BATCHED
+    fit_b, sse_b = gpu_fitness(dds, words, offsets, kernel=KERNEL_BATCHED)
+    assert np.array_equal(sse.view(np.int64), sse_b.view(np.int64))
+    assert np.array_equal(fit.view(np.int32), fit_b.view(np.int32))
     oisa = O.isa_from(isa)
     with O.unary_math(math_host_path()):
         ofit, osse = O.eval_csr(oisa, words, offsets, x, y, threads=0)

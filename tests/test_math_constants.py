@@ -35,32 +35,70 @@ def _header_values():
         vals[name] = nums if len(nums) > 1 else nums[0]
     words = re.search(r"TWO_OVER_PI_W\[8\] = \{([^}]*)\}", text).group(1)
     vals["TWO_OVER_PI_W"] = [int(w.strip().rstrip("u"), 16) for w in words.split(",")]
+    table = re.search(r"SINCOS_T\[128\] = \{([^}]*)\}", text).group(1)
+    vals["SINCOS_T"] = [float.fromhex(t.strip()) for t in table.split(",") if t.strip()]
     return vals
 
 
 def test_constants_match_derivation():
     c = MC.constants()
     h = _header_values()
-    assert [h["PIO2_0"], h["PIO2_1"], h["PIO2_2"]] == c["PIO2"]
+    assert [h["PIO32_0"], h["PIO32_1"], h["PIO32_2"]] == c["PIO32"]
     assert [h["LN2_0"], h["LN2_1"], h["LN2_2"]] == c["LN2"]
-    assert h["TWO_OVER_PI"] == c["TWO_OVER_PI"]
-    assert h["PIO2_SCALED62"] == c["PIO2_SCALED62"]
+    assert h["INV_PIO32"] == c["INV_PIO32"]
+    assert h["PIO32_SCALED64"] == c["PIO32_SCALED64"]
     assert h["INV_LN2"] == c["INV_LN2"]
     assert h["EXP_C"] == c["EXP_POLY"]
-    assert h["SIN_C"] == c["SIN_POLY"]
-    assert h["COS_C"] == c["COS_POLY"]
+    assert h["SINT_C"] == c["SINT_POLY"]
+    assert h["COST_C"] == c["COST_POLY"]
     assert h["LOG_C"] == c["LOG_POLY"]
     assert h["TWO_OVER_PI_W"] == c["TWO_OVER_PI_WORDS"][:8]
+    assert h["SINCOS_T"] == c["SINCOS_T"]
     # the splits are exact decompositions with short leading parts
     from fractions import Fraction
-    for parts, bits in ((c["PIO2"], 28), (c["LN2"], 32)):
+    for parts, bits in ((c["PIO32"], 24), (c["LN2"], 32)):
         for p in parts[:2]:
             m = Fraction(p)
             while m.denominator != 1:
                 m *= 2
             assert abs(m.numerator).bit_length() <= bits
     import math
-    assert sum(c["PIO2"]) == math.pi / 2 and sum(c["LN2"]) == math.log(2)
+    assert sum(c["PIO32"]) == math.pi / 32 and sum(c["LN2"]) == math.log(2)
+
+
+def test_sincos_table_identities():
+    """The table is pinned independently of the series that made it:
+    exact values at multiples of pi/4 and pi/2 (sqrt(2)/2 from an integer
+    square root), sin^2 + cos^2 = 1, the double-angle and addition formulas
+    (each within a few units of 2^-53), and numpy's sin/cos for small j."""
+    from fractions import Fraction
+    from math import isqrt
+    t = MC.constants()["SINCOS_T"]
+    S = [Fraction(t[2 * j]) for j in range(64)]
+    C = [Fraction(t[2 * j + 1]) for j in range(64)]
+    half_sqrt2 = float(Fraction(isqrt(2 << 240), 1 << 121))  # sqrt(2)/2, correctly rounded
+    assert t[2 * 8] == half_sqrt2 and t[2 * 8 + 1] == half_sqrt2
+    for q in range(4):
+        assert (t[2 * 16 * q], t[2 * 16 * q + 1]) == [(0.0, 1.0), (1.0, 0.0), (0.0, -1.0), (-1.0, 0.0)][q]
+    eps = Fraction(1, 1 << 51)
+    for j in range(64):
+        assert abs(S[j] ** 2 + C[j] ** 2 - 1) < eps
+        assert abs(S[(2 * j) % 64] - 2 * S[j] * C[j]) < 2 * eps
+        assert abs(S[(j + 1) % 64] - (S[j] * C[1] + C[j] * S[1])) < 2 * eps
+        assert abs(C[(j + 1) % 64] - (C[j] * C[1] - S[j] * S[1])) < 2 * eps
+    for j in range(9):
+        assert abs(t[2 * j] - np.sin(j * np.pi / 32)) <= 2.3e-16
+        assert abs(t[2 * j + 1] - np.cos(j * np.pi / 32)) <= 2.3e-16
+
+
+def test_planted_wrong_table_entry_is_caught():
+    from fractions import Fraction
+    t = list(MC.constants()["SINCOS_T"])
+    t[2 * 21] = float(Fraction(t[2 * 21]) * (1 + Fraction(1, 1 << 40)))
+    S = [Fraction(t[2 * j]) for j in range(64)]
+    C = [Fraction(t[2 * j + 1]) for j in range(64)]
+    eps = Fraction(1, 1 << 51)
+    assert any(abs(S[j] ** 2 + C[j] ** 2 - 1) >= eps for j in range(64))
 
 
 def test_planted_wrong_word_is_caught():
