@@ -85,6 +85,16 @@ __device__ __forceinline__ float op_cos(float b, galop_math::CoefPtr C) {
 __device__ __forceinline__ float op_exp(float b, galop_math::CoefPtr C) {
     return galop_math::expf_r(b, C);
 }
+// Arguments whose result needs no arithmetic, and that result (a word whose
+// every case is such skips the evaluation).
+__device__ __forceinline__ bool op_exp_special(float b) { return galop_math::expf_special(b); }
+__device__ __forceinline__ float op_exp_special_value(float b) {
+    return galop_math::expf_special_value(b);
+}
+__device__ __forceinline__ bool op_plog_special(float b) { return galop_math::plogf_special(b); }
+__device__ __forceinline__ float op_plog_special_value(float b) {
+    return galop_math::plogf_special_value(b);
+}
 __device__ __forceinline__ float op_plog(float b, galop_math::CoefPtr C) {
     return galop_math::plogf_r(b, C);
 }
@@ -186,8 +196,49 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
         } else {                                                          \
             const galop_math::CoefPtr C = galop_math::coef_table();       \
             float bv[K], cv[K];                                           \
-            _Pragma("unroll") for (int k = 0; k < K; ++k) bv[k] = r.get(k, s2); \
-            _Pragma("unroll") for (int k = 0; k < K; ++k) cv[k] = FN(bv[k], C); \
+            bool sp = true;                                               \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
+                bv[k] = r.get(k, s2);                                     \
+                sp = sp && FN##_special(bv[k]);                           \
+            }                                                             \
+            if (__all(sp)) {                                              \
+                _Pragma("unroll") for (int k = 0; k < K; ++k)             \
+                    cv[k] = FN##_special_value(bv[k]);                    \
+            } else {                                                      \
+                _Pragma("unroll") for (int k = 0; k < K; ++k) cv[k] = FN(bv[k], C); \
+            }                                                             \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]); \
+        }                                                                 \
+    } while (0)
+
+// sin / cos: the branch-free Cody-Waite path for the whole word unless some
+// lane holds, in some case, a finite argument of magnitude >= 2^25.
+#define GALOP_SINCOS(U, WANT_COS)                                          \
+    do {                                                                  \
+        if (kc) {                                                         \
+            const float v = wave_bcast(cr.table, (U) * 16 + s2);          \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, v);    \
+        } else {                                                          \
+            const galop_math::CoefPtr C = galop_math::coef_table();       \
+            float bv[K], cv[K];                                           \
+            bool ph = false, sp = true;                                   \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
+                bv[k] = r.get(k, s2);                                     \
+                ph |= galop_math::sincos_needs_ph(bv[k]);                 \
+                sp = sp && galop_math::sincos_special(bv[k]);             \
+            }                                                             \
+            if (__all(sp)) {                                              \
+                _Pragma("unroll") for (int k = 0; k < K; ++k)             \
+                    cv[k] = galop_math::sincos_special_value(bv[k], WANT_COS); \
+            } else if (__any(ph)) {                                       \
+                _Pragma("unroll") for (int k = 0; k < K; ++k)             \
+                    cv[k] = galop_math::sincos_needs_ph(bv[k])            \
+                                ? galop_math::sincos_ph(bv[k], WANT_COS, C) \
+                                : galop_math::sincos_cw(bv[k], WANT_COS, C); \
+            } else {                                                      \
+                _Pragma("unroll") for (int k = 0; k < K; ++k)             \
+                    cv[k] = galop_math::sincos_cw(bv[k], WANT_COS, C);    \
+            }                                                             \
             _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]); \
         }                                                                 \
     } while (0)
@@ -197,8 +248,8 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
     case 1: GALOP_BIN(a - b); break;
     case 2: GALOP_BIN(a * b); break;
     case 3: GALOP_BIN(op_div(a, b)); break;
-    case 4: GALOP_UNARY(0, op_sin); break;
-    case 5: GALOP_UNARY(1, op_cos); break;
+    case 4: GALOP_SINCOS(0, 0); break;
+    case 5: GALOP_SINCOS(1, 1); break;
     case 6:
         if constexpr (EXT) GALOP_UNARY(2, op_exp);
         else GALOP_BIN((a < b) ? 1.0f : 0.0f);
@@ -210,6 +261,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
     }
 #undef GALOP_BIN
 #undef GALOP_UNARY
+#undef GALOP_SINCOS
 }
 
 // Run `len` words starting at `prog` on the K register vectors.  The next

@@ -199,28 +199,37 @@ GM_HD float u2f(uint32_t u) {
 }
 
 // ---- exp -------------------------------------------------------------------
+// Arguments whose exp is fixed without arithmetic: NaN, and those beyond the
+// clamp below (+inf above 89, +0 below -104).
+GM_HD bool expf_special(float x) { return !(x >= -104.0f && x <= 89.0f); }
+GM_HD float expf_special_value(float x) {
+    return (x != x) ? x : (x > 0.0f ? u2f(0x7f800000u) : 0.0f);
+}
+
+// No branches, so the K cases of an interpreter word can run interleaved.
+// The argument is clamped to [-104.5, 89.5]: exp(89.5) rounds to +inf and
+// exp(-104.5) to +0 in FP32, as every argument beyond does; NaN is patched
+// at the end (the clamp maps it to -104.5).
 GM_HD float expf_r(float x, CoefPtr C) {
-    if (x != x) return x;
-    if (x > 89.0f) return u2f(0x7f800000u);   // exp(89) > FLT_MAX
-    if (x < -104.0f) return 0.0f;             // exp(-104) < half the least subnormal
-    const double xd = (double)x;
+    const float xc = __builtin_fminf(__builtin_fmaxf(x, -104.5f), 89.5f);
+    const double xd = (double)xc;
     const double k = rint_d(xd * C[kInvLn2]);
     double r = fma_d(-k, C[kLn2], xd);
     r = fma_d(-k, C[kLn2 + 1], r);
     r = fma_d(-k, C[kLn2 + 2], r);
     double p = C[kExpC + 12];
     for (int n = 11; n >= 0; --n) p = fma_d(p, r, C[kExpC + n]);
-    return (float)ldexp(p, (int)k);
+    const float v = (float)ldexp(p, (int)k);
+    return (x != x) ? x : v;
 }
 
 // ---- log of a positive, finite, non-zero float --------------------------
 GM_HD float logf_pos(float x, CoefPtr C) {
     int e;
     double m = frexp((double)x, &e);  // m in [0.5, 1)
-    if (m < C[kSqrtHalf]) {
-        m *= 2.0;
-        e -= 1;
-    }
+    const bool low = m < C[kSqrtHalf];
+    m = low ? 2.0 * m : m;
+    e = low ? e - 1 : e;
     const double s = (m - 1.0) / (m + 1.0);
     const double s2 = s * s;
     double p = C[kLogC + 8];
@@ -231,34 +240,44 @@ GM_HD float logf_pos(float x, CoefPtr C) {
 }
 
 // protected log of the interpreter's EXT8 table: b != 0 ? log|b| : 0
-GM_HD float plogf_r(float b, CoefPtr C) {
-    if (b != b) return b;
-    if (b == 0.0f) return 0.0f;
+// Arguments whose protected log is fixed without arithmetic: 0 -> 0,
+// +-inf -> +inf, NaN -> NaN.
+GM_HD bool plogf_special(float b) {
     const float a = u2f(f2u(b) & 0x7fffffffu);
-    if (a == u2f(0x7f800000u)) return a;
-    return logf_pos(a, C);
+    return !(a > 0.0f && a < u2f(0x7f800000u));
+}
+GM_HD float plogf_special_value(float b) {
+    const float a = u2f(f2u(b) & 0x7fffffffu);
+    return (b != b) ? b : (a == 0.0f ? 0.0f : a);
+}
+
+// Without branches: the logarithm is taken of 1 in place of 0, infinity and
+// NaN, whose results (0, +inf, NaN) are selected at the end.
+GM_HD float plogf_r(float b, CoefPtr C) {
+    const float a = u2f(f2u(b) & 0x7fffffffu);
+    const bool regular = a > 0.0f && a < u2f(0x7f800000u);  // false for 0, inf, NaN
+    const float r = logf_pos(regular ? a : 1.0f, C);
+    const float special = (b != b) ? b : (a == 0.0f ? 0.0f : a);
+    return regular ? r : special;
 }
 
 // ---- sin / cos ---------------------------------------------------------------
-// Reduce a finite x to x = k pi/32 + t, |t| <= pi/64 (plus rounding), and
-// j = k mod 64.
-GM_HD double reduce_pio32(float x, int& j, CoefPtr C) {
-    const double xd = (double)x;
+// x = k pi/32 + t, |t| <= pi/64 (plus rounding), j = k mod 64.
+
+// Finite arguments with |x| >= 2^25 take the Payne-Hanek reduction.
+GM_HD bool sincos_needs_ph(float x) {
+    const uint32_t ex = (f2u(x) >> 23) & 0xffu;
+    return ex >= 127u + 25u && ex != 255u;
+}
+
+// Payne-Hanek for finite |x| >= 2^25: |x| = M 2^E with E = ex - 150 >= 2.
+// Only bits 2^-i of 2/pi with i >= E-1 matter modulo 4 quarter turns; take
+// 128 of them starting there (W): then |x| * 2/pi = M * W * 2^-126 (mod 4).
+// In P = M * W, bits 127..126 are the quarter turn, so bits 127..122 are k mod
+// 64 for |x| * 32/pi = 16 |x| * 2/pi, and bits 121..0 its fraction.
+GM_HD double reduce_ph(float x, int& j, CoefPtr C) {
     const uint32_t u = f2u(x);
     const uint32_t ex = (u >> 23) & 0xffu;
-    if (ex < 127u + 25u) {  // |x| < 2^25: Cody-Waite
-        const double k = rint_d(xd * C[kInvPio32]);
-        double t = fma_d(-k, C[kPio32], xd);
-        t = fma_d(-k, C[kPio32 + 1], t);
-        t = fma_d(-k, C[kPio32 + 2], t);
-        j = ((int)k) & 63;
-        return t;
-    }
-    // |x| >= 2^25: |x| = M 2^E with E = ex - 150 >= 2.  Only bits 2^-i of 2/pi
-    // with i >= E-1 matter modulo 4 quarter turns; take 128 of them starting
-    // there (W): then |x| * 2/pi = M * W * 2^-126 (mod 4).  In P = M * W, bits
-    // 127..126 are the quarter turn, so bits 127..122 are k mod 64 for
-    // |x| * 32/pi = 16 |x| * 2/pi, and bits 121..0 its fraction.
     const uint32_t M = (u & 0x7fffffu) | 0x800000u;
     const int i0 = (int)ex - 150 - 1;  // first bit index needed (>= 1)
     const int jw = (i0 - 1) >> 5;
@@ -268,7 +287,7 @@ GM_HD double reduce_pio32(float x, int& j, CoefPtr C) {
     uint32_t W[4];
     for (int t = 0; t < 4; ++t)
         W[t] = sh ? ((w[t] << sh) | (w[t + 1] >> (32 - sh))) : w[t];
-    // P = M * (W0 W1 W2 W3) mod 2^128: hi64 = bits 64..127, lo64 = bits 0..63
+    // P = M * (W0 W1 W2 W3) mod 2^128: mid = bits 64..127, lo = bits 0..63
     const uint64_t a0 = (uint64_t)M * W[0];
     const uint64_t a1 = (uint64_t)M * W[1];
     const uint64_t a2 = (uint64_t)M * W[2];
@@ -284,7 +303,7 @@ GM_HD double reduce_pio32(float x, int& j, CoefPtr C) {
     if (fh >> 63) k += 1;
     const double f = (double)(int64_t)fh + (double)fl * 0x1p-64;  // fraction * 2^64
     double t = f * C[kPio32Scaled64];
-    if (xd < 0.0) {
+    if (x < 0.0f) {
         t = -t;
         k = -k;
     }
@@ -311,21 +330,47 @@ GM_HD double sin_from_reduced(double t, int j, CoefPtr C) {
     return s + fma_d(c, sin_t, s * cm1);
 }
 
-GM_HD float sinf_r(float x, CoefPtr C) {
-    if (x == 0.0f) return x;  // keeps the sign of zero
+// sin (want_cos = 0) or cos (1) of an x that does not need Payne-Hanek, by
+// Cody-Waite, without branches: the K cases of an interpreter word run
+// interleaved.  Infinities and NaN are reduced as 0 and give NaN at the end;
+// sin keeps the sign of a zero argument.
+GM_HD float sincos_cw(float x, int want_cos, CoefPtr C) {
     const float ax = u2f(f2u(x) & 0x7fffffffu);
-    if (!(ax < u2f(0x7f800000u))) return x - x;  // inf, NaN -> NaN
+    const bool fin = ax < u2f(0x7f800000u);  // false for inf, NaN
+    const double xd = fin ? (double)x : 0.0;
+    const double k = rint_d(xd * C[kInvPio32]);  // |k| < 2^28.4
+    double t = fma_d(-k, C[kPio32], xd);
+    t = fma_d(-k, C[kPio32 + 1], t);
+    t = fma_d(-k, C[kPio32 + 2], t);
+    const int j = (((int)k) + 16 * want_cos) & 63;  // cos x = sin(x + pi/2)
+    const float r = (float)sin_from_reduced(t, j, C);
+    const float zero_case = want_cos ? r : x;
+    return !fin ? x - x : (x == 0.0f ? zero_case : r);
+}
+
+// Arguments whose sin / cos is fixed without arithmetic: zeros (sin keeps the
+// sign, cos is 1) and infinities / NaN (NaN).
+GM_HD bool sincos_special(float x) {
+    const float ax = u2f(f2u(x) & 0x7fffffffu);
+    return x == 0.0f || !(ax < u2f(0x7f800000u));
+}
+GM_HD float sincos_special_value(float x, int want_cos) {
+    return x == 0.0f ? (want_cos ? 1.0f : x) : x - x;
+}
+
+// sin or cos of any x after Payne-Hanek.
+GM_HD float sincos_ph(float x, int want_cos, CoefPtr C) {
     int j;
-    const double t = reduce_pio32(x, j, C);
-    return (float)sin_from_reduced(t, j, C);
+    const double t = reduce_ph(x, j, C);
+    return (float)sin_from_reduced(t, (j + 16 * want_cos) & 63, C);
+}
+
+GM_HD float sinf_r(float x, CoefPtr C) {
+    return sincos_needs_ph(x) ? sincos_ph(x, 0, C) : sincos_cw(x, 0, C);
 }
 
 GM_HD float cosf_r(float x, CoefPtr C) {
-    const float ax = u2f(f2u(x) & 0x7fffffffu);
-    if (!(ax < u2f(0x7f800000u))) return x - x;
-    int j;
-    const double t = reduce_pio32(x, j, C);
-    return (float)sin_from_reduced(t, (j + 16) & 63, C);  // cos x = sin(x + pi/2)
+    return sincos_needs_ph(x) ? sincos_ph(x, 1, C) : sincos_cw(x, 1, C);
 }
 
 }  // namespace galop_math
