@@ -45,6 +45,22 @@ class galop_isa(ctypes.Structure):
                 ("constants", ctypes.c_float * 16)]
 
 
+class galop_vary_cfg(ctypes.Structure):
+    """include/galop_engine.h: the engine's compile-time configuration."""
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "population", "stride", "elite_count", "tournament_size", "num_registers",
+        "input_protect", "op_set", "effmut", "macro_inserts", "macro_random_pos",
+        "starting_length", "reserved")] + [(n, ctypes.c_float) for n in (
+        "crossover_rate", "micro_rate", "micro_per_instr", "effmut_rate", "insert_rate",
+        "delete_rate")]
+
+
+class galop_generation_stats(ctypes.Structure):
+    _fields_ = [("generation", ctypes.c_int32), ("best_fitness", ctypes.c_float),
+                ("mean_fitness", ctypes.c_float), ("best_length", ctypes.c_int32),
+                ("mean_length", ctypes.c_float), ("best_index", ctypes.c_int32)]
+
+
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
 _i64 = ctypes.c_int64
@@ -81,6 +97,10 @@ SIGNATURES = {
     "galop_wait": (ctypes.c_int, [_vp, _i64, ctypes.c_double]),
     "galop_request_times": (ctypes.c_int, [_vp, _i64, _pf64, _pf64]),
     "galop_server_stats": (ctypes.c_int, [_vp, _pi64, _pi64, _pi64, _pi64]),
+    # include/galop_engine.h
+    "galop_vary_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_uint64, _vp]),
+    "galop_init_population_device": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, _vp, _vp, _vp, _vp]),
+    "galop_generation_stats_device": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp, _vp]),
     # include/galop_workload.h
     "galop_fill_random_words_device": (ctypes.c_int, [_vp, _i64, _i32, ctypes.c_uint64, _vp]),
     "galop_fill_effective_programs_device": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32, _i32,

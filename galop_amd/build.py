@@ -31,7 +31,7 @@ BUILD = os.path.join(REPO, "_build")
 LIB = os.path.join(PKG, "libgalop_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels.hip", "api.hip", "batch.hip", "server.hip", "workload.hip"]
+SOURCES = ["kernels.hip", "api.hip", "batch.hip", "server.hip", "workload.hip", "engine.hip"]
 
 DEVICE_FLAGS = [
     "--offload-arch=gfx950",
@@ -89,6 +89,20 @@ def build_math_host(verbose=False):
     return out
 
 
+def build_engine_host(verbose=False):
+    """The engine operators (csrc/lgp_engine.h) compiled for the host: the CPU
+    tests check them against the reference's vectors, the GPU tests compare
+    the device build with them."""
+    out = os.path.join(PKG, "libgalop_engine_host.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+          "-Wall", "-o", tmp, os.path.join(CSRC, "engine_host.cpp")])
+    os.replace(tmp, out)
+    if verbose:
+        print("built", out)
+    return out
+
+
 def build_oracle(verbose=False):
     """The CPU checker (oracle/).  The reference builds under oracle/_ref are
     made only where the reference sources exist."""
@@ -119,6 +133,7 @@ def build_adapter_driver(verbose=False):
 def build(verbose=False):
     build_library(verbose)
     build_math_host(verbose)
+    build_engine_host(verbose)
     build_adapter_driver(verbose)
     build_oracle(verbose)  # last: its engine driver links the library
 

@@ -99,6 +99,10 @@ struct Staging {
     DeviceBuf<int64_t> d_eff_offsets;
     DeviceBuf<uint32_t> d_eff_words;
     DeviceBuf<double> d_partials;  // case-split block partials
+    DeviceBuf<uint64_t> d_keys;    // elite selection: keys, sorted keys
+    DeviceBuf<uint64_t> d_keys_sorted;
+    DeviceBuf<uint8_t> d_sort_tmp;
+    DeviceBuf<uint8_t> d_small;    // small device results (generation statistics)
     BatchBufs batch;
     void release() {
         d_words.release();
@@ -114,6 +118,10 @@ struct Staging {
         d_eff_offsets.release();
         d_eff_words.release();
         d_partials.release();
+        d_keys.release();
+        d_keys_sorted.release();
+        d_sort_tmp.release();
+        d_small.release();
         batch.release();
     }
 };

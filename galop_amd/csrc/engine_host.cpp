@@ -1,0 +1,80 @@
+// engine_host.cpp -- the engine operators of lgp_engine.h compiled for the
+// host, so the CPU tests can check them against vectors the reference engine
+// wrote, and the GPU tests can compare the device build with them.
+#include <algorithm>
+#include <cmath>
+#include <functional>
+#include <vector>
+
+#include "lgp_engine.h"
+
+namespace ge = galop_engine;
+
+extern "C" {
+
+// LGPEngine::vary minus the flip: elites by descending key, then every pair.
+void gv_vary(const ge::VaryCfg* c, const uint32_t* cur_instr, const uint8_t* cur_len,
+             const float* cur_fit, uint32_t* next_instr, uint8_t* next_len, float* next_fit,
+             uint64_t stream_base) {
+    const int64_t P = c->population;
+    std::vector<uint64_t> keys((size_t)P);
+    for (int64_t i = 0; i < P; ++i) keys[(size_t)i] = ge::elite_key(cur_fit[i], i);
+    std::sort(keys.begin(), keys.end(), std::greater<uint64_t>());
+    for (int e = 0; e < c->elite_count; ++e) {
+        const int64_t src = (int64_t)(uint32_t)keys[(size_t)e];
+        for (int k = 0; k < cur_len[src]; ++k)
+            next_instr[(int64_t)e * c->stride + k] = cur_instr[src * c->stride + k];
+        next_len[e] = cur_len[src];
+        next_fit[e] = cur_fit[src];
+    }
+    const int64_t n_pairs = (P - c->elite_count) / 2;
+    for (int64_t i = 0; i < n_pairs; ++i)
+        ge::vary_pair(i, *c, cur_instr, cur_len, cur_fit, next_instr, next_len, next_fit,
+                      stream_base);
+}
+
+void gv_init(const ge::VaryCfg* c, uint64_t base, uint32_t* instr, uint8_t* len, float* fit) {
+    for (int64_t i = 0; i < c->population; ++i) {
+        ge::init_program(i, *c, base, instr);
+        len[i] = (uint8_t)c->starting_length;
+        fit[i] = ge::bits_to_float(0x7fc00000u);
+    }
+}
+
+// out: generation, best fitness, mean fitness, best length, mean length, best index
+void gv_stats(const float* fit, const uint8_t* len, int64_t P, int32_t generation, double* out) {
+    const int C = 256;
+    double sum = 0.0;
+    long long len_sum = 0;
+    float best = -1.0f;
+    int best_idx = 0;
+    for (int k = 0; k < C; ++k) {
+        const long long lo = (long long)P * k / C, hi = (long long)P * (k + 1) / C;
+        double s = 0.0;
+        long long ls = 0;
+        float b = -1.0f;
+        int bi = (int)lo;
+        for (long long i = lo; i < hi; ++i) {
+            ls += len[i];
+            s += (double)fit[i];
+            if (fit[i] > b) {
+                b = fit[i];
+                bi = (int)i;
+            }
+        }
+        sum += s;
+        len_sum += ls;
+        if (b > best) {
+            best = b;
+            best_idx = bi;
+        }
+    }
+    out[0] = generation;
+    out[1] = best;
+    out[2] = (float)(sum / (double)P);
+    out[3] = len[best_idx];
+    out[4] = (float)((double)len_sum / (double)P);
+    out[5] = best_idx;
+}
+
+}  // extern "C"

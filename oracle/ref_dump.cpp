@@ -171,11 +171,74 @@ int dump_engine(uint32_t seed, const std::string& csv, const std::string& out) {
     dump_pop("init");
     engine->evolve_sr(ds);
     dump_pop("evolved");
+    // compute_stats of the final population (src/LGPEngine.cpp:477-517)
+    const GenerationStats& st = engine->history_view().back();
+    std::vector<double> stats{(double)st.generation, (double)st.best_fitness, (double)st.mean_fitness,
+                              (double)st.best_length, (double)st.mean_length, (double)st.best_index};
+    write_vec(out + ".stats.f64", stats);
     std::vector<float> inputs(ds.inputs.begin(), ds.inputs.begin() + ds.N);
     std::vector<float> targets(ds.targets.begin(), ds.targets.begin() + ds.N);
     write_vec(out + ".inputs.f32", inputs);
     write_vec(out + ".targets.f32", targets);
     write_vec(out + ".ss_tot.f64", std::vector<double>{ds.ss_tot});
+    return 0;
+}
+
+// One generation of selection and variation (LGPEngine::vary,
+// src/LGPEngine.cpp:421-456) on an evaluated population: the population and
+// fitness before, the per-generation stream base vary() draws from the
+// engine's mt19937 (recomputed from a second mt19937 with the same seed,
+// advanced past init_population's draws), and the next generation after.  The
+// same build's init population is dumped too (GALOP_PARALLEL_INIT builds draw
+// it from per-program PCG32 streams, src/LGPEngine.cpp:52-67).
+int dump_vary(uint32_t seed, const std::string& csv, const std::string& out) {
+    Dataset ds = load_csv_1d(csv);
+    const int n = LGPConfig::POPULATION_SIZE;
+    const int S = LGPConfig::MAX_PROGRAM_SIZE;
+    auto engine = std::make_unique<LGPEngine>(seed);
+    auto dump_cur = [&](const std::string& tag) {
+        const auto& data = engine->get_data();
+        const int buf = engine->current_buffer_index();
+        const uint32_t* instr = data.instructions_buf[buf].data();
+        std::vector<uint32_t> dense(instr, instr + (size_t)n * S);
+        std::vector<uint8_t> lens(data.program_lengths_buf[buf].begin(),
+                                  data.program_lengths_buf[buf].begin() + n);
+        std::vector<float> fit(data.fitness_scores_buf[buf].begin(),
+                               data.fitness_scores_buf[buf].begin() + n);
+        write_vec(out + "." + tag + ".dense.u32", dense);
+        write_vec(out + "." + tag + ".lengths.u8", lens);
+        write_vec(out + "." + tag + ".fitness.f32", fit);
+    };
+    std::mt19937 shadow(seed);
+    std::uniform_int_distribution<uint32_t> d32(0, UINT32_MAX);
+    engine->init_evolution();
+#ifdef GALOP_PARALLEL_INIT
+    const uint64_t init_hi = d32(shadow), init_lo = d32(shadow);
+    std::vector<uint64_t> init_base{(init_hi << 32) | init_lo};
+#else
+    for (long long i = 0; i < LGPConfig::TOTAL_INSTRUCTIONS; ++i) (void)d32(shadow);
+    std::vector<uint64_t> init_base{0};
+#endif
+    dump_cur("init");
+    engine->evaluate_all_sr(ds);
+    dump_cur("before");
+    const uint64_t hi = d32(shadow), lo = d32(shadow);
+    std::vector<uint64_t> base{(hi << 32) | lo};
+    engine->vary();
+    dump_cur("after");
+    write_vec(out + ".stream_base.u64", base);
+    write_vec(out + ".init_base.u64", init_base);
+    std::vector<int32_t> cfg{n, S, LGPConfig::ELITE_COUNT, LGPConfig::TOURNAMENT_SIZE,
+                             LGPConfig::NUM_REGISTERS, (int)LGPConfig::INPUT_PROTECT,
+                             (int)LGPConfig::EFFMUT, LGPConfig::MACRO_INSERTS,
+                             (int)LGPConfig::MACRO_RANDOM_POS, LGPConfig::STARTING_PROGRAM_SIZE};
+    std::vector<float> rates{LGPConfig::CROSSOVER_RATE, LGPConfig::MICRO_RATE,
+                             LGPConfig::MICRO_PER_INSTR, LGPConfig::EFFMUT_RATE,
+                             LGPConfig::INSERT_TAIL_RATE, LGPConfig::DELETE_TAIL_RATE};
+    write_vec(out + ".cfg.i32", cfg);
+    write_vec(out + ".rates.f32", rates);
+    std::printf("vary %s: P=%d stride=%d elites=%d tournament=%d\n", out.c_str(), n, S,
+                LGPConfig::ELITE_COUNT, LGPConfig::TOURNAMENT_SIZE);
     return 0;
 }
 
@@ -188,9 +251,12 @@ int main(int argc, char** argv) {
                             argv[7]);
     if (argc >= 5 && std::strcmp(argv[1], "engine") == 0)
         return dump_engine((uint32_t)std::strtoul(argv[2], nullptr, 10), argv[3], argv[4]);
+    if (argc >= 5 && std::strcmp(argv[1], "vary") == 0)
+        return dump_vary((uint32_t)std::strtoul(argv[2], nullptr, 10), argv[3], argv[4]);
     std::fprintf(stderr,
                  "usage: %s percase <seed> <n_programs> <max_len> <n_cases> <n_inputs> <out>\n"
-                 "       %s engine <seed> <dataset.csv> <out>\n",
-                 argv[0], argv[0]);
+                 "       %s engine <seed> <dataset.csv> <out>\n"
+                 "       %s vary <seed> <dataset.csv> <out>\n",
+                 argv[0], argv[0], argv[0]);
     return 2;
 }

@@ -13,6 +13,12 @@ Fixtures:
                      (src/Introns.h:28-46)
   percase_r16.npz    same with NUM_REGISTERS_OVERRIDE=16, 5 inputs
   percase_r16ip.npz  same with INPUT_PROTECT_OVERRIDE=1 as well
+  vary_default.npz   one LGPEngine::vary() of the default build (seed 7,
+                     nguyen1_train.csv): population, fitness and PCG32
+                     stream base before, next generation after
+  vary_em.npz        the same for the build with the effective-mutation
+                     bundle and the other variation arms (seed 8), with its
+                     per-program PCG32 init population
   engine_seed42.npz  LGPEngine(42).init_evolution() population and the
                      population after evolve_sr on datasets/nguyen1_train.csv
                      (the cuda_diff populations, tests/cuda_diff.cu:57-73,
@@ -87,7 +93,30 @@ def engine(seed, csv, out_name):
         arrays["inputs"] = _read(prefix, "inputs.f32", np.float32)
         arrays["targets"] = _read(prefix, "targets.f32", np.float32)
         arrays["ss_tot"] = _read(prefix, "ss_tot.f64", np.float64)
+        arrays["final_stats"] = _read(prefix, "stats.f64", np.float64)
         arrays["stride"] = np.array(stride)
+    np.savez_compressed(os.path.join(HERE, out_name), **arrays)
+    print("wrote", out_name)
+
+
+def vary(tool, seed, csv, out_name):
+    with tempfile.TemporaryDirectory() as tmp:
+        prefix = os.path.join(tmp, "v")
+        subprocess.run([os.path.join(REFBIN, tool), "vary", str(seed), csv, prefix], check=True)
+        cfg = _read(prefix, "cfg.i32", np.int32)
+        n, stride = int(cfg[0]), int(cfg[1])
+        arrays = {"cfg": cfg, "rates": _read(prefix, "rates.f32", np.float32),
+                  "stream_base": _read(prefix, "stream_base.u64", np.uint64),
+                  "init_base": _read(prefix, "init_base.u64", np.uint64)}
+        for tag in ("init", "before", "after"):
+            dense = _read(prefix, f"{tag}.dense.u32", np.uint32).reshape(n, stride)
+            lengths = _read(prefix, f"{tag}.lengths.u8", np.uint8)
+            # programs as CSR: only the first lengths[i] words of a slot are a program
+            arrays[f"{tag}_words"] = np.concatenate([dense[i, :lengths[i]] for i in range(n)])
+            arrays[f"{tag}_lengths"] = lengths
+            arrays[f"{tag}_fitness"] = _read(prefix, f"{tag}.fitness.f32", np.float32)
+            if tag == "init" and int(arrays["init_base"][0]) != 0:
+                arrays["init_dense"] = dense  # the per-program PCG32 initializer fills every slot
     np.savez_compressed(os.path.join(HERE, out_name), **arrays)
     print("wrote", out_name)
 
@@ -105,6 +134,9 @@ def main():
     percase("ref_dump_r16ip", 3, 256, 48, 64, 5, "percase_r16ip.npz",
             dict(num_registers=16, input_protect=1, op_set=0, seed=3))
     engine(42, os.path.join(args.ref, "datasets", "nguyen1_train.csv"), "engine_seed42.npz")
+    csv = os.path.join(args.ref, "datasets", "nguyen1_train.csv")
+    vary("ref_dump_r8", 7, csv, "vary_default.npz")
+    vary("ref_dump_em", 8, csv, "vary_em.npz")
     return 0
 
 

@@ -1,0 +1,128 @@
+"""One generation of the engine (galop_amd/csrc/lgp_engine.h), host build,
+against vectors the compiled reference engine wrote (tests/golden/):
+
+* vary_default / vary_em: LGPEngine::vary (src/LGPEngine.cpp:421-456) on an
+  evaluated population -- elites by fitness key, then per pair two
+  tournaments, crossover and mutation from a PCG32 stream -- for the default
+  build and for a build with effective mutation, input protection, three
+  insert attempts, per-instruction micro mutation and tail-only macro
+  operators.  The next generation must match word for word, with lengths and
+  fitness (elites' copied, NaN otherwise).
+* vary_em's init population: the per-program PCG32 initializer.
+* engine_seed42's evolved population: compute_stats of the final generation.
+
+A planted mistake (a shifted PCG32 stream) must break the match.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+from conftest import REPO, load_fixture
+
+HOSTLIB = os.path.join(REPO, "galop_amd", "libgalop_engine_host.so")
+
+
+def _host():
+    if not os.path.exists(HOSTLIB):
+        pytest.skip("libgalop_engine_host.so not built")
+    L = ctypes.CDLL(HOSTLIB)
+    vp = ctypes.c_void_p
+    L.gv_vary.argtypes = [vp, vp, vp, vp, vp, vp, vp, ctypes.c_uint64]
+    L.gv_init.argtypes = [vp, ctypes.c_uint64, vp, vp, vp]
+    L.gv_stats.argtypes = [vp, vp, ctypes.c_int64, ctypes.c_int32, vp]
+    return L
+
+
+def cfg_from_fixture(f):
+    from galop_amd._lib import galop_vary_cfg
+    c = galop_vary_cfg()
+    n, S, elite, tour, nr, ip, effmut, inserts, rpos, start = (int(v) for v in f["cfg"])
+    c.population, c.stride, c.elite_count, c.tournament_size = n, S, elite, tour
+    c.num_registers, c.input_protect, c.op_set, c.effmut = nr, ip, 0, effmut
+    c.macro_inserts, c.macro_random_pos, c.starting_length, c.reserved = inserts, rpos, start, 0
+    (c.crossover_rate, c.micro_rate, c.micro_per_instr, c.effmut_rate, c.insert_rate,
+     c.delete_rate) = (float(v) for v in f["rates"])
+    return c
+
+
+def dense_from_csr(words, lengths, stride):
+    n = lengths.size
+    dense = np.zeros((n, stride), np.uint32)
+    off = np.concatenate([[0], np.cumsum(lengths.astype(np.int64))])
+    for i in range(n):
+        dense[i, :lengths[i]] = words[off[i]:off[i + 1]]
+    return dense
+
+
+def csr_from_dense(dense, lengths):
+    return np.concatenate([dense[i, :lengths[i]] for i in range(lengths.size)])
+
+
+def assert_same_generation(f, nxt, nlen, nfit):
+    assert np.array_equal(nlen, f["after_lengths"])
+    assert np.array_equal(csr_from_dense(nxt, nlen), f["after_words"])
+    ref = f["after_fitness"]
+    assert np.array_equal(np.isnan(nfit), np.isnan(ref))
+    keep = ~np.isnan(ref)
+    assert np.array_equal(nfit[keep].view(np.int32), ref[keep].view(np.int32))
+
+
+def run_host_vary(L, f, base=None):
+    c = cfg_from_fixture(f)
+    n, S = c.population, c.stride
+    cur = dense_from_csr(f["before_words"], f["before_lengths"], S)
+    clen = np.ascontiguousarray(f["before_lengths"])
+    cfit = np.ascontiguousarray(f["before_fitness"])
+    nxt = np.zeros((n, S), np.uint32)
+    nlen = np.zeros(n, np.uint8)
+    nfit = np.zeros(n, np.float32)
+    b = int(f["stream_base"][0]) if base is None else base
+    L.gv_vary(ctypes.byref(c), cur.ctypes.data, clen.ctypes.data, cfit.ctypes.data,
+              nxt.ctypes.data, nlen.ctypes.data, nfit.ctypes.data, b)
+    return nxt, nlen, nfit
+
+
+@pytest.mark.parametrize("name", ["vary_default.npz", "vary_em.npz"])
+def test_vary_matches_reference(name):
+    L = _host()
+    f = load_fixture(name)
+    nxt, nlen, nfit = run_host_vary(L, f)
+    assert_same_generation(f, nxt, nlen, nfit)
+    # the generation exercised the operators: children differ from clones
+    assert nlen.mean() != f["before_lengths"].mean()
+
+
+def test_planted_stream_shift_is_caught():
+    L = _host()
+    f = load_fixture("vary_default.npz")
+    nxt, nlen, nfit = run_host_vary(L, f, base=int(f["stream_base"][0]) + 1)
+    with pytest.raises(AssertionError):
+        assert_same_generation(f, nxt, nlen, nfit)
+
+
+def test_pcg32_init_matches_reference():
+    L = _host()
+    f = load_fixture("vary_em.npz")
+    c = cfg_from_fixture(f)
+    n, S = c.population, c.stride
+    instr = np.zeros((n, S), np.uint32)
+    lens = np.zeros(n, np.uint8)
+    fit = np.zeros(n, np.float32)
+    L.gv_init(ctypes.byref(c), int(f["init_base"][0]), instr.ctypes.data, lens.ctypes.data,
+              fit.ctypes.data)
+    assert np.array_equal(instr, f["init_dense"])
+    assert np.array_equal(lens, f["init_lengths"])
+    assert np.isnan(fit).all()
+
+
+def test_stats_match_reference():
+    L = _host()
+    f = load_fixture("engine_seed42.npz")
+    fit = np.ascontiguousarray(f["evolved_fitness"])
+    lens = np.ascontiguousarray(f["evolved_lengths"])
+    out = np.zeros(6, np.float64)
+    L.gv_stats(fit.ctypes.data, lens.ctypes.data, fit.size, int(f["final_stats"][0]),
+               out.ctypes.data)
+    assert np.array_equal(out, f["final_stats"])

@@ -1,0 +1,132 @@
+"""The engine generation on the GPU (include/galop_engine.h) against the
+reference's vectors and against the host build of the same operators:
+
+* galop_vary_device on vary_default / vary_em (elites + all pairs) matches the
+  reference's next generation word for word;
+* galop_init_population_device matches the reference's PCG32 initializer;
+* galop_generation_stats_device matches compute_stats of engine_seed42;
+* a 200,000-program population with effective mutation, two generations
+  chained on the device, equals the host build (itself pinned to the
+  reference by tests/test_engine_host.py).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import load_fixture
+from test_engine_host import (_host, assert_same_generation, cfg_from_fixture, csr_from_dense,
+                              dense_from_csr)
+
+pytestmark = pytest.mark.gpu
+
+
+def _cfg_obj(c):
+    from galop_amd.engine import VaryCfg
+    return VaryCfg(**{k: getattr(c, k) for k in VaryCfg.FIELDS_I + VaryCfg.FIELDS_F})
+
+
+def _dev(a):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(a))
+    if t.dtype == torch.uint32:
+        t = t.view(torch.int32)
+    return t.cuda()
+
+
+def _u32(a):
+    return np.ascontiguousarray(a).view(np.int32)
+
+
+def _device_vary(ctx, cfg, cur, clen, cfit, base):
+    import torch
+    from galop_amd.engine import vary_device
+    from gpu_util import stream
+    s = stream()  # current before any tensor work, so everything below is ordered
+    n, S = cfg.population, cfg.stride
+    d_cur, d_len, d_fit = _dev(_u32(cur)), _dev(clen), _dev(cfit)
+    d_nxt = torch.zeros(n * S, dtype=torch.int32, device="cuda")
+    d_nlen = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    d_nfit = torch.zeros(n, dtype=torch.float32, device="cuda")
+    vary_device(ctx, cfg, d_cur, d_len, d_fit, d_nxt, d_nlen, d_nfit, base, stream=s)
+    torch.cuda.synchronize()
+    return (d_nxt.cpu().numpy().view(np.uint32).reshape(n, S), d_nlen.cpu().numpy(),
+            d_nfit.cpu().numpy())
+
+
+@pytest.mark.parametrize("name", ["vary_default.npz", "vary_em.npz"])
+def test_device_vary_matches_reference(gpu_ctx, name):
+    f = load_fixture(name)
+    c = cfg_from_fixture(f)
+    cur = dense_from_csr(f["before_words"], f["before_lengths"], c.stride)
+    nxt, nlen, nfit = _device_vary(gpu_ctx, _cfg_obj(c), cur, f["before_lengths"],
+                                   f["before_fitness"], int(f["stream_base"][0]))
+    assert_same_generation(f, nxt, nlen, nfit)
+
+
+def test_device_init_matches_reference(gpu_ctx):
+    import torch
+    from galop_amd.engine import init_population_device
+    from gpu_util import stream
+    s = stream()
+    f = load_fixture("vary_em.npz")
+    c = cfg_from_fixture(f)
+    n, S = c.population, c.stride
+    instr = torch.zeros(n * S, dtype=torch.int32, device="cuda")
+    lens = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    fit = torch.zeros(n, dtype=torch.float32, device="cuda")
+    init_population_device(gpu_ctx, _cfg_obj(c), int(f["init_base"][0]), instr, lens, fit, stream=s)
+    torch.cuda.synchronize()
+    assert np.array_equal(instr.cpu().numpy().view(np.uint32).reshape(n, S), f["init_dense"])
+    assert np.array_equal(lens.cpu().numpy(), f["init_lengths"])
+    assert torch.isnan(fit).all()
+
+
+def test_device_stats_match_reference(gpu_ctx):
+    from galop_amd.engine import generation_stats_device
+    from gpu_util import stream
+    s = stream()
+    f = load_fixture("engine_seed42.npz")
+    st = generation_stats_device(gpu_ctx, _dev(f["evolved_fitness"]), _dev(f["evolved_lengths"]),
+                                 f["evolved_fitness"].size, int(f["final_stats"][0]), stream=s)
+    got = [st["generation"], st["best_fitness"], st["mean_fitness"], st["best_length"],
+           st["mean_length"], st["best_index"]]
+    assert np.array_equal(np.array(got, np.float64), f["final_stats"])
+
+
+def test_large_population_two_generations_device_equals_host(gpu_ctx):
+    from galop_amd._lib import galop_vary_cfg
+    L = _host()
+    rng = np.random.default_rng(5)
+    c = galop_vary_cfg()
+    c.population, c.stride, c.elite_count, c.tournament_size = 200_000, 64, 10_000, 4
+    c.num_registers, c.input_protect, c.op_set, c.effmut = 16, 1, 1, 1
+    c.macro_inserts, c.macro_random_pos, c.starting_length, c.reserved = 2, 1, 8, 0
+    (c.crossover_rate, c.micro_rate, c.micro_per_instr, c.effmut_rate, c.insert_rate,
+     c.delete_rate) = (0.9, 0.8, 0.0, 0.7, 0.6, 0.5)
+    n, S = c.population, c.stride
+    lens = rng.integers(1, S + 1, n).astype(np.uint8)
+    cur = rng.integers(0, 2**32, (n, S), dtype=np.uint64).astype(np.uint32)
+    fit = rng.random(n).astype(np.float32)
+    fit[rng.integers(0, n, 5000)] = fit[0]  # ties: the index breaks them
+    bases = [0x0123456789ABCDEF, 0xFEDCBA9876543210]
+    h_cur, h_len, h_fit = cur, lens, fit
+    d_cur, d_len, d_fit = cur, lens, fit
+    cfg = _cfg_obj(c)
+    for g, base in enumerate(bases):
+        nxt = np.zeros((n, S), np.uint32)
+        nlen = np.zeros(n, np.uint8)
+        nfit = np.zeros(n, np.float32)
+        L.gv_vary(ctypes.byref(c), np.ascontiguousarray(h_cur).ctypes.data,
+                  np.ascontiguousarray(h_len).ctypes.data, np.ascontiguousarray(h_fit).ctypes.data,
+                  nxt.ctypes.data, nlen.ctypes.data, nfit.ctypes.data, base)
+        dn, dl, df = _device_vary(gpu_ctx, cfg, d_cur, d_len, d_fit, base)
+        assert np.array_equal(dl, nlen)
+        assert np.array_equal(csr_from_dense(dn, dl), csr_from_dense(nxt, nlen))
+        assert np.array_equal(np.isnan(df), np.isnan(nfit))
+        keep = ~np.isnan(nfit)
+        assert np.array_equal(df[keep].view(np.int32), nfit[keep].view(np.int32))
+        # the next generation needs fitness everywhere: give the children some
+        nfit = np.where(np.isnan(nfit), rng.random(n).astype(np.float32), nfit)
+        h_cur, h_len, h_fit = nxt, nlen, nfit
+        d_cur, d_len, d_fit = dn, dl, nfit
