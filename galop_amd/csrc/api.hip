@@ -421,6 +421,26 @@ int galop_eval_csr_device(galop_ctx* ctx, const galop_dataset* ds, const uint32_
     return set_error(ctx, GALOP_ERR_INVALID, "galop_eval_csr_device: unknown kernel variant");
 }
 
+int galop_eval_dense_device(galop_ctx* ctx, const galop_dataset* ds, const uint32_t* d_instructions,
+                            const uint8_t* d_lengths, int32_t stride, int64_t n_programs,
+                            float* d_fitness, double* d_sse, void* stream) {
+    int rc = check_eval_args(ctx, ds, n_programs, "galop_eval_dense_device");
+    if (rc != GALOP_OK) return rc;
+    if (n_programs == 0) return GALOP_OK;
+    if (!d_instructions || !d_lengths || !d_fitness)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_eval_dense_device: NULL device pointer");
+    if (stride <= 0 || stride > 255)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_eval_dense_device: stride outside 1..255");
+    GALOP_HIP(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    double* partials = nullptr;
+    GALOP_HIP(ctx, split_scratch(ctx, ds, n_programs, &partials));
+    GALOP_HIP(ctx, launch_plain(ds->dev, ds->isa.num_registers, ctx->tuning, d_instructions, nullptr,
+                                n_programs, d_fitness, d_sse, partials, ctx->n_cu, s, d_lengths,
+                                stride));
+    return GALOP_OK;
+}
+
 int galop_run_cases_device(galop_ctx* ctx, const galop_dataset* ds, const uint32_t* d_words,
                            const int64_t* d_offsets, int64_t n_programs, float* d_out,
                            void* stream) {

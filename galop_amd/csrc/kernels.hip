@@ -14,6 +14,22 @@
 
 namespace galop {
 
+// Where program p lives: CSR (words[offsets[p], offsets[p+1])), or, with
+// dense_len, the engine's dense layout (words[p*stride, p*stride + len[p]));
+// a dense length above the stride reads only the program's own slot.
+__device__ __forceinline__ void program_bounds(const int64_t* __restrict__ offsets,
+                                               const uint8_t* __restrict__ dense_len,
+                                               int32_t stride, int64_t p, int64_t& lo,
+                                               int64_t& hi) {
+    if (dense_len) {
+        lo = p * (int64_t)stride;
+        hi = lo + min((int32_t)dense_len[p], stride);
+    } else {
+        lo = offsets[p];
+        hi = offsets[p + 1];
+    }
+}
+
 // The interpreter kernels ask for at least 4 waves per SIMD (at most 128
 // VGPRs).  At 8 cases per lane the unrolled FP64 evaluations of a unary word
 // otherwise take 130-140 VGPRs and 3 waves; 4 waves with a few spilled
@@ -26,15 +42,15 @@ namespace galop {
 template <int NR, int K, int EXT>
 __global__ void GALOP_INTERP_BOUNDS
 plain_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ offsets,
-             int64_t first, int64_t n, DevData ds, float* __restrict__ fitness,
-             double* __restrict__ sse_out) {
+             const uint8_t* __restrict__ dense_len, int32_t stride, int64_t first, int64_t n,
+             DevData ds, float* __restrict__ fitness, double* __restrict__ sse_out) {
     const int lane = (int)(threadIdx.x & (kWave - 1));
     const int wave_in_block = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int64_t p = first + (int64_t)blockIdx.x * (int64_t)(blockDim.x >> 6) + wave_in_block;
     if (p >= n) return;  // wave-uniform
     const ConstRegs cr = load_const_regs(ds.consts, lane);
-    const int64_t lo = offsets[p];
-    const int64_t hi = offsets[p + 1];
+    int64_t lo, hi;
+    program_bounds(offsets, dense_len, stride, p, lo, hi);
     const double sse = eval_program_wave<NR, K, EXT>(words + lo, hi - lo, ds, lane, cr, nullptr);
     if (lane == 0) {
         fitness[p] = fitness_of_sse(sse, ds);
@@ -50,8 +66,9 @@ plain_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ off
 // fitness are the bits the plain kernel gives.
 template <int NR, int K, int EXT>
 __global__ void GALOP_INTERP_BOUNDS
-split_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ offsets, int64_t n,
-             int splits, int64_t bpw, DevData ds, double* __restrict__ partials) {
+split_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ offsets,
+             const uint8_t* __restrict__ dense_len, int32_t stride, int64_t n, int splits,
+             int64_t bpw, DevData ds, double* __restrict__ partials) {
     const int lane = (int)(threadIdx.x & (kWave - 1));
     const int wave_in_block = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int64_t w = (int64_t)blockIdx.x * (int64_t)(blockDim.x >> 6) + wave_in_block;
@@ -62,7 +79,8 @@ split_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ off
     const int64_t b1 = (b0 + bpw) < n_blocks ? (b0 + bpw) : n_blocks;
     const ConstRegs cr = load_const_regs(ds.consts, lane);
     const DestDecode dd = dest_decode(NR, ds.input_protect);
-    const int64_t lo = offsets[p], hi = offsets[p + 1];
+    int64_t lo, hi;
+    program_bounds(offsets, dense_len, stride, p, lo, hi);
     for (int64_t b = b0; b < b1; ++b) {
         const double part = block_sse<NR, K, EXT>(words + lo, hi - lo, ds, lane, cr, dd,
                                                   b * kCaseBlock, nullptr);
@@ -149,15 +167,16 @@ int64_t plain_split_partials(const DevData& ds, int64_t n, int n_cu) {
 
 template <int NR, int K, int EXT>
 static hipError_t launch_plain_t(const DevData& ds, int wpb, const uint32_t* words,
-                                 const int64_t* offsets, int64_t n, float* fitness,
-                                 double* sse, double* partials, int n_cu, hipStream_t stream) {
+                                 const int64_t* offsets, const uint8_t* dense_len,
+                                 int32_t stride, int64_t n, float* fitness, double* sse,
+                                 double* partials, int n_cu, hipStream_t stream) {
     int64_t bpw = 0;
     const int splits = partials ? plain_splits(ds, n, n_cu, &bpw) : 1;
     if (splits > 1) {
         const int64_t waves = n * splits;
         hipLaunchKernelGGL((split_kernel<NR, K, EXT>), dim3((unsigned)((waves + wpb - 1) / wpb)),
-                           dim3(wpb * kWave), 0, stream, words, offsets, n, splits, bpw, ds,
-                           partials);
+                           dim3(wpb * kWave), 0, stream, words, offsets, dense_len, stride, n,
+                           splits, bpw, ds, partials);
         hipLaunchKernelGGL(split_finish_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
                            stream, partials, n, (ds.n_cases + kCaseBlock - 1) / kCaseBlock,
                            ds, fitness, sse);
@@ -167,7 +186,8 @@ static hipError_t launch_plain_t(const DevData& ds, int wpb, const uint32_t* wor
         const int64_t cnt = (n - first) < kLaunchWindow ? (n - first) : kLaunchWindow;
         const unsigned grid = (unsigned)((cnt + wpb - 1) / wpb);
         hipLaunchKernelGGL((plain_kernel<NR, K, EXT>), dim3(grid), dim3(wpb * kWave), 0, stream,
-                           words, offsets, first, first + cnt, ds, fitness, sse);
+                           words, offsets, dense_len, stride, first, first + cnt, ds, fitness,
+                           sse);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
@@ -177,14 +197,14 @@ static hipError_t launch_plain_t(const DevData& ds, int wpb, const uint32_t* wor
 hipError_t launch_plain(const DevData& ds, int num_registers, PlainTuning t,
                         const uint32_t* words, const int64_t* offsets, int64_t n,
                         float* fitness, double* sse, double* partials, int n_cu,
-                        hipStream_t stream) {
+                        hipStream_t stream, const uint8_t* dense_len, int32_t stride) {
     if (n <= 0) return hipSuccess;
     t = resolve_plain_tuning(ds, num_registers, t);
     const int wpb = t.waves_per_block;
 #define GALOP_CASE(NRV, KV)                                                            \
     if (num_registers == NRV && t.cases_per_lane == KV)                                \
-        return ds.op_set ? launch_plain_t<NRV, KV, 1>(ds, wpb, words, offsets, n, fitness, sse, partials, n_cu, stream) \
-                         : launch_plain_t<NRV, KV, 0>(ds, wpb, words, offsets, n, fitness, sse, partials, n_cu, stream);
+        return ds.op_set ? launch_plain_t<NRV, KV, 1>(ds, wpb, words, offsets, dense_len, stride, n, fitness, sse, partials, n_cu, stream) \
+                         : launch_plain_t<NRV, KV, 0>(ds, wpb, words, offsets, dense_len, stride, n, fitness, sse, partials, n_cu, stream);
     GALOP_CASE(8, 1) GALOP_CASE(8, 2) GALOP_CASE(8, 4) GALOP_CASE(8, 8)
     GALOP_CASE(16, 1) GALOP_CASE(16, 2) GALOP_CASE(16, 4)
 #undef GALOP_CASE

@@ -8,6 +8,8 @@ VaryCfg mirrors the engine's compile-time configuration (src/LGPConfig.h);
 """
 import ctypes
 
+import numpy as np
+
 from . import _lib
 from ._lib import check
 
@@ -26,15 +28,20 @@ class VaryCfg:
             raise TypeError(f"unknown fields {sorted(kw)}")
 
     @classmethod
-    def default(cls, population=1000, stride=50):
+    def default(cls, population=1000, stride=50, **overrides):
         """src/LGPConfig.h defaults: tournament 3, 10% elites, crossover 1.0,
         micro 0.8, one insert attempt at 0.8, delete 0.75, random positions,
-        no effective mutation, starting length 8."""
-        return cls(population=population, stride=stride, elite_count=int(population * 0.1),
-                   tournament_size=3, num_registers=8, input_protect=0, op_set=_lib.OPS_REF8,
-                   effmut=0, macro_inserts=1, macro_random_pos=1, starting_length=8,
-                   crossover_rate=1.0, micro_rate=0.8, micro_per_instr=0.0, effmut_rate=1.0,
-                   insert_rate=0.8, delete_rate=0.75)
+        no effective mutation, starting length 8.  ELITE_COUNT is
+        static_cast<int>(POPULATION_SIZE * ELITES) in float arithmetic
+        (src/LGPConfig.h:251); keyword overrides replace any field."""
+        elites = int(np.float32(population) * np.float32(overrides.pop("elites", 0.1)))
+        kw = dict(population=population, stride=stride, elite_count=elites,
+                  tournament_size=3, num_registers=8, input_protect=0, op_set=_lib.OPS_REF8,
+                  effmut=0, macro_inserts=1, macro_random_pos=1, starting_length=8,
+                  crossover_rate=1.0, micro_rate=0.8, micro_per_instr=0.0, effmut_rate=1.0,
+                  insert_rate=0.8, delete_rate=0.75)
+        kw.update(overrides)
+        return cls(**kw)
 
     def to_c(self):
         s = _lib.galop_vary_cfg()

@@ -131,6 +131,17 @@ int galop_eval_csr_device(galop_ctx* ctx, const galop_dataset* ds,
                           const uint32_t* d_words, const int64_t* d_offsets,
                           int64_t n_programs, float* d_fitness, double* d_sse,
                           int32_t kernel, void* stream);
+/* evaluate_kernel (src/EvaluatorCuda.cu:163-190) on a device-resident
+ * population in the engine's dense layout: program i is
+ * d_instructions[i*stride, i*stride + d_lengths[i]), stride 1..255 (a length
+ * above the stride reads the stride's words).  Fitness, and the FP64 SSE when
+ * d_sse is not NULL, of programs [0, n_programs); evaluate a window by
+ * offsetting all three pointers.  Plain kernel (case-split for populations
+ * too small to fill the GPU). */
+int galop_eval_dense_device(galop_ctx* ctx, const galop_dataset* ds,
+                            const uint32_t* d_instructions, const uint8_t* d_lengths,
+                            int32_t stride, int64_t n_programs, float* d_fitness,
+                            double* d_sse, void* stream);
 /* Per-case outputs r[OUTPUT_REG] (src/Interpreter.cpp:55-57) of every program
  * on every case: d_out[p*n_cases + c]. */
 int galop_run_cases_device(galop_ctx* ctx, const galop_dataset* ds,

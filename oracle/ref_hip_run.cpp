@@ -11,7 +11,11 @@
 // The reference's fixed starting length is kept (STARTING_PROGRAM_SIZE = 8);
 // it has no option for initial lengths drawn from a range.
 //
-// usage: ref_hip_run <cpu|hip|hip-effective> <seed> <history.csv>
+// With a fourth argument, also writes <prefix>.inputs.f32, .targets.f32 (the
+// dataset's N cases) and the final generation: .words.u32 (dense, stride
+// MAX_PROGRAM_SIZE), .lengths.u8, .fitness.f32.
+//
+// usage: ref_hip_run <cpu|hip|hip-effective> <seed> <history.csv> [prefix]
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -58,9 +62,18 @@ static Dataset make_dataset() {
     return ds;
 }
 
+template <class T>
+static bool write_raw(const std::string& path, const T* p, size_t n) {
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    const bool ok = std::fwrite(p, sizeof(T), n, f) == n;
+    return std::fclose(f) == 0 && ok;
+}
+
 int main(int argc, char** argv) {
-    if (argc != 4) {
-        std::fprintf(stderr, "usage: %s <cpu|hip|hip-effective> <seed> <history.csv>\n", argv[0]);
+    if (argc != 4 && argc != 5) {
+        std::fprintf(stderr, "usage: %s <cpu|hip|hip-effective> <seed> <history.csv> [prefix]\n",
+                     argv[0]);
         return 2;
     }
     const std::string mode = argv[1];
@@ -81,6 +94,21 @@ int main(int argc, char** argv) {
         std::fprintf(f, "%d,%.9g,%.9g,%d,%.6g\n", h.generation, h.best_fitness, h.mean_fitness,
                      h.best_length, h.mean_length);
     std::fclose(f);
+    if (argc == 5) {
+        const std::string pre = argv[4];
+        const PopulationData& data = engine.get_data();
+        const int buf = engine.current_buffer_index();
+        const bool ok =
+            write_raw(pre + ".inputs.f32", ds.inputs.data(), (size_t)ds.N) &&
+            write_raw(pre + ".targets.f32", ds.targets.data(), (size_t)ds.N) &&
+            write_raw(pre + ".words.u32", data.instructions_buf[buf].data(),
+                      data.instructions_buf[buf].size()) &&
+            write_raw(pre + ".lengths.u8", data.program_lengths_buf[buf].data(),
+                      data.program_lengths_buf[buf].size()) &&
+            write_raw(pre + ".fitness.f32", data.fitness_scores_buf[buf].data(),
+                      data.fitness_scores_buf[buf].size());
+        if (!ok) return 2;
+    }
     std::printf("%s seed %u: %zu generations, final best fitness %.9g\n", mode.c_str(), seed,
                 engine.history_view().size(), engine.history_view().back().best_fitness);
     return 0;

@@ -19,6 +19,14 @@ Fixtures:
   vary_em.npz        the same for the build with the effective-mutation
                      bundle and the other variation arms (seed 8), with its
                      per-program PCG32 init population
+  evolve_c1_s<seed>.npz
+                     the whole LGPEngine::evolve_sr run of the engine-only
+                     regression configuration (1,000 programs, max length 64,
+                     tournament 4, 50 generations; oracle/ref_hip_run.cpp)
+                     with this repository's HipEvaluator behind EvalBackend:
+                     the dataset, the per-generation history and the final
+                     generation.  Needs the GPU (--evolve-only --out DIR runs
+                     just this part, from an already built oracle/_ref)
   engine_seed42.npz  LGPEngine(42).init_evolution() population and the
                      population after evolve_sr on datasets/nguyen1_train.csv
                      (the cuda_diff populations, tests/cuda_diff.cu:57-73,
@@ -121,10 +129,46 @@ def vary(tool, seed, csv, out_name):
     print("wrote", out_name)
 
 
+def evolve(seed, out_dir, mode="hip"):
+    with tempfile.TemporaryDirectory() as tmp:
+        prefix = os.path.join(tmp, "d")
+        csv = os.path.join(tmp, "history.csv")
+        subprocess.run([os.path.join(REFBIN, "ref_hip_run"), mode, str(seed), csv, prefix],
+                       check=True)
+        with open(csv) as fh:
+            rows = [line.strip().split(",") for line in fh.readlines()[1:]]
+        lengths = _read(prefix, "lengths.u8", np.uint8)
+        n = lengths.size
+        dense = _read(prefix, "words.u32", np.uint32).reshape(n, -1)
+        arrays = dict(
+            seed=np.int64(seed),
+            inputs=_read(prefix, "inputs.f32", np.float32),
+            targets=_read(prefix, "targets.f32", np.float32),
+            generation=np.array([int(r[0]) for r in rows], np.int32),
+            best_fitness=np.array([float(r[1]) for r in rows], np.float32),
+            mean_fitness=np.array([float(r[2]) for r in rows], np.float32),
+            best_length=np.array([int(r[3]) for r in rows], np.int32),
+            mean_length_text=np.array([r[4] for r in rows]),
+            stride=np.int64(dense.shape[1]),
+            final_words=np.concatenate([dense[i, :lengths[i]] for i in range(n)]),
+            final_lengths=lengths,
+            final_fitness=_read(prefix, "fitness.f32", np.float32))
+    name = os.path.join(out_dir, f"evolve_c1_s{seed}.npz")
+    np.savez_compressed(name, **arrays)
+    print("wrote", name)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
+    ap.add_argument("--evolve-only", action="store_true",
+                    help="only the evolve_c1 fixtures (needs the GPU and oracle/_ref)")
+    ap.add_argument("--out", default=HERE)
     args = ap.parse_args()
+    if args.evolve_only:
+        for seed in (3, 11):
+            evolve(seed, args.out)
+        return 0
     subprocess.run(["make", "-C", os.path.join(REPO, "oracle"), "ref", f"REF={args.ref}"],
                    check=True)
     percase("ref_dump_r8", 1, 512, 48, 64, 1, "percase_r8.npz",
