@@ -150,6 +150,102 @@ def bench_uneven(ctx, steps, warmup, programs=0):
     return res
 
 
+def quartic_dataset(D):
+    """The reference's speed-grid dataset (tools/speed_grid.sh:36-46): Python's
+    random seeded 1234, x = uniform(-1, 1), y = x^4 + x^3 + x^2 + x, D cases."""
+    import random
+    from galop_amd import Dataset
+    rnd = random.Random(1234)
+    xs = [rnd.uniform(-1, 1) for _ in range(D)]
+    ys = [x ** 4 + x ** 3 + x ** 2 + x for x in xs]
+    return Dataset(np.array(xs, np.float32), np.array(ys, np.float32))
+
+
+def bench_engine(ctx, population, cases, generations, seeds):
+    """Whole evolutionary runs with the population resident in HBM
+    (galop_amd/evolve.py), on the reference's host-share cell: the standard
+    configuration (src/LGPConfig.h:29-36: input protection, 3 insert
+    attempts; 8 registers, max length 50, tournament 3, 10% elites), P
+    programs, the quartic dataset at D cases, `generations` generations, one
+    run per seed after a discarded warm-up run.  Wall time of each run
+    (initialisation included), and per-phase times from HIP events for the
+    last run."""
+    import torch
+    from galop_amd.engine import VaryCfg
+    from galop_amd.evolve import DeviceEngine
+    cfg = VaryCfg.default(population, stride=50, input_protect=1, macro_inserts=3)
+    ds = quartic_dataset(cases)
+    cur = torch.cuda.current_stream()
+    walls, hist = [], None
+    eng = DeviceEngine(ctx, ds, cfg, seeds[0], generations, stream=cur)
+    eng.max_generations = 3
+    eng.evolve_sr()  # warm-up: sizes the library's scratch
+    eng.max_generations = generations
+    for seed in seeds:
+        eng.seed = seed
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hist = eng.evolve_sr()
+        torch.cuda.synchronize()
+        walls.append(time.perf_counter() - t0)
+    # the last seed again, phase by phase
+    E, P = cfg.elite_count, cfg.population
+    t0 = time.perf_counter()
+    eng.init_evolution()
+    torch.cuda.synchronize()
+    t_init = time.perf_counter() - t0
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    eng.evaluate_all_sr()
+    ev[1].record()
+    eng.history.append(eng.compute_stats())
+    t_eval0 = ev[0].elapsed_time(ev[1])
+    marks = []
+    for gen in range(1, generations):
+        eng.current_generation = gen
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        e[0].record()
+        eng.vary()
+        e[1].record()
+        eng.evaluate_range(E, P - E)
+        e[2].record()
+        eng.history.append(eng.compute_stats())
+        e[3].record()
+        marks.append(e)
+    torch.cuda.synchronize()
+    vary_ms = sum(m[0].elapsed_time(m[1]) for m in marks)
+    eval_ms = sum(m[1].elapsed_time(m[2]) for m in marks) + t_eval0
+    stats_ms = sum(m[2].elapsed_time(m[3]) for m in marks)
+    same = [(h["best_fitness"], h["mean_fitness"]) for h in eng.history] == \
+        [(h["best_fitness"], h["mean_fitness"]) for h in hist]
+    mean_len = float(np.mean([h["mean_length"] for h in eng.history]))
+    eng.close()
+    walls_sorted = sorted(walls)
+    med = walls_sorted[len(walls) // 2]
+    raw_ops = sum(float(h["mean_length"]) * P * cases for h in hist)
+    return {
+        "workload": f"standard configuration, P={P}, max length 50, quartic dataset D={cases}, "
+                    f"{generations} generations, seeds {seeds[0]}..{seeds[-1]}",
+        "run_s_median": round(med, 4), "run_s": [round(w, 4) for w in walls],
+        "spread": round((walls_sorted[-1] - walls_sorted[0]) / med, 3),
+        "ms_per_generation": round(med * 1e3 / generations, 3),
+        "phases_last_run_s": {"init_host_draws_and_upload": round(t_init, 4),
+                              "eval": round(eval_ms / 1e3, 4),
+                              "elites_selection_variation": round(vary_ms / 1e3, 4),
+                              "stats_incl_readback": round(stats_ms / 1e3, 4)},
+        "whole_run_raw_gpops": round(raw_ops / med / 1e9, 2),
+        "mean_length_over_run": round(mean_len, 2),
+        "final_best_fitness": hist[-1]["best_fitness"],
+        "phase_run_history_equal": bool(same),
+        "reference_h100": {"whole_run_s": 2.53, "eval_s": 1.011, "kernel_s": 0.19,
+                           "selection_variation_s": 0.99, "topk_s": 0.13, "elites_s": 0.046,
+                           "stats_s": 0.024,
+                           "source": "results_h100/breakdown_h100.csv:62-66 (H100 + 16 host "
+                                     "threads, medians of 5 seeds)"},
+        "speedup_vs_reference_whole_run": round(2.53 / med, 2),
+    }
+
+
 def bench_casebound(ctx, programs, steps, warmup, threads):
     """The case-bound population at full shape (1,048,576 cases of 5 inputs,
     16 registers, 128-instruction programs), timed on its first `programs`
@@ -233,6 +329,12 @@ def main():
     ap.add_argument("--no-casebound", action="store_true",
                     help="skip the case-bound sample (1,048,576 cases) with its FP64 oracle check")
     ap.add_argument("--casebound-programs", type=int, default=128)
+    ap.add_argument("--no-engine", action="store_true",
+                    help="skip the device-resident evolutionary runs")
+    ap.add_argument("--engine-population", type=int, default=1_000_000)
+    ap.add_argument("--engine-generations", type=int, default=100)
+    ap.add_argument("--engine-only", action="store_true",
+                    help="only the device-resident evolutionary runs")
     ap.add_argument("--no-host-entry", action="store_true",
                     help="skip timing the host-pointer entry point (pinned staging + H2D + D2H)")
     args = ap.parse_args()
@@ -260,6 +362,15 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] calibration copy of {args.calibrate_copy} bytes done")
         del src, dst
+    if args.engine_only:
+        res = bench_engine(ctx, args.engine_population, 32, args.engine_generations,
+                           [70, 71, 72, 73, 74])
+        print(json.dumps({"metric": "device-resident evolutionary run: seconds per run",
+                          "value": res["run_s_median"], "unit": "s", "n_gpus": 1,
+                          "higher_is_better": False, "dtype": "f32",
+                          "data": "the reference's quartic dataset (regenerated)",
+                          "engine": res}), flush=True)
+        return 0
     if args.serving_only:
         from galop_amd.serving_mix import run_mix
         res = run_mix(ctx, n_runs=args.serving_runs, generations=args.serving_generations,
@@ -428,6 +539,12 @@ def main():
         res = bench_casebound(ctx, args.casebound_programs, 1, 1, threads)
         log(f"[bench] casebound: {res}")
         line["casebound"] = res
+    if not args.no_engine and args.workload == "plain":
+        log("[bench] device-resident evolutionary runs")
+        res = bench_engine(ctx, args.engine_population, 32, args.engine_generations,
+                           [70, 71, 72, 73, 74])
+        log(f"[bench] engine: {res}")
+        line["engine"] = res
     if not args.no_serving and args.workload == "plain":
         from galop_amd.serving_mix import run_mix
         log("[bench] serving mix")

@@ -25,6 +25,7 @@ FITNESS_MSE = 0
 FITNESS_R2 = 1
 KERNEL_PLAIN = 0
 KERNEL_BATCHED = 1
+KERNEL_EFFECTIVE = 2
 LAYOUT_PACKED = 1
 LAYOUT_EFFECTIVE = 2
 BATCH_STRIP_INTRONS = 1
@@ -83,7 +84,8 @@ SIGNATURES = {
     "galop_eval_dense": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i64, _vp, _i32]),
     "galop_eval_csr": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "galop_eval_csr_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _i32, _vp]),
-    "galop_eval_dense_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i64, _vp, _vp, _vp]),
+    "galop_eval_dense_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i64, _vp, _vp, _i32,
+                                               _vp]),
     "galop_run_cases_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "galop_strip_introns_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _pi64,
                                                   _vp]),

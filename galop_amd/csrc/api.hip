@@ -400,11 +400,12 @@ int galop_eval_csr_device(galop_ctx* ctx, const galop_dataset* ds, const uint32_
         return set_error(ctx, GALOP_ERR_INVALID, "galop_eval_csr_device: NULL device pointer");
     GALOP_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-    if (kernel == GALOP_KERNEL_PLAIN) {
+    if (kernel == GALOP_KERNEL_PLAIN || kernel == GALOP_KERNEL_EFFECTIVE) {
         double* partials = nullptr;
         GALOP_HIP(ctx, split_scratch(ctx, ds, n_programs, &partials));
         GALOP_HIP(ctx, launch_plain(ds->dev, ds->isa.num_registers, ctx->tuning, d_words, d_offsets,
-                                    n_programs, d_fitness, d_sse, partials, ctx->n_cu, s));
+                                    n_programs, d_fitness, d_sse, partials, ctx->n_cu, s, nullptr,
+                                    0, kernel == GALOP_KERNEL_EFFECTIVE));
         return GALOP_OK;
     }
     if (kernel == GALOP_KERNEL_BATCHED) {
@@ -423,7 +424,7 @@ int galop_eval_csr_device(galop_ctx* ctx, const galop_dataset* ds, const uint32_
 
 int galop_eval_dense_device(galop_ctx* ctx, const galop_dataset* ds, const uint32_t* d_instructions,
                             const uint8_t* d_lengths, int32_t stride, int64_t n_programs,
-                            float* d_fitness, double* d_sse, void* stream) {
+                            float* d_fitness, double* d_sse, int32_t layout, void* stream) {
     int rc = check_eval_args(ctx, ds, n_programs, "galop_eval_dense_device");
     if (rc != GALOP_OK) return rc;
     if (n_programs == 0) return GALOP_OK;
@@ -431,13 +432,16 @@ int galop_eval_dense_device(galop_ctx* ctx, const galop_dataset* ds, const uint3
         return set_error(ctx, GALOP_ERR_INVALID, "galop_eval_dense_device: NULL device pointer");
     if (stride <= 0 || stride > 255)
         return set_error(ctx, GALOP_ERR_INVALID, "galop_eval_dense_device: stride outside 1..255");
+    if (layout != GALOP_LAYOUT_PACKED && layout != GALOP_LAYOUT_EFFECTIVE)
+        return set_error(ctx, GALOP_ERR_INVALID,
+                         "galop_eval_dense_device: layout must be PACKED or EFFECTIVE");
     GALOP_HIP(ctx, hipSetDevice(ctx->device));
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     double* partials = nullptr;
     GALOP_HIP(ctx, split_scratch(ctx, ds, n_programs, &partials));
     GALOP_HIP(ctx, launch_plain(ds->dev, ds->isa.num_registers, ctx->tuning, d_instructions, nullptr,
                                 n_programs, d_fitness, d_sse, partials, ctx->n_cu, s, d_lengths,
-                                stride));
+                                stride, layout == GALOP_LAYOUT_EFFECTIVE));
     return GALOP_OK;
 }
 

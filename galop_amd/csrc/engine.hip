@@ -100,57 +100,89 @@ __global__ void init_kernel(ge::VaryCfg c, uint64_t base, uint32_t* __restrict__
     fit[i] = ge::bits_to_float(0x7fc00000u);
 }
 
-// LGPEngine::compute_stats: thread t sums chunk t in index order; thread 0
-// then combines the 256 chunks in order.
+// LGPEngine::compute_stats (src/LGPEngine.cpp:477-517) over 256 fixed chunks.
+// One wave per chunk: the wave loads 64 consecutive programs at a time
+// (coalesced) and adds their fitness to one FP64 running sum strictly in index
+// order, which is the reference's order within a chunk; the length sum is exact
+// in any order, and the first maximum (strict >, from -1) is reduced across
+// lanes by value, then lowest index.
 constexpr int kStatChunks = 256;
-__global__ void __launch_bounds__(kStatChunks)
-stats_kernel(const float* __restrict__ fit, const uint8_t* __restrict__ len, int64_t P,
-             int32_t generation, galop_generation_stats* __restrict__ out) {
-    __shared__ double csum[kStatChunks];
-    __shared__ long long clen[kStatChunks];
-    __shared__ float cbest[kStatChunks];
-    __shared__ int cidx[kStatChunks];
-    const int c = (int)threadIdx.x;
-    const long long lo = P * c / kStatChunks, hi = P * (c + 1) / kStatChunks;
+struct StatChunk {
+    double sum;
+    long long len;
+    float best;
+    int idx;
+};
+
+__global__ void __launch_bounds__(64)
+stats_chunk_kernel(const float* __restrict__ fit, const uint8_t* __restrict__ len, int64_t P,
+                   StatChunk* __restrict__ chunks) {
+    const int c = (int)blockIdx.x;
+    const int lane = (int)threadIdx.x;
+    const int64_t lo = P * c / kStatChunks, hi = P * (c + 1) / kStatChunks;
     double s = 0.0;
     long long ls = 0;
     float b = -1.0f;
-    int bi = (int)lo;
-    for (long long i = lo; i < hi; ++i) {
-        ls += len[i];
-        s += (double)fit[i];
-        if (fit[i] > b) {
-            b = fit[i];
-            bi = (int)i;
-        }
-    }
-    csum[c] = s;
-    clen[c] = ls;
-    cbest[c] = b;
-    cidx[c] = bi;
-    __syncthreads();
-    if (c == 0) {
-        double sum = 0.0;
-        long long len_sum = 0;
-        float best = -1.0f;
-        int best_idx = 0;
-        for (int k = 0; k < kStatChunks; ++k) {
-            sum += csum[k];
-            len_sum += clen[k];
-            if (cbest[k] > best) {
-                best = cbest[k];
-                best_idx = cidx[k];
+    int64_t bi = INT64_MAX;
+    for (int64_t base = lo; base < hi; base += 64) {
+        const int64_t i = base + lane;
+        float f = 0.0f;
+        if (i < hi) {
+            f = fit[i];
+            ls += len[i];
+            if (f > b) {
+                b = f;
+                bi = i;
             }
         }
-        galop_generation_stats st;
-        st.generation = generation;
-        st.best_fitness = best;
-        st.mean_fitness = (float)(sum / (double)P);
-        st.best_length = (int32_t)len[best_idx];
-        st.mean_length = (float)((double)len_sum / (double)P);
-        st.best_index = best_idx;
-        *out = st;
+        const int n = (int)min((int64_t)64, hi - base);  // wave-uniform
+        const int fb = __float_as_int(f);
+        for (int j = 0; j < n; ++j) s += (double)__int_as_float(__builtin_amdgcn_readlane(fb, j));
     }
+    for (int off = 32; off >= 1; off >>= 1) {
+        ls += __shfl_xor(ls, off);
+        const float ob = __shfl_xor(b, off);
+        const int64_t oi = __shfl_xor(bi, off);
+        if (ob > b || (ob == b && oi < bi)) {
+            b = ob;
+            bi = oi;
+        }
+    }
+    if (lane == 0) {
+        StatChunk r;
+        r.sum = s;
+        r.len = ls;
+        r.best = b;
+        r.idx = (int)(b > -1.0f ? bi : lo);  // nothing above -1: the chunk's first index
+        chunks[c] = r;
+    }
+}
+
+// The fixed combine order of the chunks; strict > keeps the lowest index.
+__global__ void stats_finish_kernel(const StatChunk* __restrict__ chunks,
+                                    const uint8_t* __restrict__ len, int64_t P,
+                                    int32_t generation, galop_generation_stats* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double sum = 0.0;
+    long long len_sum = 0;
+    float best = -1.0f;
+    int best_idx = 0;
+    for (int k = 0; k < kStatChunks; ++k) {
+        sum += chunks[k].sum;
+        len_sum += chunks[k].len;
+        if (chunks[k].best > best) {
+            best = chunks[k].best;
+            best_idx = chunks[k].idx;
+        }
+    }
+    galop_generation_stats st;
+    st.generation = generation;
+    st.best_fitness = best;
+    st.mean_fitness = (float)(sum / (double)P);
+    st.best_length = (int32_t)len[best_idx];
+    st.mean_length = (float)((double)len_sum / (double)P);
+    st.best_index = best_idx;
+    *out = st;
 }
 
 }  // namespace
@@ -224,11 +256,15 @@ int galop_generation_stats_device(galop_ctx* ctx, const float* d_fit, const uint
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
     Staging& st = ctx->staging;
-    e = st.d_small.reserve(sizeof(galop_generation_stats));
+    e = st.d_small.reserve(kStatChunks * sizeof(StatChunk) + sizeof(galop_generation_stats));
     if (e != hipSuccess) return hip_fail(ctx, e, "galop_generation_stats_device: scratch");
-    auto* d_out = reinterpret_cast<galop_generation_stats*>(st.d_small.ptr);
+    auto* d_chunks = reinterpret_cast<StatChunk*>(st.d_small.ptr);
+    auto* d_out = reinterpret_cast<galop_generation_stats*>(st.d_small.ptr +
+                                                            kStatChunks * sizeof(StatChunk));
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-    hipLaunchKernelGGL(stats_kernel, dim3(1), dim3(kStatChunks), 0, s, d_fit, d_len,
+    hipLaunchKernelGGL(stats_chunk_kernel, dim3(kStatChunks), dim3(64), 0, s, d_fit, d_len,
+                       (int64_t)population, d_chunks);
+    hipLaunchKernelGGL(stats_finish_kernel, dim3(1), dim3(64), 0, s, d_chunks, d_len,
                        (int64_t)population, generation, d_out);
     e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpyAsync(out, d_out, sizeof(*out), hipMemcpyDeviceToHost, s);

@@ -26,12 +26,14 @@ PlainTuning resolve_plain_tuning(const DevData& ds, int num_registers, PlainTuni
 // or, with dense_len, of the engine's dense layout (program p at
 // words + p*stride, dense_len[p] words; offsets unused).  With `partials`
 // (plain_split_partials(ds, n, n_cu) doubles), a population too small to fill
-// the GPU is evaluated case-split instead (same bits).
+// the GPU is evaluated case-split instead (same bits).  effective_only: the
+// words of programs of at most 256 words that cannot reach the output
+// register are skipped (same bits).
 hipError_t launch_plain(const DevData& ds, int num_registers, PlainTuning t,
                         const uint32_t* words, const int64_t* offsets, int64_t n,
                         float* fitness, double* sse, double* partials, int n_cu,
                         hipStream_t stream, const uint8_t* dense_len = nullptr,
-                        int32_t stride = 0);
+                        int32_t stride = 0, bool effective_only = false);
 // Block partials launch_plain needs for a case-split launch (0: no split).
 int64_t plain_split_partials(const DevData& ds, int64_t n, int n_cu);
 

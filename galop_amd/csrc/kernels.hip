@@ -39,7 +39,9 @@ __device__ __forceinline__ void program_bounds(const int64_t* __restrict__ offse
 // ---------------------------------------------------------------------------
 // plain kernel
 // ---------------------------------------------------------------------------
-template <int NR, int K, int EXT>
+// EFF: programs of at most kSkipMaxWords words run their effective words only
+// (bitwise the same fitness).
+template <int NR, int K, int EXT, int EFF>
 __global__ void GALOP_INTERP_BOUNDS
 plain_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ offsets,
              const uint8_t* __restrict__ dense_len, int32_t stride, int64_t first, int64_t n,
@@ -51,7 +53,8 @@ plain_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ off
     const ConstRegs cr = load_const_regs(ds.consts, lane);
     int64_t lo, hi;
     program_bounds(offsets, dense_len, stride, p, lo, hi);
-    const double sse = eval_program_wave<NR, K, EXT>(words + lo, hi - lo, ds, lane, cr, nullptr);
+    const double sse =
+        eval_program_wave<NR, K, EXT, EFF>(words + lo, hi - lo, ds, lane, cr, nullptr);
     if (lane == 0) {
         fitness[p] = fitness_of_sse(sse, ds);
         if (sse_out) sse_out[p] = sse;
@@ -64,7 +67,7 @@ plain_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ off
 // per block.  split_finish_kernel then sums each program's partials in block
 // order, the same FP64 additions eval_program_wave makes, so the SSE and
 // fitness are the bits the plain kernel gives.
-template <int NR, int K, int EXT>
+template <int NR, int K, int EXT, int EFF>
 __global__ void GALOP_INTERP_BOUNDS
 split_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ offsets,
              const uint8_t* __restrict__ dense_len, int32_t stride, int64_t n, int splits,
@@ -81,6 +84,17 @@ split_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ off
     const DestDecode dd = dest_decode(NR, ds.input_protect);
     int64_t lo, hi;
     program_bounds(offsets, dense_len, stride, p, lo, hi);
+    if (EFF) {
+        EffProgram e;
+        if (hi - lo <= kSkipMaxWords)
+            mark_effective<NR, EXT>(words + lo, hi - lo, lane, ds.out_reg, ds.input_protect, e);
+        for (int64_t b = b0; b < b1; ++b) {
+            const double part = block_sse_eff<NR, K, EXT>(words + lo, hi - lo, e, ds, lane, cr,
+                                                          dd, b * kCaseBlock, nullptr);
+            if (lane == 0) partials[p * n_blocks + b] = part;
+        }
+        return;
+    }
     for (int64_t b = b0; b < b1; ++b) {
         const double part = block_sse<NR, K, EXT>(words + lo, hi - lo, ds, lane, cr, dd,
                                                   b * kCaseBlock, nullptr);
@@ -165,7 +179,7 @@ int64_t plain_split_partials(const DevData& ds, int64_t n, int n_cu) {
     return n * ((ds.n_cases + kCaseBlock - 1) / kCaseBlock);
 }
 
-template <int NR, int K, int EXT>
+template <int NR, int K, int EXT, int EFF>
 static hipError_t launch_plain_t(const DevData& ds, int wpb, const uint32_t* words,
                                  const int64_t* offsets, const uint8_t* dense_len,
                                  int32_t stride, int64_t n, float* fitness, double* sse,
@@ -174,7 +188,7 @@ static hipError_t launch_plain_t(const DevData& ds, int wpb, const uint32_t* wor
     const int splits = partials ? plain_splits(ds, n, n_cu, &bpw) : 1;
     if (splits > 1) {
         const int64_t waves = n * splits;
-        hipLaunchKernelGGL((split_kernel<NR, K, EXT>), dim3((unsigned)((waves + wpb - 1) / wpb)),
+        hipLaunchKernelGGL((split_kernel<NR, K, EXT, EFF>), dim3((unsigned)((waves + wpb - 1) / wpb)),
                            dim3(wpb * kWave), 0, stream, words, offsets, dense_len, stride, n,
                            splits, bpw, ds, partials);
         hipLaunchKernelGGL(split_finish_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
@@ -185,7 +199,7 @@ static hipError_t launch_plain_t(const DevData& ds, int wpb, const uint32_t* wor
     for (int64_t first = 0; first < n; first += kLaunchWindow) {
         const int64_t cnt = (n - first) < kLaunchWindow ? (n - first) : kLaunchWindow;
         const unsigned grid = (unsigned)((cnt + wpb - 1) / wpb);
-        hipLaunchKernelGGL((plain_kernel<NR, K, EXT>), dim3(grid), dim3(wpb * kWave), 0, stream,
+        hipLaunchKernelGGL((plain_kernel<NR, K, EXT, EFF>), dim3(grid), dim3(wpb * kWave), 0, stream,
                            words, offsets, dense_len, stride, first, first + cnt, ds, fitness,
                            sse);
         const hipError_t e = hipGetLastError();
@@ -197,17 +211,31 @@ static hipError_t launch_plain_t(const DevData& ds, int wpb, const uint32_t* wor
 hipError_t launch_plain(const DevData& ds, int num_registers, PlainTuning t,
                         const uint32_t* words, const int64_t* offsets, int64_t n,
                         float* fitness, double* sse, double* partials, int n_cu,
-                        hipStream_t stream, const uint8_t* dense_len, int32_t stride) {
+                        hipStream_t stream, const uint8_t* dense_len, int32_t stride,
+                        bool effective_only) {
     if (n <= 0) return hipSuccess;
     t = resolve_plain_tuning(ds, num_registers, t);
+    // the effective-word variant keeps four chunk registers and the masks
+    // live: at the largest tile of each register count it would spill the
+    // register file, so it runs the next tile down
+    if (effective_only) t.cases_per_lane = std::min(t.cases_per_lane, num_registers == 8 ? 4 : 2);
     const int wpb = t.waves_per_block;
+#define GALOP_ARGS ds, wpb, words, offsets, dense_len, stride, n, fitness, sse, partials, n_cu, stream
 #define GALOP_CASE(NRV, KV)                                                            \
-    if (num_registers == NRV && t.cases_per_lane == KV)                                \
-        return ds.op_set ? launch_plain_t<NRV, KV, 1>(ds, wpb, words, offsets, dense_len, stride, n, fitness, sse, partials, n_cu, stream) \
-                         : launch_plain_t<NRV, KV, 0>(ds, wpb, words, offsets, dense_len, stride, n, fitness, sse, partials, n_cu, stream);
+    if (num_registers == NRV && t.cases_per_lane == KV && !effective_only)             \
+        return ds.op_set ? launch_plain_t<NRV, KV, 1, 0>(GALOP_ARGS)                   \
+                         : launch_plain_t<NRV, KV, 0, 0>(GALOP_ARGS);
+#define GALOP_CASE_EFF(NRV, KV)                                                        \
+    if (num_registers == NRV && t.cases_per_lane == KV && effective_only)              \
+        return ds.op_set ? launch_plain_t<NRV, KV, 1, 1>(GALOP_ARGS)                   \
+                         : launch_plain_t<NRV, KV, 0, 1>(GALOP_ARGS);
     GALOP_CASE(8, 1) GALOP_CASE(8, 2) GALOP_CASE(8, 4) GALOP_CASE(8, 8)
     GALOP_CASE(16, 1) GALOP_CASE(16, 2) GALOP_CASE(16, 4)
+    GALOP_CASE_EFF(8, 1) GALOP_CASE_EFF(8, 2) GALOP_CASE_EFF(8, 4)
+    GALOP_CASE_EFF(16, 1) GALOP_CASE_EFF(16, 2)
 #undef GALOP_CASE
+#undef GALOP_CASE_EFF
+#undef GALOP_ARGS
     return hipErrorInvalidValue;
 }
 

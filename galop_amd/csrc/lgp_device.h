@@ -282,6 +282,104 @@ __device__ __forceinline__ void run_program(const uint32_t* __restrict__ prog,
     }
 }
 
+// ---- effective words only (src/Introns.h:28-46) ---------------------------
+// A program of at most kSkipMaxWords words is held in kSkipChunks chunk
+// registers (word 64c + l in lane l of chunk c).  One backward pass over the
+// words on the scalar unit marks those whose result can reach the output
+// register (bit j of mask c for word 64c + j); the case tiles then run only the
+// marked words.  An unmarked word writes a register that is overwritten before
+// any marked word or the output reads it, so every case's output -- and the
+// fitness -- is bitwise that of the whole program.
+constexpr int kSkipChunks = 4;
+constexpr int64_t kSkipMaxWords = kSkipChunks * kWave;
+
+struct EffProgram {
+    uint32_t ch0, ch1, ch2, ch3;  // chunk registers
+    uint64_t m0, m1, m2, m3;      // effective-word masks (wave-uniform)
+    int nchunks;
+};
+
+// One step of the backward liveness pass: is word w effective given the set of
+// registers live after it, and the set live before it.
+template <int NR, int EXT>
+__device__ __forceinline__ bool live_step(uint32_t w, int out_reg, int iprot, uint32_t& live) {
+    constexpr uint32_t rmask = (uint32_t)NR - 1u;
+    const int d = iprot ? out_reg + (int)((w >> 8) & (rmask >> 1)) : (int)((w >> 8) & rmask);
+    if (!((live >> d) & 1u)) return false;
+    live &= ~(1u << d);
+    const int op = (int)(w & 7u);
+    const bool unary = (op == 4 || op == 5) || (EXT && (op == 6 || op == 7));
+    if (!unary) live |= 1u << ((w >> 16) & rmask);
+    if (!(w >> 31)) live |= 1u << ((w >> 24) & rmask);
+    return true;
+}
+
+// Mask of the effective words among the first `cnt` of a chunk, last to first.
+template <int NR, int EXT>
+__device__ __forceinline__ uint64_t mark_chunk(uint32_t ch, int cnt, int out_reg, int iprot,
+                                               uint32_t& live) {
+    uint64_t m = 0;
+    for (int j = cnt - 1; j >= 0 && live; --j)
+        if (live_step<NR, EXT>(wave_word(ch, j), out_reg, iprot, live)) m |= 1ull << j;
+    return m;
+}
+
+// Words [0, n) of `prog`, n <= kSkipMaxWords, into the chunk registers, every
+// word marked.
+__device__ __forceinline__ void load_window(const uint32_t* __restrict__ prog, int n, int lane,
+                                            EffProgram& e) {
+    e.nchunks = (n + kWave - 1) / kWave;
+    e.ch0 = lane < n ? prog[lane] : 0u;
+    e.ch1 = kWave + lane < n ? prog[kWave + lane] : 0u;
+    e.ch2 = 2 * kWave + lane < n ? prog[2 * kWave + lane] : 0u;
+    e.ch3 = 3 * kWave + lane < n ? prog[3 * kWave + lane] : 0u;
+    const auto all = [&](int c) {
+        const int k = min(kWave, max(0, n - c * kWave));
+        return k == kWave ? ~0ull : ((1ull << k) - 1ull);
+    };
+    e.m0 = all(0);
+    e.m1 = all(1);
+    e.m2 = all(2);
+    e.m3 = all(3);
+}
+
+template <int NR, int EXT>
+__device__ __forceinline__ void mark_effective(const uint32_t* __restrict__ prog, int64_t len,
+                                               int lane, int out_reg, int iprot, EffProgram& e) {
+    const int n = (int)len;  // <= kSkipMaxWords
+    load_window(prog, n, lane, e);
+    uint32_t live = 1u << out_reg;
+    const auto cnt = [&](int c) { return min(kWave, max(0, n - c * kWave)); };
+    e.m3 = mark_chunk<NR, EXT>(e.ch3, cnt(3), out_reg, iprot, live);
+    e.m2 = mark_chunk<NR, EXT>(e.ch2, cnt(2), out_reg, iprot, live);
+    e.m1 = mark_chunk<NR, EXT>(e.ch1, cnt(1), out_reg, iprot, live);
+    e.m0 = mark_chunk<NR, EXT>(e.ch0, cnt(0), out_reg, iprot, live);
+}
+
+// The marked words in program order; one instance of the interpreter step.
+// The chunk in use is chosen by selects on register values (the fields are
+// copied first, so no select of their addresses can put them in memory).
+template <int NR, int K, int EXT>
+__device__ __forceinline__ void run_effective(const EffProgram& e, RegFile<NR, K>& r,
+                                              ConstRegs cr, DestDecode dd) {
+    const uint32_t ch1 = e.ch1, ch2 = e.ch2, ch3 = e.ch3;
+    const uint64_t m1 = e.m1, m2 = e.m2, m3 = e.m3;
+    const int nchunks = e.nchunks;
+    int c = 0;
+    uint64_t m = e.m0;
+    uint32_t ch = e.ch0;
+    for (;;) {
+        while (!m) {
+            if (++c >= nchunks) return;
+            m = c == 1 ? m1 : (c == 2 ? m2 : m3);
+            ch = c == 1 ? ch1 : (c == 2 ? ch2 : ch3);
+        }
+        const int j = (int)__builtin_ctzll(m);
+        m &= m - 1;
+        exec_word<NR, K, EXT>(wave_word(ch, j), r, cr, dd);
+    }
+}
+
 // Load the K cases of the tile starting at case `first` (a multiple of
 // 64*K) for this lane into registers (src/Interpreter.cpp:43-52).
 // Addresses are a wave-uniform column pointer (SGPRs) plus the lane offset,
@@ -376,11 +474,10 @@ constexpr int64_t kCaseBlock = 16384;
 // Partial of the block of cases [c0, min(c0 + kCaseBlock, n_cases)); valid in
 // lane 0.  When case_out is non-null the per-case outputs are written there
 // instead of being accumulated.
-template <int NR, int K, int EXT>
-__device__ __forceinline__ double block_sse(const uint32_t* __restrict__ prog, int64_t len,
-                                            const DevData& ds, int lane, ConstRegs cr,
-                                            DestDecode dd, int64_t c0,
-                                            float* __restrict__ case_out) {
+// `run(r)` runs the program on one tile's registers.
+template <int NR, int K, typename Run>
+__device__ __forceinline__ double block_sse_run(const DevData& ds, int lane, int64_t c0,
+                                                float* __restrict__ case_out, Run run) {
     const int64_t tile_cases = (int64_t)kWave * K;
     const int64_t c1 = (c0 + kCaseBlock) < ds.n_cases ? (c0 + kCaseBlock) : ds.n_cases;
     const int out_reg = ds.out_reg;
@@ -388,7 +485,7 @@ __device__ __forceinline__ double block_sse(const uint32_t* __restrict__ prog, i
     for (int64_t first = c0; first < c1; first += tile_cases) {
         RegFile<NR, K> r;
         load_cases<NR, K>(ds, first, lane, r);
-        run_program<NR, K, EXT>(prog, len, lane, r, cr, dd);
+        run(r);
         const float* __restrict__ tgt = ds.targets + first;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
@@ -402,16 +499,58 @@ __device__ __forceinline__ double block_sse(const uint32_t* __restrict__ prog, i
     return wave_sum_f64(acc.total());
 }
 
+template <int NR, int K, int EXT>
+__device__ __forceinline__ double block_sse(const uint32_t* __restrict__ prog, int64_t len,
+                                            const DevData& ds, int lane, ConstRegs cr,
+                                            DestDecode dd, int64_t c0,
+                                            float* __restrict__ case_out) {
+    return block_sse_run<NR, K>(ds, lane, c0, case_out, [&](RegFile<NR, K>& r) {
+        run_program<NR, K, EXT>(prog, len, lane, r, cr, dd);
+    });
+}
+
+// The same block with only the effective words of `e` when the program has at
+// most kSkipMaxWords words; a longer one runs whole, window by window through
+// the same interpreter instance.
+// (A long program reloads `e` window by window, so e's registers are reused.)
+template <int NR, int K, int EXT>
+__device__ __forceinline__ double block_sse_eff(const uint32_t* __restrict__ prog, int64_t len,
+                                                EffProgram& e, const DevData& ds, int lane,
+                                                ConstRegs cr, DestDecode dd, int64_t c0,
+                                                float* __restrict__ case_out) {
+    return block_sse_run<NR, K>(ds, lane, c0, case_out, [&](RegFile<NR, K>& r) {
+        for (int64_t w0 = 0; w0 < len; w0 += kSkipMaxWords) {
+            if (len > kSkipMaxWords) {
+                const int64_t left = len - w0;
+                load_window(prog + w0, (int)(left < kSkipMaxWords ? left : kSkipMaxWords), lane,
+                            e);
+            }
+            run_effective<NR, K, EXT>(e, r, cr, dd);
+        }
+    });
+}
+
 // Evaluate one program on the whole dataset with one wave: returns the FP64
 // SSE in lane 0 (other lanes hold partial sums).  When case_out is non-null
 // the per-case outputs are written there instead of being accumulated.
-template <int NR, int K, int EXT>
+// With EFF, a program of at most kSkipMaxWords words runs its effective words
+// only (same bits); longer ones run whole.  Either way there is one instance
+// of the interpreter step in the kernel.
+template <int NR, int K, int EXT, int EFF = 0>
 __device__ __forceinline__ double eval_program_wave(const uint32_t* __restrict__ prog,
                                                     int64_t len, const DevData& ds,
                                                     int lane, ConstRegs cr,
                                                     float* __restrict__ case_out) {
     const DestDecode dd = dest_decode(NR, ds.input_protect);
     double sse = 0.0;
+    if (EFF) {
+        EffProgram e;
+        if (len <= kSkipMaxWords)
+            mark_effective<NR, EXT>(prog, len, lane, ds.out_reg, ds.input_protect, e);
+        for (int64_t c0 = 0; c0 < ds.n_cases; c0 += kCaseBlock)
+            sse += block_sse_eff<NR, K, EXT>(prog, len, e, ds, lane, cr, dd, c0, case_out);
+        return sse;
+    }
     for (int64_t c0 = 0; c0 < ds.n_cases; c0 += kCaseBlock)
         sse += block_sse<NR, K, EXT>(prog, len, ds, lane, cr, dd, c0, case_out);
     return sse;

@@ -215,6 +215,21 @@ def eval_csr_device(ds, d_words, d_offsets, n, d_fitness, d_sse=None,
                                         stream), ctx.handle)
 
 
+def eval_dense_device(ds, d_instr, d_lengths, stride, n, d_fitness, d_sse=None,
+                      layout=_lib.LAYOUT_PACKED, stream=None, first=0):
+    """Programs [first, first + n) of a device-resident dense population
+    (program i at d_instr[i*stride:], d_lengths[i] words).  LAYOUT_EFFECTIVE
+    runs only the words that reach the output register (same fitness bits).
+    d_fitness / d_sse are indexed like the population."""
+    ctx = ds.ctx
+    first = int(first)
+    check(ctx.lib.galop_eval_dense_device(
+        ctx.handle, ds.handle, d_instr.data_ptr() + 4 * first * int(stride),
+        d_lengths.data_ptr() + first, int(stride), int(n), d_fitness.data_ptr() + 4 * first,
+        (d_sse.data_ptr() + 8 * first) if d_sse is not None else None, int(layout), stream),
+        ctx.handle)
+
+
 def run_cases_device(ds, d_words, d_offsets, n, d_out, stream=None):
     ctx = ds.ctx
     check(ctx.lib.galop_run_cases_device(ctx.handle, ds.handle, _dptr(d_words), _dptr(d_offsets),

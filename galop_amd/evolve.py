@@ -25,8 +25,7 @@ import torch
 
 from . import _lib
 from .engine import VaryCfg, generation_stats_device, init_population_device, vary_device
-from .evaluator import DeviceDataset, Isa
-from ._lib import check
+from .evaluator import DeviceDataset, Isa, eval_dense_device
 
 
 def _encode_from_random(raw, num_registers):
@@ -66,10 +65,13 @@ class DeviceEngine:
     parallel_init    the GALOP_PARALLEL_INIT initializer (one PCG32 stream per
               program) instead of the serial mt19937 one
     fitness   _lib.FITNESS_MSE (Evaluator::evaluate_sr_mse) or FITNESS_R2
+    layout    _lib.LAYOUT_EFFECTIVE (default: only the words that reach the
+              output register run, found in the kernel) or LAYOUT_PACKED
+              (every word); the fitness is bitwise the same
     """
 
     def __init__(self, ctx, dataset, cfg, seed, max_generations, parallel_init=False,
-                 fitness=_lib.FITNESS_MSE, stream=None):
+                 fitness=_lib.FITNESS_MSE, layout=_lib.LAYOUT_EFFECTIVE, stream=None):
         if cfg.population <= cfg.elite_count or (cfg.population - cfg.elite_count) % 2:
             raise ValueError("population - elite_count must be positive and even")
         if not 1 <= cfg.stride <= 255:
@@ -81,6 +83,7 @@ class DeviceEngine:
         self.seed = int(seed)
         self.max_generations = int(max_generations)
         self.parallel_init = bool(parallel_init)
+        self.layout = int(layout)
         self.isa = Isa(cfg.num_registers, cfg.op_set, bool(cfg.input_protect), fitness=fitness)
         self.ds = DeviceDataset(ctx, self.isa, dataset)
         self._stream = stream if stream is not None else torch.cuda.Stream()
@@ -139,13 +142,9 @@ class DeviceEngine:
     def evaluate_range(self, start, count):
         if count <= 0:
             return
-        S = self.cfg.stride
         b = self.current_buffer
-        instr, lens, fit = self.instr[b], self.lengths[b], self.fitness[b]
-        check(self.ctx.lib.galop_eval_dense_device(
-            self.ctx.handle, self.ds.handle, instr.data_ptr() + 4 * start * S,
-            lens.data_ptr() + start, S, int(count), fit.data_ptr() + 4 * start, None,
-            self.stream), self.ctx.handle)
+        eval_dense_device(self.ds, self.instr[b], self.lengths[b], self.cfg.stride, count,
+                          self.fitness[b], layout=self.layout, stream=self.stream, first=start)
 
     def evaluate_all_sr(self):
         self.evaluate_range(0, self.cfg.population)

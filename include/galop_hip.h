@@ -50,8 +50,13 @@ extern "C" {
 #define GALOP_OPS_EXT8 1  /* {+,-,*,/p,sin,cos,exp,logp}: opcodes 6,7 replaced */
 
 /* ---- kernel variants ---- */
-#define GALOP_KERNEL_PLAIN   0 /* one program per wave64, one launch        */
-#define GALOP_KERNEL_BATCHED 1 /* length-bucketed persistent work queue     */
+#define GALOP_KERNEL_PLAIN     0 /* one program per wave64, one launch      */
+#define GALOP_KERNEL_BATCHED   1 /* length-bucketed persistent work queue   */
+#define GALOP_KERNEL_EFFECTIVE 2 /* the plain kernel running only the words
+                                    that reach the output register
+                                    (src/Introns.h:28-46) of programs of at
+                                    most 256 words; longer ones run whole.
+                                    Fitness bitwise that of PLAIN.          */
 
 /* ---- dense layouts (CudaEvaluator::Layout, src/EvaluatorCuda.h:24) ---- */
 #define GALOP_LAYOUT_PACKED    1 /* CSR of the raw programs                 */
@@ -136,12 +141,15 @@ int galop_eval_csr_device(galop_ctx* ctx, const galop_dataset* ds,
  * d_instructions[i*stride, i*stride + d_lengths[i]), stride 1..255 (a length
  * above the stride reads the stride's words).  Fitness, and the FP64 SSE when
  * d_sse is not NULL, of programs [0, n_programs); evaluate a window by
- * offsetting all three pointers.  Plain kernel (case-split for populations
- * too small to fill the GPU). */
+ * offsetting all three pointers.  layout GALOP_LAYOUT_PACKED runs every word;
+ * GALOP_LAYOUT_EFFECTIVE (the reference's PackedEffective,
+ * src/EvaluatorCuda.cu:375-422) runs only the effective words, found in the
+ * kernel, with bitwise the same fitness.  Plain kernel (case-split for
+ * populations too small to fill the GPU). */
 int galop_eval_dense_device(galop_ctx* ctx, const galop_dataset* ds,
                             const uint32_t* d_instructions, const uint8_t* d_lengths,
                             int32_t stride, int64_t n_programs, float* d_fitness,
-                            double* d_sse, void* stream);
+                            double* d_sse, int32_t layout, void* stream);
 /* Per-case outputs r[OUTPUT_REG] (src/Interpreter.cpp:55-57) of every program
  * on every case: d_out[p*n_cases + c]. */
 int galop_run_cases_device(galop_ctx* ctx, const galop_dataset* ds,

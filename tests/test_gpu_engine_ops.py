@@ -94,6 +94,50 @@ def test_device_stats_match_reference(gpu_ctx):
     assert np.array_equal(np.array(got, np.float64), f["final_stats"])
 
 
+def _stats_population(kind, P):
+    rng = np.random.default_rng(P)
+    lens = rng.integers(0, 256, P).astype(np.uint8)
+    fit = (rng.random(P) * 10.0 ** -rng.integers(0, 30, P)).astype(np.float32)
+    if kind == "order":
+        # each chunk opens with 1e30, -1e30: in index order they cancel before
+        # the rest is added; an order that adds any value to either of them
+        # first loses it, which the float mean shows
+        fit = rng.random(P).astype(np.float32)
+        for c in range(256):
+            lo = P * c // 256
+            fit[lo], fit[lo + 1] = np.float32(1e30), np.float32(-1e30)
+    else:
+        fit[rng.choice(P, size=min(P, 5), replace=False)] = np.float32(0.875)  # ties at the top
+        if kind == "nan":
+            fit[rng.choice(P, size=3, replace=False)] = np.nan
+            fit[rng.choice(P, size=3, replace=False)] = np.float32(-2.0)
+        if kind == "floor":
+            fit[:] = np.float32(-1.0)  # nothing above the -1 start: every chunk keeps its first index
+    return fit, lens
+
+
+@pytest.mark.parametrize("kind,P", [("mixed", 1), ("mixed", 37), ("mixed", 255), ("mixed", 256),
+                                    ("mixed", 65_601), ("nan", 1000), ("floor", 1000),
+                                    ("order", 1_000_003)])
+def test_device_stats_equal_host_build(gpu_ctx, kind, P):
+    """Chunked statistics against the host build (pinned to the reference's
+    compute_stats) on populations that exercise every rule: the in-order FP64
+    sum ("order": any other order changes the mean), the first maximum among
+    ties, empty chunks (P < 256), NaN (into the mean, never the best) and the
+    -1 floor."""
+    from galop_amd.engine import generation_stats_device
+    from gpu_util import stream
+    s = stream()
+    L = _host()
+    fit, lens = _stats_population(kind, P)
+    exp = np.zeros(6, np.float64)
+    L.gv_stats(fit.ctypes.data, lens.ctypes.data, P, 17, exp.ctypes.data)
+    st = generation_stats_device(gpu_ctx, _dev(fit), _dev(lens), P, 17, stream=s)
+    got = np.array([st["generation"], st["best_fitness"], st["mean_fitness"], st["best_length"],
+                    st["mean_length"], st["best_index"]], np.float64)
+    assert np.array_equal(got, exp, equal_nan=True), (got, exp)
+
+
 def test_large_population_two_generations_device_equals_host(gpu_ctx):
     from galop_amd._lib import galop_vary_cfg
     L = _host()
