@@ -329,6 +329,8 @@ def main():
     ap.add_argument("--no-casebound", action="store_true",
                     help="skip the case-bound sample (1,048,576 cases) with its FP64 oracle check")
     ap.add_argument("--casebound-programs", type=int, default=128)
+    ap.add_argument("--no-effective", action="store_true",
+                    help="skip the effective-word variant of the plain launch")
     ap.add_argument("--no-engine", action="store_true",
                     help="skip the device-resident evolutionary runs")
     ap.add_argument("--engine-population", type=int, default=1_000_000)
@@ -504,6 +506,33 @@ def main():
             "vs_device_math_oracle": {"max_rel_err": cbn["device_math_max_rel_err"],
                                       "within_1e-6": cbn["device_math_within_1e-6"]},
         }
+    if args.workload == "plain" and args.kernel == "plain" and not args.no_effective:
+        # The same launch with the effective-word variant: only the words that
+        # reach the output register run (found in the kernel); same bits.
+        from galop_amd import strip_introns_device
+        from galop_amd._lib import KERNEL_EFFECTIVE
+        fit_e = torch.empty(P, dtype=torch.float32, device="cuda")
+        sse_e = torch.empty(P, dtype=torch.float64, device="cuda")
+        ms_e = time_steps(lambda: eval_csr_device(wl.dds, wl.words_d, wl.offsets_d, P, fit_e,
+                                                  sse_e, kernel=KERNEL_EFFECTIVE,
+                                                  stream=stream), max(3, args.steps // 4), 1)
+        same = bool(torch.equal(fit_e.view(torch.int32), fitness.view(torch.int32)) and
+                    torch.equal(sse_e.view(torch.int64), sse.view(torch.int64)))
+        eff_w = torch.empty(wl.raw_words, dtype=torch.int32, device="cuda")
+        eff_o = torch.empty(P + 1, dtype=torch.int64, device="cuda")
+        eff_total = strip_introns_device(wl.dds, wl.words_d, wl.offsets_d, P, eff_w, eff_o,
+                                         stream=stream)
+        del eff_w, eff_o, fit_e, sse_e
+        line["plain_effective"] = {
+            "ms": round(ms_e, 3), "gpops_raw": round(ops / (ms_e / 1e3) / 1e9, 3),
+            "gpops_executed": round(float(eff_total) * wl.n_cases / (ms_e / 1e3) / 1e9, 3),
+            "effective_words": int(eff_total),
+            "effective_fraction": round(eff_total / wl.raw_words, 4),
+            "fitness_and_sse_bitwise_equal_plain": same,
+            "note": "GALOP_KERNEL_EFFECTIVE on the same launch: the kernel's backward liveness "
+                    "pass skips the words that cannot reach the output register; not `value`",
+        }
+        log(f"[bench] plain, effective words only: {line['plain_effective']}")
     if not args.no_host_entry and args.workload == "plain":
         # The same population through the synchronous host-pointer entry point
         # (galop_eval_csr): host memcpy into pinned staging, H2D, kernel, D2H.

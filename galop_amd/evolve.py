@@ -28,15 +28,12 @@ from .engine import VaryCfg, generation_stats_device, init_population_device, va
 from .evaluator import DeviceDataset, Isa, eval_dense_device
 
 
-def _encode_from_random(raw, num_registers):
-    """ISA::encode_from_random (src/ISA.h:106-138): keep the opcode bits, the
-    three register fields masked to the register count, and the constant
-    flag."""
-    rmask = np.uint32(num_registers - 1)
-    raw = raw.astype(np.uint32)
-    return ((raw & np.uint32(7)) | (raw & (rmask << np.uint32(8))) |
-            (raw & (rmask << np.uint32(16))) | (raw & (rmask << np.uint32(24))) |
-            (raw & np.uint32(0x80000000)))
+def encode_mask(num_registers):
+    """ISA::encode_from_random (src/ISA.h:106-138) keeps the opcode bits, the
+    three register fields masked to the register count and the constant flag:
+    the fields are disjoint, so the encoding is one AND with this mask."""
+    r = num_registers - 1
+    return np.uint32(7 | (r << 8) | (r << 16) | (r << 24) | 0x80000000)
 
 
 def host_generator(seed):
@@ -48,9 +45,12 @@ def host_generator(seed):
 
 def serial_init_words(bg, population, stride, num_registers):
     """The serial initializer (src/LGPEngine.cpp:70-75): every word of the
-    dense buffer, in order, from one generator draw."""
-    raw = bg.random_raw(population * stride).astype(np.uint32)
-    return _encode_from_random(raw, num_registers).reshape(population, stride)
+    dense buffer, in order, from one generator draw.  Generator.integers over
+    the full 32-bit range returns the generator's words unchanged, one draw
+    each, as uint32."""
+    raw = np.random.Generator(bg).integers(0, 2**32, size=population * stride, dtype=np.uint32)
+    raw &= encode_mask(num_registers)
+    return raw.reshape(population, stride)
 
 
 class DeviceEngine:
@@ -133,10 +133,8 @@ class DeviceEngine:
                                        self.fitness[b], stream=self.stream)
             else:
                 words = serial_init_words(self._rng, P, S, cfg.num_registers)
-                host = torch.from_numpy(words.reshape(-1).view(np.int32)).pin_memory()
-                self.instr[b].copy_(host, non_blocking=True)
+                self.instr[b].copy_(torch.from_numpy(words.reshape(-1).view(np.int32)))
                 self.lengths[b].fill_(cfg.starting_length)
-                self._stream.synchronize()  # the pinned staging buffer is released below
 
     # ---- src/LGPEngine.cpp:132-145 ---------------------------------------------
     def evaluate_range(self, start, count):

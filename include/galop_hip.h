@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define GALOP_ABI_VERSION 2
+#define GALOP_ABI_VERSION 3
 
 /* ---- status codes ---- */
 #define GALOP_OK            0

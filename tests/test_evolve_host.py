@@ -11,7 +11,7 @@ import numpy as np
 
 from conftest import load_fixture
 from galop_amd.engine import VaryCfg
-from galop_amd.evolve import host_generator, serial_init_words
+from galop_amd.evolve import encode_mask, host_generator, serial_init_words
 
 
 def test_serial_init_matches_reference_engine():
@@ -28,6 +28,24 @@ def test_serial_init_is_sensitive_to_the_seed():
     f = load_fixture("engine_seed42.npz")
     w = serial_init_words(host_generator(43), 4, int(f["stride"]), 8)
     assert not np.array_equal(w[0, :8], f["init_words"][:8])
+
+
+def test_serial_init_leaves_the_generator_where_the_reference_does():
+    # the draws after the initial population (the first generation's stream
+    # base, src/LGPEngine.cpp:435-436) follow P * S single draws
+    a, b = host_generator(9), host_generator(9)
+    serial_init_words(a, 37, 11, 16)
+    b.random_raw(37 * 11)
+    assert np.array_equal(a.random_raw(4), b.random_raw(4))
+
+
+def test_encoding_is_the_field_mask():
+    raw = np.random.default_rng(0).integers(0, 2**32, 10000, dtype=np.uint64).astype(np.uint32)
+    for nr in (8, 16):
+        r = np.uint32(nr - 1)
+        fields = ((raw & np.uint32(7)) | (raw & (r << np.uint32(8))) | (raw & (r << np.uint32(16)))
+                  | (raw & (r << np.uint32(24))) | (raw & np.uint32(0x80000000)))
+        assert np.array_equal(raw & encode_mask(nr), fields)
 
 
 def test_parallel_init_base_is_the_first_two_draws():
