@@ -81,14 +81,219 @@ __global__ void elite_copy_kernel(ge::VaryCfg c, const uint64_t* __restrict__ so
     }
 }
 
-__global__ void vary_pairs_kernel(ge::VaryCfg c, const uint32_t* __restrict__ cur_instr,
-                                  const uint8_t* __restrict__ cur_len,
-                                  const float* __restrict__ cur_fit, uint32_t* next_instr,
-                                  uint8_t* next_len, float* next_fit, uint64_t base,
-                                  int64_t n_pairs) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_pairs)
-        ge::vary_pair(i, c, cur_instr, cur_len, cur_fit, next_instr, next_len, next_fit, base);
+// ---- one pair per wave ---------------------------------------------------------
+// The wave-wide form of ge::vary_pair: every lane runs the same PCG32 stream,
+// so each random decision is wave-uniform and lives on the scalar unit, while
+// the lanes hold the programs -- word k of a program in lane k % 64 of register
+// k / 64 -- so parent loads, crossover, the shifts of insertion and deletion,
+// and the child stores are coalesced vector operations.  The draws, their
+// order and every result equal ge::vary_pair's (the host build of the same
+// operators checks it).  Only a child's first `len` words are defined
+// afterwards, as in the reference, where nothing reads past a program's
+// length.
+template <int NW>
+struct WaveProg {
+    uint32_t w[NW];
+
+    // word k (wave-uniform k) to every lane
+    __device__ __forceinline__ uint32_t get(int k) const {
+        const int r = k >> 6, j = k & 63;
+        uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)w[0], j);
+#pragma unroll
+        for (int q = 1; q < NW; ++q) {
+            const uint32_t y = (uint32_t)__builtin_amdgcn_readlane((int)w[q], j);
+            if (r == q) x = y;
+        }
+        return x;
+    }
+    __device__ __forceinline__ void set(int k, uint32_t v, int lane) {
+#pragma unroll
+        for (int q = 0; q < NW; ++q)
+            if (k == 64 * q + lane) w[q] = v;
+    }
+    __device__ __forceinline__ void load(const uint32_t* __restrict__ p, int n, int lane) {
+#pragma unroll
+        for (int q = 0; q < NW; ++q) w[q] = (64 * q + lane < n) ? p[64 * q + lane] : 0u;
+    }
+    __device__ __forceinline__ void store(uint32_t* __restrict__ p, int n, int lane) const {
+#pragma unroll
+        for (int q = 0; q < NW; ++q)
+            if (64 * q + lane < n) p[64 * q + lane] = w[q];
+    }
+    // words [pos, len) move up one place and v goes to pos
+    __device__ __forceinline__ void insert_at(int pos, uint32_t v, int lane) {
+        uint32_t up[NW];
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+            up[q] = (uint32_t)__shfl_up((int)w[q], 1, 64);
+            if (q > 0 && lane == 0) up[q] = (uint32_t)__builtin_amdgcn_readlane((int)w[q - 1], 63);
+        }
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+            const int k = 64 * q + lane;
+            w[q] = k > pos ? up[q] : (k == pos ? v : w[q]);
+        }
+    }
+    // words [pos + 1, len) move down one place
+    __device__ __forceinline__ void remove_at(int pos, int len, int lane) {
+        uint32_t dn[NW];
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+            dn[q] = (uint32_t)__shfl_down((int)w[q], 1, 64);
+            if (q + 1 < NW && lane == 63) dn[q] = (uint32_t)__builtin_amdgcn_readlane((int)w[q + 1], 0);
+        }
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+            const int k = 64 * q + lane;
+            if (k >= pos && k < len - 1) w[q] = dn[q];
+        }
+    }
+};
+
+// ge::effective_mask on a wave-held program: bit i % 64 of m[i / 64].
+template <int NW>
+__device__ __forceinline__ int wave_effective_mask(const WaveProg<NW>& p, int len,
+                                                   const ge::VaryCfg& c, uint64_t (&m)[NW]) {
+    const uint32_t rmask = (uint32_t)c.num_registers - 1u;
+    const int half = c.num_registers / 2;
+    uint32_t live = 1u << (c.input_protect ? half : 0);
+    int n = 0;
+#pragma unroll
+    for (int q = NW - 1; q >= 0; --q) {
+        uint64_t mq = 0;
+        const int top = min(64, len - 64 * q);
+        for (int j = top - 1; j >= 0; --j) {
+            const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)p.w[q], j);
+            const int d = c.input_protect ? half + (int)((w >> 8) & (uint32_t)(half - 1))
+                                          : (int)((w >> 8) & rmask);
+            if (!((live >> d) & 1u)) continue;
+            mq |= 1ull << j;
+            ++n;
+            live &= ~(1u << d);
+            const int op = (int)(w & 7u);
+            const bool unary = op == 4 || op == 5 || (c.op_set && (op == 6 || op == 7));
+            if (!unary) live |= 1u << ((w >> 16) & rmask);
+            if (!(w >> 31)) live |= 1u << ((w >> 24) & rmask);
+        }
+        m[q] = mq;
+    }
+    return n;
+}
+
+// ge::effective_at: index of the k-th set bit (k < the count)
+template <int NW>
+__device__ __forceinline__ int wave_effective_at(const uint64_t (&m)[NW], int k) {
+    int res = 0;
+    bool found = false;
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
+        const int pc = __popcll(m[q]);
+        if (!found && k < pc) {
+            uint64_t x = m[q];
+            for (int t = 0; t < k; ++t) x &= x - 1;
+            res = 64 * q + (int)__builtin_ctzll(x);
+            found = true;
+        } else if (!found) {
+            k -= pc;
+        }
+    }
+    return res;
+}
+
+// ge::mutate on a wave-held program
+template <int NW>
+__device__ __forceinline__ void wave_mutate(WaveProg<NW>& p, int& len, const ge::VaryCfg& c,
+                                            ge::Pcg32& g, int lane) {
+    if (len == 0) return;
+    const uint32_t rmask = (uint32_t)c.num_registers - 1u;
+    uint64_t effm[NW];
+#pragma unroll
+    for (int q = 0; q < NW; ++q) effm[q] = 0;
+    int neff = 0;
+    if (c.effmut) neff = wave_effective_mask<NW>(p, len, c, effm);
+    if (c.micro_per_instr > 0.0f) {
+        for (int i = 0; i < len; ++i)
+            if (ge::uniform01(g) < c.micro_per_instr)
+                p.set(i, ge::micro_mutate(p.get(i), g, rmask), lane);
+    } else if (ge::uniform01(g) < c.micro_rate) {
+        int idx;
+        if (c.effmut && neff > 0 && ge::uniform01(g) < c.effmut_rate)
+            idx = wave_effective_at<NW>(effm, ge::uniform_int(g, 0, neff - 1));
+        else
+            idx = ge::uniform_int(g, 0, len - 1);
+        p.set(idx, ge::micro_mutate(p.get(idx), g, rmask), lane);
+    }
+    for (int ins = 0; ins < c.macro_inserts; ++ins) {
+        if (ge::uniform01(g) < c.insert_rate) {
+            if (len < c.stride) {
+                if (c.macro_random_pos) {
+                    int pos;
+                    if (c.effmut && neff > 0 && ge::uniform01(g) < c.effmut_rate)
+                        pos = wave_effective_at<NW>(effm, ge::uniform_int(g, 0, neff - 1));
+                    else
+                        pos = ge::uniform_int(g, 0, len);
+                    p.insert_at(pos, ge::encode_from_random(g.next(), rmask), lane);
+                } else {
+                    p.set(len, ge::encode_from_random(g.next(), rmask), lane);
+                }
+                ++len;
+            }
+        }
+    }
+    if (ge::uniform01(g) < c.delete_rate) {
+        if (len > 1) {
+            if (c.macro_random_pos) p.remove_at(ge::uniform_int(g, 0, len - 1), len, lane);
+            --len;
+        }
+    }
+}
+
+template <int NW>
+__global__ void __launch_bounds__(256)
+vary_pairs_wave_kernel(ge::VaryCfg c, const uint32_t* __restrict__ cur_instr,
+                       const uint8_t* __restrict__ cur_len, const float* __restrict__ cur_fit,
+                       uint32_t* __restrict__ next_instr, uint8_t* __restrict__ next_len,
+                       float* __restrict__ next_fit, uint64_t base, int64_t n_pairs) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int64_t i = (int64_t)blockIdx.x * (blockDim.x >> 6) +
+                      __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    if (i >= n_pairs) return;  // wave-uniform
+    ge::Pcg32 g = ge::pcg32_make(base, (uint64_t)i);
+    const int pa = ge::tournament(cur_fit, c, g);
+    const int pb = ge::tournament(cur_fit, c, g);
+    const int64_t da = c.elite_count + 2 * i, db = da + 1;
+    const int la = cur_len[pa], lb = cur_len[pb];
+    WaveProg<NW> a, b;
+    a.load(cur_instr + (int64_t)pa * c.stride, la, lane);
+    b.load(cur_instr + (int64_t)pb * c.stride, lb, lane);
+    int len_a, len_b;
+    const int min_len = la < lb ? la : lb;
+    if (!(ge::uniform01(g) < c.crossover_rate) || min_len < 2) {
+        len_a = la;  // clone
+        len_b = lb;
+    } else {
+        // child A = a's prefix + b's suffix (length lb); child B the other way
+        const int cut = ge::uniform_int(g, 1, min_len - 1);
+#pragma unroll
+        for (int q = 0; q < NW; ++q) {
+            const bool pre = 64 * q + lane < cut;
+            const uint32_t x = a.w[q], y = b.w[q];
+            a.w[q] = pre ? x : y;
+            b.w[q] = pre ? y : x;
+        }
+        len_a = lb;
+        len_b = la;
+    }
+    wave_mutate<NW>(a, len_a, c, g, lane);
+    wave_mutate<NW>(b, len_b, c, g, lane);
+    a.store(next_instr + da * c.stride, len_a, lane);
+    b.store(next_instr + db * c.stride, len_b, lane);
+    if (lane == 0) {
+        next_len[da] = (uint8_t)len_a;
+        next_len[db] = (uint8_t)len_b;
+        next_fit[da] = ge::bits_to_float(0x7fc00000u);
+        next_fit[db] = ge::bits_to_float(0x7fc00000u);
+    }
 }
 
 __global__ void init_kernel(ge::VaryCfg c, uint64_t base, uint32_t* __restrict__ instr,
@@ -223,10 +428,21 @@ int galop_vary_device(galop_ctx* ctx, const galop_vary_cfg* cfg, const uint32_t*
                            d_next_len, d_next_fit);
     }
     const int64_t n_pairs = (P - cfg->elite_count) / 2;
-    if (n_pairs > 0)
-        hipLaunchKernelGGL(vary_pairs_kernel, dim3((unsigned)((n_pairs + 127) / 128)), dim3(128), 0,
-                           s, c, d_cur_instr, d_cur_len, d_cur_fit, d_next_instr, d_next_len,
-                           d_next_fit, stream_base, n_pairs);
+    if (n_pairs > 0) {
+        // one pair per wave, four waves per workgroup; registers per lane for
+        // the longest program: ceil(stride / 64)
+        const dim3 grid((unsigned)((n_pairs + 3) / 4)), block(256);
+#define GALOP_VARY(NW)                                                                        \
+        hipLaunchKernelGGL(vary_pairs_wave_kernel<NW>, grid, block, 0, s, c, d_cur_instr,       \
+                           d_cur_len, d_cur_fit, d_next_instr, d_next_len, d_next_fit,          \
+                           stream_base, n_pairs)
+        const int nw = (cfg->stride + 63) / 64;
+        if (nw == 1) GALOP_VARY(1);
+        else if (nw == 2) GALOP_VARY(2);
+        else if (nw == 3) GALOP_VARY(3);
+        else GALOP_VARY(4);
+#undef GALOP_VARY
+    }
     e = hipGetLastError();
     return e == hipSuccess ? GALOP_OK : hip_fail(ctx, e, "galop_vary_device: launch");
 }

@@ -138,16 +138,30 @@ def test_device_stats_equal_host_build(gpu_ctx, kind, P):
     assert np.array_equal(got, exp, equal_nan=True), (got, exp)
 
 
-def test_large_population_two_generations_device_equals_host(gpu_ctx):
+@pytest.mark.parametrize("P,S,E,variant", [
+    (200_000, 64, 10_000, "effmut"),      # one register per lane, effective mutation
+    (20_000, 65, 1_000, "effmut"),        # two registers, one word in the second
+    (20_000, 130, 2_000, "per_instr"),    # three registers, per-instruction micro mutation
+    (10_000, 255, 500, "tail"),           # four registers, tail-only insertion/deletion
+    (10_000, 200, 0, "effmut"),           # no elites
+])
+def test_large_population_two_generations_device_equals_host(gpu_ctx, P, S, E, variant):
+    """Two chained generations on the device equal the host build on every
+    length, word and fitness, for each register count per lane (stride up to
+    255) and each variation arm."""
     from galop_amd._lib import galop_vary_cfg
     L = _host()
-    rng = np.random.default_rng(5)
+    rng = np.random.default_rng(S)
     c = galop_vary_cfg()
-    c.population, c.stride, c.elite_count, c.tournament_size = 200_000, 64, 10_000, 4
+    c.population, c.stride, c.elite_count, c.tournament_size = P, S, E, 4
     c.num_registers, c.input_protect, c.op_set, c.effmut = 16, 1, 1, 1
     c.macro_inserts, c.macro_random_pos, c.starting_length, c.reserved = 2, 1, 8, 0
     (c.crossover_rate, c.micro_rate, c.micro_per_instr, c.effmut_rate, c.insert_rate,
      c.delete_rate) = (0.9, 0.8, 0.0, 0.7, 0.6, 0.5)
+    if variant == "per_instr":
+        c.effmut, c.micro_per_instr, c.num_registers, c.input_protect = 0, 0.05, 8, 0
+    if variant == "tail":
+        c.macro_random_pos, c.macro_inserts, c.insert_rate = 0, 3, 0.9
     n, S = c.population, c.stride
     lens = rng.integers(1, S + 1, n).astype(np.uint8)
     cur = rng.integers(0, 2**32, (n, S), dtype=np.uint64).astype(np.uint32)
