@@ -150,7 +150,10 @@ struct WaveProg {
     }
 };
 
-// ge::effective_mask on a wave-held program: bit i % 64 of m[i / 64].
+// ge::effective_mask on a wave-held program: bit i % 64 of m[i / 64].  Each
+// lane first computes its word's transfer (written register bit low, read
+// register bits high); the live set only changes at effective words, so one
+// ballot finds the next one below the current position.
 template <int NW>
 __device__ __forceinline__ int wave_effective_mask(const WaveProg<NW>& p, int len,
                                                    const ge::VaryCfg& c, uint64_t (&m)[NW]) {
@@ -160,20 +163,26 @@ __device__ __forceinline__ int wave_effective_mask(const WaveProg<NW>& p, int le
     int n = 0;
 #pragma unroll
     for (int q = NW - 1; q >= 0; --q) {
+        const uint32_t w = p.w[q];
+        const int d = c.input_protect ? half + (int)((w >> 8) & (uint32_t)(half - 1))
+                                      : (int)((w >> 8) & rmask);
+        const int op = (int)(w & 7u);
+        const bool unary = op == 4 || op == 5 || (c.op_set && (op == 6 || op == 7));
+        uint32_t s = unary ? 0u : 1u << ((w >> 16) & rmask);
+        if (!(w >> 31)) s |= 1u << ((w >> 24) & rmask);
+        const uint32_t bits = (1u << d) | (s << 16);
+        const int top = max(0, min(64, len - 64 * q));
+        uint64_t below = top >= 64 ? ~0ull : ((1ull << top) - 1ull);
         uint64_t mq = 0;
-        const int top = min(64, len - 64 * q);
-        for (int j = top - 1; j >= 0; --j) {
-            const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)p.w[q], j);
-            const int d = c.input_protect ? half + (int)((w >> 8) & (uint32_t)(half - 1))
-                                          : (int)((w >> 8) & rmask);
-            if (!((live >> d) & 1u)) continue;
-            mq |= 1ull << j;
+        for (;;) {
+            const uint64_t hits = __ballot((bits & live) != 0u) & below;
+            if (!hits) break;
+            const int i = 63 - __builtin_clzll(hits);
+            const uint32_t x = (uint32_t)__builtin_amdgcn_readlane((int)bits, i);
+            live = (live & ~x) | (x >> 16);
+            mq |= 1ull << i;
             ++n;
-            live &= ~(1u << d);
-            const int op = (int)(w & 7u);
-            const bool unary = op == 4 || op == 5 || (c.op_set && (op == 6 || op == 7));
-            if (!unary) live |= 1u << ((w >> 16) & rmask);
-            if (!(w >> 31)) live |= 1u << ((w >> 24) & rmask);
+            below = (1ull << i) - 1ull;
         }
         m[q] = mq;
     }

@@ -299,28 +299,41 @@ struct EffProgram {
     int nchunks;
 };
 
-// One step of the backward liveness pass: is word w effective given the set of
-// registers live after it, and the set live before it.
+// A word's liveness transfer, computed in every lane at once: the bit of the
+// register it writes in the low half, the bits of the registers it reads in the
+// high half (NR <= 16).
 template <int NR, int EXT>
-__device__ __forceinline__ bool live_step(uint32_t w, int out_reg, int iprot, uint32_t& live) {
+__device__ __forceinline__ uint32_t live_bits(uint32_t w, int out_reg, int iprot) {
     constexpr uint32_t rmask = (uint32_t)NR - 1u;
     const int d = iprot ? out_reg + (int)((w >> 8) & (rmask >> 1)) : (int)((w >> 8) & rmask);
-    if (!((live >> d) & 1u)) return false;
-    live &= ~(1u << d);
     const int op = (int)(w & 7u);
     const bool unary = (op == 4 || op == 5) || (EXT && (op == 6 || op == 7));
-    if (!unary) live |= 1u << ((w >> 16) & rmask);
-    if (!(w >> 31)) live |= 1u << ((w >> 24) & rmask);
-    return true;
+    uint32_t s = unary ? 0u : 1u << ((w >> 16) & rmask);
+    if (!(w >> 31)) s |= 1u << ((w >> 24) & rmask);
+    return (1u << d) | (s << 16);
 }
 
-// Mask of the effective words among the first `cnt` of a chunk, last to first.
+// Mask of the effective words among the first `cnt` of a chunk, last to first
+// (src/Introns.h:28-46): a word is effective when it writes a live register,
+// which it then kills before adding the registers it reads.  The live set only
+// changes at effective words, so the next one below the current position is
+// the highest lane there whose written register is live: one ballot finds it,
+// and the walk takes one step per effective word instead of one per word.
 template <int NR, int EXT>
 __device__ __forceinline__ uint64_t mark_chunk(uint32_t ch, int cnt, int out_reg, int iprot,
                                                uint32_t& live) {
+    const uint32_t bits = live_bits<NR, EXT>(ch, out_reg, iprot);
+    uint64_t below = cnt >= kWave ? ~0ull : ((1ull << cnt) - 1ull);  // words not yet passed
     uint64_t m = 0;
-    for (int j = cnt - 1; j >= 0 && live; --j)
-        if (live_step<NR, EXT>(wave_word(ch, j), out_reg, iprot, live)) m |= 1ull << j;
+    for (;;) {
+        const uint64_t hits = __ballot((bits & live) != 0u) & below;
+        if (!hits) break;
+        const int i = 63 - __builtin_clzll(hits);
+        const uint32_t x = wave_word(bits, i);
+        live = (live & ~x) | (x >> 16);
+        m |= 1ull << i;
+        below = (1ull << i) - 1ull;
+    }
     return m;
 }
 
