@@ -38,11 +38,11 @@ SIMDS_PER_CU = 4               # one wave64 VALU instruction per SIMD per 4 cloc
 # (1,048,576 KiB copied: FETCH_SIZE 524,307 KiB, WRITE_SIZE 1,048,576 KiB).
 # They describe that exact population only.
 PLAIN_WORKLOAD_COUNTERS = {
-    "fetch_bytes": 322283.2 * 1024 * (1048576 / 524307.69),
-    "write_bytes": 42368.9 * 1024 * 1.0,
-    "valu_insts": 28960481956,
-    "salu_insts": 10055056156,
-    "valu_busy_pct": 82.8,
+    "fetch_bytes": 321516.3 * 1024 * (1048576 / 524307.7),
+    "write_bytes": 30543.4 * 1024 * 1.0,
+    "valu_insts": 28191080292,
+    "salu_insts": 10101607500,
+    "valu_busy_pct": 81.2,
 }
 
 

@@ -14,8 +14,14 @@
 // from exact integer arithmetic and checked by tests/test_math_constants.py):
 //   exp  x = k ln2 + r, |r| <= ln2/2, ln2 split in three (Cody-Waite);
 //        exp(r) by its Taylor series to r^12; scale by 2^k.
-//   log  x = m 2^e with m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s),
-//        s = (m-1)/(m+1), series to s^17; plus e ln2.
+//   log  x = y 2^e with y in [sqrt(1/2), sqrt(2)); j = rint((y-1) 128) picks
+//        c_j = 128/(128+j) rounded to 24 bits, so r = y c_j - 1 is exact
+//        (|r| < 0.0056); log x = e ln2 + ln(1/c_j) + log1p(r), with ln(1/c_j)
+//        from a 91-entry table and log1p(r) by its series to r^8.  No
+//        division.  (The same table method for exp, 2^(j/64) with a
+//        degree-6 series, measured slower in the interpreter than the
+//        table-free Taylor form: its per-lane table load stalls the wave
+//        behind the prefetch of the next program chunk.)
 //   sin, cos
 //        x = k pi/32 + t, |t| <= pi/64: |x| < 2^25 by Cody-Waite with pi/32
 //        split in three; |x| >= 2^25 by Payne-Hanek -- the 24-bit mantissa
@@ -65,11 +71,8 @@ constexpr double EXP_C[13] = {
 constexpr double SINT_C[3] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13};
 // (-1)^k / (2k)!, k = 1..4
 constexpr double COST_C[4] = {-0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-16};
-// 2/(2k+1), k = 0..8  (log m = s * sum_k 2/(2k+1) s^(2k))
-constexpr double LOG_C[9] = {
-    0x1.0000000000000p+1, 0x1.5555555555555p-1, 0x1.999999999999ap-2, 0x1.2492492492492p-2,
-    0x1.c71c71c71c71cp-3, 0x1.745d1745d1746p-3, 0x1.3b13b13b13b14p-3, 0x1.1111111111111p-3,
-    0x1.e1e1e1e1e1e1ep-4};
+// (-1)^(n+1)/n, n = 2..8  (log1p r = r + r^2 sum_n (-1)^(n+1)/n r^(n-2))
+constexpr double LOG1P_C[7] = {-0x1.0000000000000p-1, 0x1.5555555555555p-2, -0x1.0000000000000p-2, 0x1.999999999999ap-3, -0x1.5555555555555p-3, 0x1.2492492492492p-3, -0x1.0000000000000p-3};
 // bits 2^-1 .. 2^-256 of 2/pi (word 0 first)
 GM_CONST uint32_t TWO_OVER_PI_W[8] = {0xA2F9836Eu, 0x4E441529u, 0xFC2757D1u, 0xF534DDC0u,
                                       0xDB629599u, 0x3C439041u, 0xFE5163ABu, 0xDEBBC561u};
@@ -141,6 +144,101 @@ GM_CONST double SINCOS_T[128] = {
     -0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1,
 };
 
+// (c_j, ln(1/c_j)), j = -37..53 at index j + 37, interleaved
+GM_CONST double LOG_T[182] = {
+    0x1.6816820000000p+0, -0x1.5d5bdfa595f2ap-2,
+    0x1.642c860000000p+0, -0x1.522ae1b38a3d5p-2,
+    0x1.6058160000000p+0, -0x1.4718dc171c41bp-2,
+    0x1.5c98820000000p+0, -0x1.3c2525533317bp-2,
+    0x1.58ed240000000p+0, -0x1.314f20fd35cd3p-2,
+    0x1.5555560000000p+0, -0x1.269623134db8ap-2,
+    0x1.51d07e0000000p+0, -0x1.1bf99425a6b8cp-2,
+    0x1.4e5e0a0000000p+0, -0x1.1178e6c27e478p-2,
+    0x1.4afd6a0000000p+0, -0x1.071385f4d5862p-2,
+    0x1.47ae140000000p+0, -0x1.f991c3cb3b370p-3,
+    0x1.446f860000000p+0, -0x1.e530edde7100ep-3,
+    0x1.4141420000000p+0, -0x1.d10383e655e65p-3,
+    0x1.3e22cc0000000p+0, -0x1.bd0874c3bd8abp-3,
+    0x1.3b13b20000000p+0, -0x1.a93ed8c8ad9cap-3,
+    0x1.3813820000000p+0, -0x1.95a5b2ef70165p-3,
+    0x1.3521d00000000p+0, -0x1.823c18551a3bep-3,
+    0x1.323e340000000p+0, -0x1.6f01247756aaap-3,
+    0x1.2f684c0000000p+0, -0x1.5bf407b543db1p-3,
+    0x1.2c9fb40000000p+0, -0x1.4913d2733b540p-3,
+    0x1.29e4120000000p+0, -0x1.365fc6c159004p-3,
+    0x1.27350c0000000p+0, -0x1.23d715e49c1f7p-3,
+    0x1.24924a0000000p+0, -0x1.1178ee227e458p-3,
+    0x1.21fb780000000p+0, -0x1.fe89129dbd565p-4,
+    0x1.1f70480000000p+0, -0x1.da727838446a0p-4,
+    0x1.1cf06a0000000p+0, -0x1.b6ac7c9ad5ad1p-4,
+    0x1.1a7b960000000p+0, -0x1.9335e4d594988p-4,
+    0x1.1811820000000p+0, -0x1.700d3deeac089p-4,
+    0x1.15b1e60000000p+0, -0x1.4d31165207eacp-4,
+    0x1.135c820000000p+0, -0x1.2aa0580471746p-4,
+    0x1.1111120000000p+0, -0x1.08599959e39a5p-4,
+    0x1.0ecf560000000p+0, -0x1.ccb7265ddb24dp-5,
+    0x1.0c97140000000p+0, -0x1.894a8349fb262p-5,
+    0x1.0a68100000000p+0, -0x1.466ad942de386p-5,
+    0x1.0842100000000p+0, -0x1.0415c89e74404p-5,
+    0x1.0624de0000000p+0, -0x1.8492858c8c979p-6,
+    0x1.0410420000000p+0, -0x1.0205a38935667p-6,
+    0x1.0204080000000p+0, -0x1.01014f588de6dp-7,
+    0x1.0000000000000p+0, 0x0.0p+0,
+    0x1.fc07f00000000p-1, 0x1.fe02b6b106791p-8,
+    0x1.f81f820000000p-1, 0x1.fc0a890fc03e4p-7,
+    0x1.f4465a0000000p-1, 0x1.7b91acfd5b11cp-6,
+    0x1.f07c200000000p-1, 0x1.f82990e783380p-6,
+    0x1.ecc07c0000000p-1, 0x1.39e86e1febd8dp-5,
+    0x1.e9131a0000000p-1, 0x1.77459be32dd23p-5,
+    0x1.e573ac0000000p-1, 0x1.b42de091971d5p-5,
+    0x1.e1e1e20000000p-1, 0x1.f0a30a01162a7p-5,
+    0x1.de5d6e0000000p-1, 0x1.1653710a37ae3p-4,
+    0x1.dae6080000000p-1, 0x1.341d7461bd1ddp-4,
+    0x1.d77b660000000p-1, 0x1.51b06dd061852p-4,
+    0x1.d41d420000000p-1, 0x1.6f0d272e56b4dp-4,
+    0x1.d0cb580000000p-1, 0x1.8c3465e319b45p-4,
+    0x1.cd85680000000p-1, 0x1.a926d8a4ad570p-4,
+    0x1.ca4b300000000p-1, 0x1.c5e54bf5bc748p-4,
+    0x1.c71c720000000p-1, 0x1.e27074e2af2e8p-4,
+    0x1.c3f8f00000000p-1, 0x1.fec9141dbeabbp-4,
+    0x1.c0e0700000000p-1, 0x1.0d77e8cd08e5ap-3,
+    0x1.bdd2b80000000p-1, 0x1.1b72b012f67a8p-3,
+    0x1.bacf920000000p-1, 0x1.29552c41ff52ep-3,
+    0x1.b7d6c40000000p-1, 0x1.371fc161e8f75p-3,
+    0x1.b4e81c0000000p-1, 0x1.44d2b38cb7d29p-3,
+    0x1.b203640000000p-1, 0x1.526e5e5a1b438p-3,
+    0x1.af286c0000000p-1, 0x1.5ff3060a793d5p-3,
+    0x1.ac57020000000p-1, 0x1.6d60fce19d21fp-3,
+    0x1.a98ef60000000p-1, 0x1.7ab890410d909p-3,
+    0x1.a6d01a0000000p-1, 0x1.87fa08620c915p-3,
+    0x1.a41a420000000p-1, 0x1.9525a80f456b8p-3,
+    0x1.a16d400000000p-1, 0x1.a23bbffe2b567p-3,
+    0x1.9ec8ea0000000p-1, 0x1.af3c91880bffep-3,
+    0x1.9c2d140000000p-1, 0x1.bc286be2d8cecp-3,
+    0x1.99999a0000000p-1, 0x1.c8ff7a79a9a26p-3,
+    0x1.970e500000000p-1, 0x1.d5c21434fbb98p-3,
+    0x1.948b100000000p-1, 0x1.e27075e2af2e7p-3,
+    0x1.920fb40000000p-1, 0x1.ef0adfddc5940p-3,
+    0x1.8f9c180000000p-1, 0x1.fb918bd5e3e44p-3,
+    0x1.8d30180000000p-1, 0x1.04025b6b4d04ap-2,
+    0x1.8acb900000000p-1, 0x1.0a3250a7390f0p-2,
+    0x1.886e600000000p-1, 0x1.1058bd1ae4ae2p-2,
+    0x1.8618620000000p-1, 0x1.1675c97aba611p-2,
+    0x1.83c9780000000p-1, 0x1.1c898b36999fdp-2,
+    0x1.8181820000000p-1, 0x1.22941e6cf7969p-2,
+    0x1.7f40600000000p-1, 0x1.2895a0bde86a4p-2,
+    0x1.7d05f40000000p-1, 0x1.2e8e2bee11d31p-2,
+    0x1.7ad2200000000p-1, 0x1.347ddb2987d59p-2,
+    0x1.78a4c80000000p-1, 0x1.3a64c596945eap-2,
+    0x1.767dce0000000p-1, 0x1.404309206a7e5p-2,
+    0x1.745d180000000p-1, 0x1.4618ba21c5ecap-2,
+    0x1.7242880000000p-1, 0x1.4be5f937778a1p-2,
+    0x1.702e060000000p-1, 0x1.51aad7c2df82ep-2,
+    0x1.6e1f760000000p-1, 0x1.5767736c55a74p-2,
+    0x1.6c16c20000000p-1, 0x1.5d1bda55809d0p-2,
+    0x1.6a13ce0000000p-1, 0x1.62c82c939c7a3p-2,
+};
+
 // Every FP64 constant above in one table; the functions below take it as an
 // argument.  On the device the table sits in constant memory and is reached
 // through coef_table(), a pointer the compiler cannot see through, so each
@@ -150,22 +248,22 @@ GM_CONST double SINCOS_T[128] = {
 // gfx950 takes no 64-bit literal), about 60 VGPRs that cost occupancy.
 enum CoefSlot : int {
     kExpC = 0,        // EXP_C[0..12]
-    kLogC = 13,       // LOG_C[0..8]
-    kSinT = 22,       // SINT_C[0..2]
-    kCosT = 25,       // COST_C[0..3]
-    kPio32 = 29,      // PIO32_0, PIO32_1, PIO32_2
-    kInvPio32 = 32,
-    kPio32Scaled64 = 33,
-    kLn2 = 34,        // LN2_0, LN2_1, LN2_2
-    kInvLn2 = 37,
-    kSqrtHalf = 38,
-    kCoefCount = 39
+    kLog1pC = 13,     // LOG1P_C[0..6]
+    kSinT = 20,       // SINT_C[0..2]
+    kCosT = 23,       // COST_C[0..3]
+    kPio32 = 27,      // PIO32_0, PIO32_1, PIO32_2
+    kInvPio32 = 30,
+    kPio32Scaled64 = 31,
+    kLn2 = 32,        // LN2_0, LN2_1, LN2_2
+    kInvLn2 = 35,
+    kSqrtHalf = 36,
+    kCoefCount = 37
 };
 #define GM_COEF_INIT                                                              \
     {EXP_C[0], EXP_C[1], EXP_C[2], EXP_C[3], EXP_C[4], EXP_C[5], EXP_C[6],        \
      EXP_C[7], EXP_C[8], EXP_C[9], EXP_C[10], EXP_C[11], EXP_C[12],               \
-     LOG_C[0], LOG_C[1], LOG_C[2], LOG_C[3], LOG_C[4], LOG_C[5], LOG_C[6], LOG_C[7], \
-     LOG_C[8],                                                                    \
+     LOG1P_C[0], LOG1P_C[1], LOG1P_C[2], LOG1P_C[3], LOG1P_C[4], LOG1P_C[5],      \
+     LOG1P_C[6],                                                                  \
      SINT_C[0], SINT_C[1], SINT_C[2], COST_C[0], COST_C[1], COST_C[2], COST_C[3],  \
      PIO32_0, PIO32_1, PIO32_2, INV_PIO32, PIO32_SCALED64,                        \
      LN2_0, LN2_1, LN2_2, INV_LN2, SQRT_HALF}
@@ -226,17 +324,18 @@ GM_HD float expf_r(float x, CoefPtr C) {
 // ---- log of a positive, finite, non-zero float --------------------------
 GM_HD float logf_pos(float x, CoefPtr C) {
     int e;
-    double m = frexp((double)x, &e);  // m in [0.5, 1)
-    const bool low = m < C[kSqrtHalf];
-    m = low ? 2.0 * m : m;
+    double y = frexp((double)x, &e);  // y in [0.5, 1), 24 significant bits
+    const bool low = y < C[kSqrtHalf];
+    y = low ? 2.0 * y : y;
     e = low ? e - 1 : e;
-    const double s = (m - 1.0) / (m + 1.0);
-    const double s2 = s * s;
-    double p = C[kLogC + 8];
-    for (int k = 7; k >= 0; --k) p = fma_d(p, s2, C[kLogC + k]);
-    const double lm = p * s;
+    const int j = (int)rint_d((y - 1.0) * 128.0) + 37;  // 0..90
+    const double r = fma_d(y, LOG_T[2 * j], -1.0);     // exact: c_j has 24 bits
+    double p = C[kLog1pC + 6];
+    for (int n = 5; n >= 0; --n) p = fma_d(p, r, C[kLog1pC + n]);
+    const double q = fma_d(r * r, p, r);  // log1p(r)
     const double ed = (double)e;
-    return (float)fma_d(ed, C[kLn2], fma_d(ed, C[kLn2 + 1], lm));
+    // e ln2 + ln(1/c_j) + log1p(r); e * LN2_0 is exact
+    return (float)(fma_d(ed, C[kLn2], LOG_T[2 * j + 1]) + fma_d(ed, C[kLn2 + 1], q));
 }
 
 // protected log of the interpreter's EXT8 table: b != 0 ? log|b| : 0

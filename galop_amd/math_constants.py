@@ -90,8 +90,6 @@ def taylor(kind, n_terms):
         return [float(Fraction((-1) ** k, factorial(2 * k + 1))) for k in range(n_terms)]
     if kind == "cos":
         return [float(Fraction((-1) ** k, factorial(2 * k))) for k in range(n_terms)]
-    if kind == "atanh":  # log m = 2 atanh(s): 2 * s^(2k+1)/(2k+1)
-        return [float(Fraction(2, 2 * k + 1)) for k in range(n_terms)]
     raise ValueError(kind)
 
 
@@ -137,6 +135,54 @@ def sincos_table():
     return out
 
 
+def round_bits(v, bits):
+    """The rational v rounded to `bits` significant bits (nearest, ties even)."""
+    e = v.numerator.bit_length() - v.denominator.bit_length()
+    if Fraction(2) ** e > v:
+        e -= 1
+    scale = Fraction(2) ** (bits - 1 - e)
+    return Fraction(round(v * scale)) / scale
+
+
+def ln_fixed(y, prec):
+    """ln(y) * 2^prec for a rational y > 0 near 1: 2 atanh(s), s = (y-1)/(y+1),
+    series in fixed point."""
+    s = (y - 1) / (y + 1)
+    neg = s < 0
+    s = abs(s)
+    guard = 64
+    p = prec + guard
+    sf = (s.numerator << p) // s.denominator
+    s2 = sf * sf >> p
+    total = 0
+    term = sf
+    k = 0
+    while term:
+        total += term // (2 * k + 1)
+        term = term * s2 >> p
+        k += 1
+    total = (2 * total) >> guard
+    return -total if neg else total
+
+
+# log x = e ln2 + ln(1/c_j) + log1p(y c_j - 1) for y in [sqrt(1/2), sqrt(2)),
+# j = rint((y - 1) * 128) in [LOG_J_LO, LOG_J_HI]; c_j is 128 / (128 + j)
+# rounded to 24 bits, so y c_j is exact in double for a 24-bit y.
+LOG_J_LO, LOG_J_HI = -37, 53
+
+
+def log_table():
+    """Pairs (c_j, -ln c_j) for j = LOG_J_LO..LOG_J_HI, rounded to double (c_j
+    exactly: it has 24 bits)."""
+    out = []
+    for j in range(LOG_J_LO, LOG_J_HI + 1):
+        c = round_bits(Fraction(128, 128 + j), 24)
+        assert Fraction(float(c)) == c
+        lj = -ln_fixed(c, PREC) if j else 0
+        out.extend([float(c), float(Fraction(lj, 1 << PREC)) + 0.0])
+    return out
+
+
 def constants():
     pi = pi_fixed()
     ln2 = ln2_fixed()
@@ -150,8 +196,10 @@ def constants():
         "EXP_POLY": taylor("exp", 13),
         "SINT_POLY": taylor("sin", 4)[1:],   # -1/3!, 1/5!, -1/7!
         "COST_POLY": taylor("cos", 5)[1:],   # -1/2!, 1/4!, -1/6!, 1/8!
-        "LOG_POLY": taylor("atanh", 9),
         "SINCOS_T": sincos_table(),
+        # log1p(r) = r + r^2 (-1/2 + r/3 - r^2/4 + ... - r^6/8)
+        "LOG1P_POLY": [float(Fraction((-1) ** (n + 1), n)) for n in range(2, 9)],
+        "LOG_T": log_table(),
     }
     return c
 

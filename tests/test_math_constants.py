@@ -35,8 +35,9 @@ def _header_values():
         vals[name] = nums if len(nums) > 1 else nums[0]
     words = re.search(r"TWO_OVER_PI_W\[8\] = \{([^}]*)\}", text).group(1)
     vals["TWO_OVER_PI_W"] = [int(w.strip().rstrip("u"), 16) for w in words.split(",")]
-    table = re.search(r"SINCOS_T\[128\] = \{([^}]*)\}", text).group(1)
-    vals["SINCOS_T"] = [float.fromhex(t.strip()) for t in table.split(",") if t.strip()]
+    for name, n in (("SINCOS_T", 128), ("LOG_T", 182)):
+        table = re.search(name + r"\[%d\] = \{([^}]*)\}" % n, text).group(1)
+        vals[name] = [float.fromhex(t.strip()) for t in table.split(",") if t.strip()]
     return vals
 
 
@@ -51,9 +52,10 @@ def test_constants_match_derivation():
     assert h["EXP_C"] == c["EXP_POLY"]
     assert h["SINT_C"] == c["SINT_POLY"]
     assert h["COST_C"] == c["COST_POLY"]
-    assert h["LOG_C"] == c["LOG_POLY"]
+    assert h["LOG1P_C"] == c["LOG1P_POLY"]
     assert h["TWO_OVER_PI_W"] == c["TWO_OVER_PI_WORDS"][:8]
     assert h["SINCOS_T"] == c["SINCOS_T"]
+    assert h["LOG_T"] == c["LOG_T"]
     # the splits are exact decompositions with short leading parts
     from fractions import Fraction
     for parts, bits in ((c["PIO32"], 24), (c["LN2"], 32)):
@@ -89,6 +91,53 @@ def test_sincos_table_identities():
     for j in range(9):
         assert abs(t[2 * j] - np.sin(j * np.pi / 32)) <= 2.3e-16
         assert abs(t[2 * j + 1] - np.cos(j * np.pi / 32)) <= 2.3e-16
+
+
+def test_log_table_identities():
+    """c_j has 24 bits and is 128/(128+j) to 2^-24; ln(1/c_j) agrees with the
+    host C library's log within 1 ulp; every float y in [sqrt(1/2), sqrt(2))
+    gets an index in range and |y c_j - 1| < 0.0056, with y c_j exact."""
+    import math
+    from fractions import Fraction
+    t = MC.constants()["LOG_T"]
+    lo, hi = MC.LOG_J_LO, MC.LOG_J_HI
+    c = np.array(t[0::2])
+    L = np.array(t[1::2])
+    assert c[-lo] == 1.0 and L[-lo] == 0.0
+    for i, j in enumerate(range(lo, hi + 1)):
+        f = Fraction(t[2 * i])
+        m = f
+        while m.denominator != 1:
+            m *= 2
+        assert m.numerator.bit_length() <= 24
+        assert abs(f * (128 + j) / 128 - 1) <= Fraction(1, 1 << 24)
+        ref = -math.log(t[2 * i])
+        assert abs(L[i] - ref) <= np.spacing(abs(ref)) + 1e-300, j
+    # every float y in [sqrt(1/2), sqrt(2)), as the device computes the index
+    lo_bits = np.float32(np.sqrt(0.5)).view(np.uint32)
+    hi_bits = np.float32(np.sqrt(2.0)).view(np.uint32)
+    y = np.arange(lo_bits - 2, hi_bits + 2, dtype=np.uint32).view(np.float32).astype(np.float64)
+    y = y[(y >= np.sqrt(0.5)) & (y < np.sqrt(2.0))]
+    j = np.rint((y - 1.0) * 128.0).astype(np.int64)
+    assert j.min() == lo and j.max() == hi
+    cj = c[j - lo]
+    r = y * cj - 1.0
+    assert np.abs(r).max() < 0.0056
+    # exact: the double product equals the product of the two 24-bit values
+    # (checked through the high/low split of y * cj)
+    prod_hi = y * cj
+    prod_lo = np.array([float(Fraction(a) * Fraction(b) - Fraction(p))
+                        for a, b, p in zip(y[::4099], cj[::4099], prod_hi[::4099])])
+    assert (prod_lo == 0).all()
+
+
+def test_planted_wrong_log_entry_is_caught():
+    import math
+    t = list(MC.constants()["LOG_T"])
+    i = 21
+    t[2 * i + 1] *= 1 + 2.0 ** -40
+    ref = -math.log(t[2 * i])
+    assert abs(t[2 * i + 1] - ref) > np.spacing(abs(ref))
 
 
 def test_planted_wrong_table_entry_is_caught():
