@@ -96,6 +96,22 @@ def cpu_baseline(wl, gpu_fit, target_s, threads):
     }
 
 
+def host_cpu_info():
+    """CPU model, online CPUs, CPUs this process may use, OMP_NUM_THREADS."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0)),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
 def time_steps(fn, steps, warmup):
     """Average milliseconds of fn() over `steps` calls, HIP events on the
     current stream (the one every launch here uses)."""
@@ -496,6 +512,7 @@ def main():
                       f"({cbn['seconds']:.1f}s, {threads} threads); 1 thread: "
                       f"{cb1['gpops']:.4f} GPop/s on {cb1['programs']} programs",
             "single_thread_value": round(cb1["gpops"], 4),
+            "host": host_cpu_info(),
         }
         line["parity_sample"] = {
             "programs": cbn["programs"],

@@ -26,7 +26,7 @@ struct galop_batch {
     // intron removal inside the generation
     bool strip = false;
     int64_t max_words = 0;
-    DeviceBuf<int64_t> counts, scan_tmp, eff_offsets;
+    DeviceBuf<int64_t> counts;  // effective length per order position
     DeviceBuf<uint32_t> eff_words;
     // graph cache
     hipGraph_t graph = nullptr;
@@ -84,9 +84,7 @@ int galop_batch_create(galop_ctx* ctx, const galop_dataset* ds, int64_t max_prog
     b->max_words = max_words;
     int rc = ensure_batch_scratch(ctx, b->bufs, max_programs);
     if (rc == GALOP_OK && b->strip) {
-        hipError_t ae = b->counts.alloc((size_t)max_programs + 1);
-        if (ae == hipSuccess) ae = b->scan_tmp.alloc((size_t)scan_temp_elems(max_programs));
-        if (ae == hipSuccess) ae = b->eff_offsets.alloc((size_t)max_programs + 1);
+        hipError_t ae = b->counts.alloc((size_t)max_programs);
         if (ae == hipSuccess) ae = b->eff_words.alloc((size_t)max_words);
         if (ae != hipSuccess) rc = hip_fail(ctx, ae, "galop_batch_create: intron buffers");
     }
@@ -105,14 +103,14 @@ void galop_batch_destroy(galop_batch* b) {
     b->drop_graph();
     b->bufs.release();
     b->counts.release();
-    b->scan_tmp.release();
-    b->eff_offsets.release();
     b->eff_words.release();
     delete b;
 }
 
 // One generation on stream s: the length order, optional intron removal in
-// that order, then the persistent evaluation.  Pure stream work (kernels
+// that order (one pass: each program's effective code goes to the end of its
+// own slot of eff_words, which is laid out like the raw CSR), then the
+// persistent evaluation.  Pure stream work (kernels
 // only), so it can be captured.
 static hipError_t batch_generation(galop_batch* b, const uint32_t* d_words, const int64_t* d_offsets,
                                    int64_t n, float* d_fitness, double* d_sse, hipStream_t s) {
@@ -123,17 +121,13 @@ static hipError_t batch_generation(galop_batch* b, const uint32_t* d_words, cons
     if (e != hipSuccess) return e;
     if (!b->strip)
         return launch_batched_eval(dd, isa.num_registers, b->tuning, b->ctx->n_cu, sc, d_words,
-                                   d_offsets, 0, n, d_fitness, d_sse, s);
-    e = launch_effective_counts(isa.num_registers, isa.op_set, isa.input_protect, d_words, d_offsets,
-                                n, b->counts.ptr, s, sc.order);
-    if (e == hipSuccess) e = launch_exclusive_scan(b->counts.ptr, n, b->eff_offsets.ptr, b->scan_tmp.ptr, s);
-    if (e == hipSuccess)
-        e = launch_effective_scatter(isa.num_registers, isa.op_set, isa.input_protect, d_words,
-                                     d_offsets, n, b->eff_offsets.ptr, b->eff_words.ptr,
-                                     (int64_t)b->eff_words.cap, s, sc.order);
+                                   d_offsets, nullptr, 0, n, d_fitness, d_sse, s);
+    const int64_t cap = (int64_t)b->eff_words.cap;
+    e = launch_effective_in_slot(isa.num_registers, isa.op_set, isa.input_protect, d_words,
+                                 d_offsets, n, sc.order, b->eff_words.ptr, cap, b->counts.ptr, s);
     if (e != hipSuccess) return e;
     return launch_batched_eval(dd, isa.num_registers, b->tuning, b->ctx->n_cu, sc, b->eff_words.ptr,
-                               b->eff_offsets.ptr, 1, n, d_fitness, d_sse, s);
+                               d_offsets, b->counts.ptr, cap, n, d_fitness, d_sse, s);
 }
 
 int galop_batch_eval_device(galop_batch* b, const uint32_t* d_words, const int64_t* d_offsets,

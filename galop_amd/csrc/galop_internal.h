@@ -48,6 +48,14 @@ hipError_t launch_const_table(float* consts, hipStream_t stream);
 // Device intron removal: per-program effective counts, scan, scatter.  With
 // `order`, entry i is program order[i] and counts / eff_offsets are indexed
 // by i; without, by program.
+// Intron removal into the programs' own slots (batched evaluation): thread i
+// writes the effective code of program order[i] to the end of its raw slot in
+// `slot` (cap words, laid out like offsets relative to offsets[0]) and its
+// length to counts[i].
+hipError_t launch_effective_in_slot(int num_registers, int op_set, int input_protect,
+                                    const uint32_t* words, const int64_t* offsets, int64_t n,
+                                    const int32_t* order, uint32_t* slot, int64_t cap,
+                                    int64_t* counts, hipStream_t stream);
 hipError_t launch_effective_counts(int num_registers, int op_set, int input_protect,
                                    const uint32_t* words, const int64_t* offsets,
                                    int64_t n, int64_t* counts, hipStream_t stream,
@@ -87,12 +95,14 @@ BatchTuning resolve_batch_tuning(const DevData& ds, int num_registers, BatchTuni
 // the queue head.
 hipError_t launch_batch_order(const int64_t* offsets, int64_t n, const BatchScratch& s,
                               hipStream_t stream);
-// Persistent evaluation over the order array (after launch_batch_order).  The
-// CSR `offs` is indexed by order position when by_position, else by program.
+// Persistent evaluation over the order array (after launch_batch_order) of
+// the CSR (words, offs) indexed by program; or, with eff_counts, of the
+// effective code effective_in_slot wrote into `words` (a buffer of `cap`
+// words laid out like offs).
 hipError_t launch_batched_eval(const DevData& ds, int num_registers, BatchTuning t, int n_cu,
                                const BatchScratch& s, const uint32_t* words,
-                               const int64_t* offs, int by_position, int64_t n,
-                               float* fitness, double* sse, hipStream_t stream);
+                               const int64_t* offs, const int64_t* eff_counts, int64_t cap,
+                               int64_t n, float* fitness, double* sse, hipStream_t stream);
 // launch_batch_order + launch_batched_eval over a CSR indexed by program.
 hipError_t launch_batched(const DevData& ds, int num_registers, BatchTuning t, int n_cu,
                           const BatchScratch& s, const uint32_t* words,

@@ -386,6 +386,61 @@ __global__ void effective_scatter_kernel(DecodeCfg cfg, const uint32_t* __restri
     flush();
 }
 
+// Intron removal for the batched evaluation in one pass, without a prefix
+// sum: thread i walks program p = order[i] backwards and writes its effective
+// words backwards from the end of p's own raw slot, in a buffer laid out like
+// the raw CSR (positions relative to offsets[0]).  A program has no more
+// effective words than raw ones, so they stay inside its slot: its effective
+// code is slot[end_p - counts[i], end_p).  Writes go through the same 16-byte
+// staging as effective_scatter_kernel; a whole block is stored when it lies
+// inside [the slot's start, min(end_p, cap)), so the words below the
+// effective code that it may cover are this slot's own, and no word at or
+// past `cap` is written.
+__global__ void effective_in_slot_kernel(DecodeCfg cfg, const uint32_t* __restrict__ words,
+                                         const int64_t* __restrict__ offsets,
+                                         const int32_t* __restrict__ order, int64_t n,
+                                         uint32_t* __restrict__ slot, int64_t cap,
+                                         int64_t* __restrict__ counts) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t p = order[i];
+    const int64_t base = offsets[0];
+    const int64_t start = offsets[p] - base;
+    const int64_t end = offsets[p + 1] - base;
+    const int64_t lim = end < cap ? end : cap;
+    int64_t pos = end;
+    int64_t sblk = -1;
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    auto flush = [&]() {
+        if (sblk < 0) return;
+        if (sblk >= start && sblk + 4 <= lim) {
+            *reinterpret_cast<uint4*>(slot + sblk) = make_uint4(s0, s1, s2, s3);
+        } else {
+            const uint32_t sv[4] = {s0, s1, s2, s3};
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (sblk + t >= start && sblk + t >= pos && sblk + t < lim) slot[sblk + t] = sv[t];
+        }
+    };
+    uint32_t live = 1u << cfg.out_reg;
+    walk_backward(words, offsets[p], offsets[p + 1], [&](uint32_t w) {
+        if (!effective_step(w, cfg, live)) return;
+        --pos;
+        const int64_t b = pos & ~(int64_t)3;
+        if (b != sblk) {
+            flush();
+            sblk = b;
+        }
+        const int o = (int)(pos & 3);
+        s0 = o == 0 ? w : s0;
+        s1 = o == 1 ? w : s1;
+        s2 = o == 2 ? w : s2;
+        s3 = o == 3 ? w : s3;
+    });
+    flush();
+    counts[i] = end - pos;
+}
+
 // Offsets past the effective-word buffer are cut to its end, so that a
 // population larger than the buffer can never make a kernel read past it.
 __global__ void clamp_offsets_kernel(int64_t* offsets, int64_t n1, int64_t cap) {
@@ -400,6 +455,17 @@ static DecodeCfg make_cfg(int nr, int op_set, int iprot) {
     c.out_reg = iprot ? nr / 2 : 0;
     c.ext = op_set;
     return c;
+}
+
+hipError_t launch_effective_in_slot(int num_registers, int op_set, int input_protect,
+                                    const uint32_t* words, const int64_t* offsets, int64_t n,
+                                    const int32_t* order, uint32_t* slot, int64_t cap,
+                                    int64_t* counts, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(effective_in_slot_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                       stream, make_cfg(num_registers, op_set, input_protect), words, offsets,
+                       order, n, slot, cap, counts);
+    return hipGetLastError();
 }
 
 hipError_t launch_effective_counts(int num_registers, int op_set, int input_protect,
@@ -617,7 +683,8 @@ template <int NR, int K, int EXT>
 __global__ void __launch_bounds__(256)
 batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ offs,
                const int32_t* __restrict__ order, int32_t* __restrict__ queue_head,
-               int64_t n, int chunk, int by_position, DevData ds, float* __restrict__ fitness,
+               int64_t n, int chunk, const int64_t* __restrict__ eff_counts, int64_t cap,
+               DevData ds, float* __restrict__ fitness,
                double* __restrict__ sse_out) {
     const int lane = (int)(threadIdx.x & (kWave - 1));
     const ConstRegs cr = load_const_regs(ds.consts, lane);
@@ -632,9 +699,17 @@ batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ o
         if (lane < cnt) {
             const int64_t q = (int64_t)base + lane;
             p_l = order[q];
-            const int64_t e = by_position ? q : (int64_t)p_l;
-            lo_l = offs[e];
-            hi_l = offs[e + 1];
+            if (eff_counts) {
+                // effective code at the end of the program's raw slot
+                // (effective_in_slot_kernel), cut at the buffer's end
+                const int64_t end = offs[p_l + 1] - offs[0];
+                hi_l = end < cap ? end : cap;
+                lo_l = end - eff_counts[q];
+                lo_l = lo_l < hi_l ? lo_l : hi_l;
+            } else {
+                lo_l = offs[p_l];
+                hi_l = offs[p_l + 1];
+            }
         }
         for (int j = 0; j < cnt; ++j) {
             const int64_t p = __builtin_amdgcn_readlane(p_l, j);
@@ -667,8 +742,8 @@ BatchTuning resolve_batch_tuning(const DevData& ds, int num_registers, BatchTuni
 template <int NR, int K, int EXT>
 static hipError_t launch_batched_t(const DevData& ds, const BatchTuning& t, int n_cu,
                                    const BatchScratch& s, const uint32_t* words,
-                                   const int64_t* offs, int by_position, int64_t n,
-                                   float* fitness, double* sse, hipStream_t stream) {
+                                   const int64_t* offs, const int64_t* eff_counts, int64_t cap,
+                                   int64_t n, float* fitness, double* sse, hipStream_t stream) {
     const int wpb = t.waves_per_block;
     const int64_t waves = (int64_t)n_cu * t.workers_per_cu;
     const unsigned grid = (unsigned)((waves + wpb - 1) / wpb);
@@ -677,8 +752,8 @@ static hipError_t launch_batched_t(const DevData& ds, const BatchTuning& t, int 
     int64_t chunk = n / (waves * 4);
     chunk = chunk < 1 ? 1 : (chunk > t.chunk ? t.chunk : chunk);
     hipLaunchKernelGGL((batched_kernel<NR, K, EXT>), dim3(grid), dim3(wpb * kWave), 0, stream,
-                       words, offs, s.order, s.queue_head, n, (int)chunk, by_position, ds, fitness,
-                       sse);
+                       words, offs, s.order, s.queue_head, n, (int)chunk, eff_counts, cap, ds,
+                       fitness, sse);
     return hipGetLastError();
 }
 
@@ -700,15 +775,16 @@ hipError_t launch_batch_order(const int64_t* offsets, int64_t n, const BatchScra
 
 hipError_t launch_batched_eval(const DevData& ds, int num_registers, BatchTuning t, int n_cu,
                                const BatchScratch& s, const uint32_t* words, const int64_t* offs,
-                               int by_position, int64_t n, float* fitness, double* sse,
+                               const int64_t* eff_counts, int64_t cap, int64_t n,
+                               float* fitness, double* sse,
                                hipStream_t stream) {
     if (n <= 0) return hipSuccess;
     if (n > s.max_programs || n > (int64_t)INT32_MAX - (int64_t(1) << 22)) return hipErrorInvalidValue;
     t = resolve_batch_tuning(ds, num_registers, t, n_cu);
 #define GALOP_CASE(NRV, KV)                                                             \
     if (num_registers == NRV && t.cases_per_lane == KV)                                 \
-        return ds.op_set ? launch_batched_t<NRV, KV, 1>(ds, t, n_cu, s, words, offs, by_position, n, fitness, sse, stream) \
-                         : launch_batched_t<NRV, KV, 0>(ds, t, n_cu, s, words, offs, by_position, n, fitness, sse, stream);
+        return ds.op_set ? launch_batched_t<NRV, KV, 1>(ds, t, n_cu, s, words, offs, eff_counts, cap, n, fitness, sse, stream) \
+                         : launch_batched_t<NRV, KV, 0>(ds, t, n_cu, s, words, offs, eff_counts, cap, n, fitness, sse, stream);
     GALOP_CASE(8, 1) GALOP_CASE(8, 2) GALOP_CASE(8, 4) GALOP_CASE(8, 8)
     GALOP_CASE(16, 1) GALOP_CASE(16, 2) GALOP_CASE(16, 4)
 #undef GALOP_CASE
@@ -720,7 +796,8 @@ hipError_t launch_batched(const DevData& ds, int num_registers, BatchTuning t, i
                           int64_t n, float* fitness, double* sse, hipStream_t stream) {
     hipError_t e = launch_batch_order(offsets, n, s, stream);
     if (e != hipSuccess) return e;
-    return launch_batched_eval(ds, num_registers, t, n_cu, s, words, offsets, 0, n, fitness, sse,
+    return launch_batched_eval(ds, num_registers, t, n_cu, s, words, offsets, nullptr, 0, n,
+                               fitness, sse,
                                stream);
 }
 

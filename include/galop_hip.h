@@ -169,9 +169,11 @@ int galop_strip_introns_device(galop_ctx* ctx, const galop_dataset* ds,
  * up to max_programs programs (and max_words instruction words) on one
  * dataset, and a hipGraph of one generation that is re-launched while the
  * program buffers stay the same:
- *   [intron removal: count -> scan -> scatter]   (flags & GALOP_BATCH_STRIP_INTRONS)
- *   length histogram -> bucket starts -> longest-first order -> persistent
- *   work-queue evaluation.
+ *   length histogram -> bucket starts -> longest-first order ->
+ *   [intron removal in that order, one pass: each program's effective code
+ *    to the end of its own raw slot]   (flags & GALOP_BATCH_STRIP_INTRONS;
+ *    max_words must cover the raw words, offsets[n] - offsets[0]) ->
+ *   persistent work-queue evaluation.
  * Intron removal is exact (src/Introns.h:28-57), so fitness is bitwise that
  * of the raw programs. */
 #define GALOP_BATCH_STRIP_INTRONS 1

@@ -53,6 +53,38 @@ def test_batched_equals_plain(gpu_ctx, strip, N, nr):
     plan.close()
 
 
+@pytest.mark.parametrize("strip", [False, True])
+def test_batched_window_and_edge_lengths(gpu_ctx, strip):
+    """A window [a, b) of a larger CSR (offsets not starting at 0: the
+    effective code is laid out relative to offsets[0]), a plan larger than
+    the window, and programs of 0, 1, 2, 3 and 5 words among long ones."""
+    from galop_amd import BatchPlan, eval_csr_device
+    rng = np.random.default_rng(77)
+    isa, dds, words, offsets = _setup(gpu_ctx, rng, P=3000, lo=1, hi=400)
+    lengths = np.diff(offsets)
+    lengths[100:110] = [0, 1, 2, 3, 5, 0, 1, 2, 3, 5]
+    offsets = np.concatenate([[0], np.cumsum(lengths)]).astype(np.int64)
+    words = rng.integers(0, 2**32, size=int(offsets[-1]), dtype=np.uint64).astype(np.uint32)
+    words &= np.uint32(0x87070707)
+    a, b = 97, 2411
+    s = stream()
+    dw, do = dev_words(words), dev_i64(offsets)
+    n = b - a
+    ref = torch.empty(n, dtype=torch.float32, device="cuda")
+    ref_s = torch.empty(n, dtype=torch.float64, device="cuda")
+    eval_csr_device(dds, dw, do[a:], n, ref, ref_s, stream=s)
+    plan = BatchPlan(dds, offsets.size, max_words=words.size, strip_introns=strip)
+    fit = torch.empty(n, dtype=torch.float32, device="cuda")
+    sse = torch.empty(n, dtype=torch.float64, device="cuda")
+    for use_graph in (False, True):
+        fit.fill_(float("nan"))
+        plan.eval(dw, do[a:], n, fit, sse, use_graph=use_graph, stream=s)
+        torch.cuda.synchronize()
+        assert same_bits(fit.cpu().numpy(), ref.cpu().numpy()).all(), use_graph
+        assert _same64(sse.cpu().numpy(), ref_s.cpu().numpy()), use_graph
+    plan.close()
+
+
 def test_graph_replays_match_eager(gpu_ctx):
     from galop_amd import BatchPlan
     rng = np.random.default_rng(41)
