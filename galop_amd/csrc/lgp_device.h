@@ -169,21 +169,31 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
     const int s2 = (int)((w >> 24) & (uint32_t)(NR - 1));
     const bool kc = (w >> 31) != 0u;
 
+// A constant operand stays one scalar value for all K cases: the compute is
+// written out in both branches, so it is never copied into K vector values.
+// (The empty asm ends the constant branch with something the other branch
+// lacks; otherwise the two identical tails are merged behind a per-case
+// select of b, which brings the copies back.)
 #define GALOP_BIN(EXPR)                                                   \
     do {                                                                  \
-        float av[K], bv[K];                                               \
+        float av[K], cv[K];                                               \
         _Pragma("unroll") for (int k = 0; k < K; ++k) av[k] = r.get(k, s1);   \
         if (kc) {                                                         \
-            const float bc = wave_bcast(cr.pool, s2);                     \
-            _Pragma("unroll") for (int k = 0; k < K; ++k) bv[k] = bc;     \
+            const float b = wave_bcast(cr.pool, s2);                      \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
+                const float a = av[k];                                    \
+                cv[k] = (EXPR);                                           \
+            }                                                             \
+            _Pragma("unroll") for (int k = 0; k < K; ++k)                 \
+                __asm__("" : "+v"(cv[k]));                                \
         } else {                                                          \
+            float bv[K];                                                  \
             _Pragma("unroll") for (int k = 0; k < K; ++k) bv[k] = r.get(k, s2); \
-        }                                                                 \
-        float cv[K];                                                      \
-        _Pragma("unroll") for (int k = 0; k < K; ++k) {                   \
-            const float a = av[k];                                        \
-            const float b = bv[k];                                        \
-            cv[k] = (EXPR);                                               \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
+                const float a = av[k];                                    \
+                const float b = bv[k];                                    \
+                cv[k] = (EXPR);                                           \
+            }                                                             \
         }                                                                 \
         _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]);    \
     } while (0)
