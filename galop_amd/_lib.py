@@ -104,6 +104,7 @@ SIGNATURES = {
     "galop_vary_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, ctypes.c_uint64, _vp]),
     "galop_init_population_device": (ctypes.c_int, [_vp, _vp, ctypes.c_uint64, _vp, _vp, _vp, _vp]),
     "galop_generation_stats_device": (ctypes.c_int, [_vp, _vp, _vp, _i32, _i32, _vp, _vp]),
+    "galop_mt19937_fill_device": (ctypes.c_int, [_vp, _vp, _i64, ctypes.c_uint32, _vp, _vp, _vp]),
     # include/galop_workload.h
     "galop_fill_random_words_device": (ctypes.c_int, [_vp, _i64, _i32, ctypes.c_uint64, _vp]),
     "galop_fill_effective_programs_device": (ctypes.c_int, [_vp, _vp, _vp, _i64, _i32, _i32, _i32,

@@ -321,6 +321,7 @@ __global__ void init_kernel(ge::VaryCfg c, uint64_t base, uint32_t* __restrict__
 // in any order, and the first maximum (strict >, from -1) is reduced across
 // lanes by value, then lowest index.
 constexpr int kStatChunks = 256;
+constexpr int kStatGroup = 8;
 struct StatChunk {
     double sum;
     long long len;
@@ -338,20 +339,45 @@ stats_chunk_kernel(const float* __restrict__ fit, const uint8_t* __restrict__ le
     long long ls = 0;
     float b = -1.0f;
     int64_t bi = INT64_MAX;
-    for (int64_t base = lo; base < hi; base += 64) {
-        const int64_t i = base + lane;
-        float f = 0.0f;
-        if (i < hi) {
-            f = fit[i];
-            ls += len[i];
-            if (f > b) {
-                b = f;
-                bi = i;
-            }
+    // Tiles of 64 programs in groups of kStatGroup; the next group's loads are
+    // issued before the current group is summed, so the serial FP64 chain does
+    // not wait on memory.
+    constexpr int G = kStatGroup;
+    float fa[G], fn[G];
+    uint32_t la[G], ln[G];
+    auto load = [&](int64_t gbase, float (&f)[G], uint32_t (&l)[G]) {
+#pragma unroll
+        for (int t = 0; t < G; ++t) {
+            const int64_t i = gbase + 64 * t + lane;
+            f[t] = i < hi ? fit[i] : 0.0f;
+            l[t] = i < hi ? (uint32_t)len[i] : 0u;
         }
-        const int n = (int)min((int64_t)64, hi - base);  // wave-uniform
-        const int fb = __float_as_int(f);
-        for (int j = 0; j < n; ++j) s += (double)__int_as_float(__builtin_amdgcn_readlane(fb, j));
+    };
+    load(lo, fa, la);
+    for (int64_t gbase = lo; gbase < hi; gbase += 64 * G) {
+        load(gbase + 64 * G, fn, ln);
+#pragma unroll
+        for (int t = 0; t < G; ++t) {
+            const int64_t tb = gbase + 64 * t;
+            const int64_t i = tb + lane;
+            if (i < hi) {
+                ls += la[t];
+                if (fa[t] > b) {
+                    b = fa[t];
+                    bi = i;
+                }
+            }
+            const int64_t rem = hi - tb;
+            const int n = (int)(rem < 64 ? (rem > 0 ? rem : 0) : 64);  // wave-uniform
+            const int fb = __float_as_int(fa[t]);
+            for (int j = 0; j < n; ++j)
+                s += (double)__int_as_float(__builtin_amdgcn_readlane(fb, j));
+        }
+#pragma unroll
+        for (int t = 0; t < G; ++t) {
+            fa[t] = fn[t];
+            la[t] = ln[t];
+        }
     }
     for (int off = 32; off >= 1; off >>= 1) {
         ls += __shfl_xor(ls, off);
@@ -373,22 +399,49 @@ stats_chunk_kernel(const float* __restrict__ fit, const uint8_t* __restrict__ le
 }
 
 // The fixed combine order of the chunks; strict > keeps the lowest index.
-__global__ void stats_finish_kernel(const StatChunk* __restrict__ chunks,
-                                    const uint8_t* __restrict__ len, int64_t P,
-                                    int32_t generation, galop_generation_stats* __restrict__ out) {
-    if (threadIdx.x != 0) return;
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int j) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), j);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// One wave: lane L loads chunks L, 64+L, 128+L and 192+L, and the chunks are
+// then combined in order through lane reads.
+__global__ void __launch_bounds__(64)
+stats_finish_kernel(const StatChunk* __restrict__ chunks, const uint8_t* __restrict__ len,
+                    int64_t P, int32_t generation, galop_generation_stats* __restrict__ out) {
+    static_assert(kStatChunks == 4 * 64, "four chunks per lane");
+    const int lane = (int)threadIdx.x;
+    double cs[4];
+    long long cl[4];
+    float cb[4];
+    int ci[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const StatChunk c = chunks[64 * q + lane];
+        cs[q] = c.sum;
+        cl[q] = c.len;
+        cb[q] = c.best;
+        ci[q] = c.idx;
+    }
     double sum = 0.0;
     long long len_sum = 0;
     float best = -1.0f;
     int best_idx = 0;
-    for (int k = 0; k < kStatChunks; ++k) {
-        sum += chunks[k].sum;
-        len_sum += chunks[k].len;
-        if (chunks[k].best > best) {
-            best = chunks[k].best;
-            best_idx = chunks[k].idx;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        for (int j = 0; j < 64; ++j) {
+            sum += __longlong_as_double((long long)lane_u64((uint64_t)__double_as_longlong(cs[q]), j));
+            len_sum += (long long)lane_u64((uint64_t)cl[q], j);
+            const float bj = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(cb[q]), j));
+            const int ij = __builtin_amdgcn_readlane(ci[q], j);
+            if (bj > best) {
+                best = bj;
+                best_idx = ij;
+            }
         }
     }
+    if (lane != 0) return;
     galop_generation_stats st;
     st.generation = generation;
     st.best_fitness = best;
@@ -397,6 +450,57 @@ __global__ void stats_finish_kernel(const StatChunk* __restrict__ chunks,
     st.mean_length = (float)((double)len_sum / (double)P);
     st.best_index = best_idx;
     *out = st;
+}
+
+// ---- the engine's std::mt19937 stream on the device ----------------------------
+// In stream form the generator is x[k] = x[k-227] ^ twist(x[k-624], x[k-623])
+// for k >= 624 (twist as in MT19937's state update), and draw i is
+// temper(x[624 + i]) when x[0..623] is a state whose next draw starts a fresh
+// twist.  The 227 words x[n .. n+226] depend only on words before n, so a
+// workgroup makes 227 words per step, thread j taking position j: x[k-227] is
+// the word the same thread made one step before (kept in a register), and
+// x[k-624], x[k-623] come from a 1024-word ring in LDS.  A step reads slots
+// n-624 .. n-397 and writes slots n .. n+226, i.e. n-1024 .. n-798 mod 1024,
+// which are no longer needed.  Steps are separated by a barrier whose fences
+// cover LDS only: the outputs' global stores are never waited for.
+constexpr int kMtN = 624, kMtStep = 227, kMtRing = 1024;
+
+__device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
+    y ^= y >> 11;
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= y >> 18;
+    return y;
+}
+
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+__global__ void __launch_bounds__(256)
+mt_stream_kernel(const uint32_t* __restrict__ key, int64_t count, uint32_t mask,
+                 uint32_t* __restrict__ out, uint32_t* __restrict__ state_out) {
+    __shared__ uint32_t ring[kMtRing];
+    const int j = (int)threadIdx.x;  // blockDim.x == 256 >= kMtStep
+    for (int i = j; i < kMtN; i += 256) ring[i] = key[i];
+    uint32_t c = j < kMtStep ? key[kMtN - kMtStep + j] : 0u;  // x[k - 227]
+    lds_barrier();
+    for (int64_t n = kMtN; n < kMtN + count; n += kMtStep) {
+        if (j < kMtStep) {
+            const int64_t k = n + j;
+            const uint32_t a = ring[(k - kMtN) & (kMtRing - 1)];
+            const uint32_t b = ring[(k - kMtN + 1) & (kMtRing - 1)];
+            const uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
+            c = c ^ (y >> 1) ^ ((b & 1u) ? 0x9908b0dfu : 0u);
+            ring[k & (kMtRing - 1)] = c;
+            if (k - kMtN < count) out[k - kMtN] = mt_temper(c) & mask;
+        }
+        lds_barrier();
+    }
+    // the state after `count` draws: x[count .. count + 623], next draw a fresh twist
+    for (int i = j; i < kMtN; i += 256) state_out[i] = ring[(count + i) & (kMtRing - 1)];
 }
 
 }  // namespace
@@ -454,6 +558,31 @@ int galop_vary_device(galop_ctx* ctx, const galop_vary_cfg* cfg, const uint32_t*
     }
     e = hipGetLastError();
     return e == hipSuccess ? GALOP_OK : hip_fail(ctx, e, "galop_vary_device: launch");
+}
+
+int galop_mt19937_fill_device(galop_ctx* ctx, const uint32_t* key, int64_t count, uint32_t mask,
+                              uint32_t* d_out, uint32_t* state_out, void* stream) {
+    if (!ctx) return set_error(ctx, GALOP_ERR_INVALID, "galop_mt19937_fill_device: NULL context");
+    if (!key || !state_out || count < 0 || (count > 0 && !d_out))
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_mt19937_fill_device: bad argument");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    Staging& st = ctx->staging;
+    e = st.d_small.reserve(2 * kMtN * sizeof(uint32_t));
+    if (e != hipSuccess) return hip_fail(ctx, e, "galop_mt19937_fill_device: scratch");
+    auto* d_key = reinterpret_cast<uint32_t*>(st.d_small.ptr);
+    uint32_t* d_state = d_key + kMtN;
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    e = hipMemcpyAsync(d_key, key, kMtN * sizeof(uint32_t), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(mt_stream_kernel, dim3(1), dim3(256), 0, s, d_key, count, mask, d_out,
+                           d_state);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(state_out, d_state, kMtN * sizeof(uint32_t), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    return e == hipSuccess ? GALOP_OK : hip_fail(ctx, e, "galop_mt19937_fill_device");
 }
 
 int galop_init_population_device(galop_ctx* ctx, const galop_vary_cfg* cfg, uint64_t base,

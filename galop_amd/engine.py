@@ -70,6 +70,20 @@ def init_population_device(ctx, cfg, base, instr, lengths, fitness, stream=None)
                                                _p(lengths), _p(fitness), stream), ctx.handle)
 
 
+def mt19937_fill_device(ctx, key, count, mask, out, stream=None):
+    """`count` draws of the std::mt19937 whose state is key[624] (next draw a
+    fresh twist), ANDed with `mask`, into the device tensor `out` (int32 view
+    of uint32).  Returns the state after the draws, in the same form."""
+    key = np.ascontiguousarray(key, np.uint32)
+    if key.size != 624:
+        raise ValueError("key must hold 624 words")
+    state = np.empty(624, np.uint32)
+    check(ctx.lib.galop_mt19937_fill_device(ctx.handle, key.ctypes.data, int(count),
+                                            ctypes.c_uint32(int(mask)), _p(out),
+                                            state.ctypes.data, stream), ctx.handle)
+    return state
+
+
 def generation_stats_device(ctx, fitness, lengths, population, generation, stream=None):
     out = _lib.galop_generation_stats()
     check(ctx.lib.galop_generation_stats_device(ctx.handle, _p(fitness), _p(lengths),

@@ -16,7 +16,10 @@ same fitness evaluator.
 numpy's MT19937 with legacy seeding produces std::mt19937's sequence for a
 32-bit seed (init_genrand), and its raw outputs are the generator's 32-bit
 words; uniform_int_distribution<uint32_t>(0, UINT32_MAX) over that generator
-returns the word unchanged (libstdc++: the ranges are equal).
+returns the word unchanged (libstdc++: the ranges are equal).  The serial
+initial population -- population x stride draws -- is drawn on the device
+from the host generator's state (galop_mt19937_fill_device), and the host
+generator then continues from the state that returns.
 """
 import math
 
@@ -24,7 +27,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from .engine import VaryCfg, generation_stats_device, init_population_device, vary_device
+from .engine import (VaryCfg, generation_stats_device, init_population_device, mt19937_fill_device,
+                     vary_device)
 from .evaluator import DeviceDataset, Isa, eval_dense_device
 
 
@@ -132,8 +136,16 @@ class DeviceEngine:
                 init_population_device(self.ctx, cfg, base, self.instr[b], self.lengths[b],
                                        self.fitness[b], stream=self.stream)
             else:
-                words = serial_init_words(self._rng, P, S, cfg.num_registers)
-                self.instr[b].copy_(torch.from_numpy(words.reshape(-1).view(np.int32)))
+                # the serial initializer's P*S draws are made on the device from
+                # the host generator's state, which then continues from theirs
+                st = self._rng.state
+                if st["state"]["pos"] != 624:
+                    raise RuntimeError("the host generator must be at a twist boundary")
+                key = mt19937_fill_device(self.ctx, st["state"]["key"], P * S,
+                                          encode_mask(cfg.num_registers), self.instr[b],
+                                          stream=self.stream)
+                self._rng.state = {"bit_generator": "MT19937",
+                                   "state": {"key": key, "pos": 624}}
                 self.lengths[b].fill_(cfg.starting_length)
 
     # ---- src/LGPEngine.cpp:132-145 ---------------------------------------------

@@ -62,6 +62,17 @@ int galop_init_population_device(galop_ctx* ctx, const galop_vary_cfg* cfg,
                                  uint64_t base, uint32_t* d_instr, uint8_t* d_len,
                                  float* d_fit, void* stream);
 
+/* The serial initializer (src/LGPEngine.cpp:70-79) on the device: `count`
+ * draws of the engine's std::mt19937, each ANDed with `mask` (one AND is
+ * ISA::encode_from_random for a register count: 7 | r<<8 | r<<16 | r<<24 |
+ * 1<<31, r = registers - 1), to d_out.  key[624] (host memory) is the
+ * generator's state with the next draw starting a fresh twist (the state
+ * right after seeding, for instance); state_out[624] (host memory) receives
+ * the state after the `count` draws in the same form, so the host generator
+ * continues from it.  Blocking. */
+int galop_mt19937_fill_device(galop_ctx* ctx, const uint32_t* key, int64_t count, uint32_t mask,
+                              uint32_t* d_out, uint32_t* state_out, void* stream);
+
 /* LGPEngine::compute_stats (src/LGPEngine.cpp:477-517): 256 fixed chunks,
  * FP64 sums in index order per chunk, combined in chunk order; best is the
  * first maximum.  Ordered after earlier work on `stream` (NULL: the context

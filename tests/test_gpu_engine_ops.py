@@ -94,6 +94,28 @@ def test_device_stats_match_reference(gpu_ctx):
     assert np.array_equal(np.array(got, np.float64), f["final_stats"])
 
 
+@pytest.mark.parametrize("count", [0, 1, 226, 227, 228, 623, 624, 625, 1_000_003])
+@pytest.mark.parametrize("seed,mask", [(42, 0xFFFFFFFF), (7, 0x87070707)])
+def test_device_mt19937_stream(gpu_ctx, count, seed, mask):
+    """galop_mt19937_fill_device: the draws of std::mt19937 (numpy's MT19937
+    with legacy seeding) after seeding, ANDed with the mask, and a returned
+    state from which the host generator continues as if it had made the draws
+    itself."""
+    import torch
+    from galop_amd.engine import mt19937_fill_device
+    from galop_amd.evolve import host_generator
+    from gpu_util import stream
+    s = stream()
+    a, b = host_generator(seed), host_generator(seed)
+    out = torch.zeros(max(count, 1), dtype=torch.int32, device="cuda")
+    state = mt19937_fill_device(gpu_ctx, a.state["state"]["key"], count, mask, out, stream=s)
+    torch.cuda.synchronize()
+    ref = b.random_raw(count).astype(np.uint32) & np.uint32(mask)
+    assert np.array_equal(out.cpu().numpy().view(np.uint32)[:count], ref)
+    a.state = {"bit_generator": "MT19937", "state": {"key": state, "pos": 624}}
+    assert np.array_equal(a.random_raw(2000), b.random_raw(2000))
+
+
 def _stats_population(kind, P):
     rng = np.random.default_rng(P)
     lens = rng.integers(0, 256, P).astype(np.uint8)
