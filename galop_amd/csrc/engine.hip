@@ -4,7 +4,7 @@
 // population in the engine's dense layout.  The operators are lgp_engine.h,
 // the same source the host build (engine_host.cpp) compiles.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include "../../include/galop_engine.h"
 #include "api_internal.h"
@@ -56,22 +56,35 @@ const char* check_cfg(const galop_vary_cfg* c) {
     return nullptr;
 }
 
+// The reference orders the population by the 64-bit key (fitness bits << 32) |
+// index, descending (fitness >= 0, so the bits order like the values).  The
+// same order comes from a stable radix sort of the 32-bit fitness bits,
+// descending, over a sequence that lists the programs from the highest index
+// down: equal fitness then keeps the higher index first.  The sort runs
+// rocPRIM's onesweep radix passes at every size (its default switches to a
+// merge sort up to 2^20 items).
 __global__ void elite_keys_kernel(const float* __restrict__ fit, int64_t n,
-                                  uint64_t* __restrict__ keys) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) keys[i] = ge::elite_key(fit[i], i);
+                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ ids) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const int64_t i = n - 1 - j;
+    keys[j] = __float_as_uint(fit[i]);
+    ids[j] = (uint32_t)i;
 }
+
+using EliteSortConfig = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                                   rocprim::default_config, 0>;
 
 // Elite e (the e-th largest key) to next slot e, with its length and fitness
 // (LGPEngine::copy_elite_to_next, src/LGPEngine.cpp:409-419).
-__global__ void elite_copy_kernel(ge::VaryCfg c, const uint64_t* __restrict__ sorted,
+__global__ void elite_copy_kernel(ge::VaryCfg c, const uint32_t* __restrict__ sorted_ids,
                                   const uint32_t* __restrict__ cur_instr,
                                   const uint8_t* __restrict__ cur_len,
                                   const float* __restrict__ cur_fit,
                                   uint32_t* __restrict__ next_instr,
                                   uint8_t* __restrict__ next_len, float* __restrict__ next_fit) {
     const int e = (int)blockIdx.x;  // one block of 64 threads per elite
-    const int64_t src = (int64_t)(uint32_t)sorted[e];
+    const int64_t src = (int64_t)sorted_ids[e];
     const int len = cur_len[src];
     for (int k = (int)threadIdx.x; k < len; k += (int)blockDim.x)
         next_instr[(int64_t)e * c.stride + k] = cur_instr[src * c.stride + k];
@@ -322,6 +335,14 @@ __global__ void init_kernel(ge::VaryCfg c, uint64_t base, uint32_t* __restrict__
 // lanes by value, then lowest index.
 constexpr int kStatChunks = 256;
 constexpr int kStatGroup = 8;
+
+// v as lane j holds it (j wave-uniform)
+__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int j) {
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, j);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), j);
+    return ((uint64_t)hi << 32) | lo;
+}
+
 struct StatChunk {
     double sum;
     long long len;
@@ -369,9 +390,16 @@ stats_chunk_kernel(const float* __restrict__ fit, const uint8_t* __restrict__ le
             }
             const int64_t rem = hi - tb;
             const int n = (int)(rem < 64 ? (rem > 0 ? rem : 0) : 64);  // wave-uniform
-            const int fb = __float_as_int(fa[t]);
-            for (int j = 0; j < n; ++j)
-                s += (double)__int_as_float(__builtin_amdgcn_readlane(fb, j));
+            if (n == 64) {
+                // each lane widens its value once; the chain reads them in order
+                const uint64_t du = (uint64_t)__double_as_longlong((double)fa[t]);
+#pragma unroll
+                for (int j = 0; j < 64; ++j) s += __longlong_as_double((long long)lane_u64(du, j));
+            } else {
+                const int fb = __float_as_int(fa[t]);
+                for (int j = 0; j < n; ++j)
+                    s += (double)__int_as_float(__builtin_amdgcn_readlane(fb, j));
+            }
         }
 #pragma unroll
         for (int t = 0; t < G; ++t) {
@@ -399,12 +427,6 @@ stats_chunk_kernel(const float* __restrict__ fit, const uint8_t* __restrict__ le
 }
 
 // The fixed combine order of the chunks; strict > keeps the lowest index.
-__device__ __forceinline__ uint64_t lane_u64(uint64_t v, int j) {
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, j);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(v >> 32), j);
-    return ((uint64_t)hi << 32) | lo;
-}
-
 // One wave: lane L loads chunks L, 64+L, 128+L and 192+L, and the chunks are
 // then combined in order through lane reads.
 __global__ void __launch_bounds__(64)
@@ -459,11 +481,13 @@ stats_finish_kernel(const StatChunk* __restrict__ chunks, const uint8_t* __restr
 // twist.  The 227 words x[n .. n+226] depend only on words before n, so a
 // workgroup makes 227 words per step, thread j taking position j: x[k-227] is
 // the word the same thread made one step before (kept in a register), and
-// x[k-624], x[k-623] come from a 1024-word ring in LDS.  A step reads slots
-// n-624 .. n-397 and writes slots n .. n+226, i.e. n-1024 .. n-798 mod 1024,
-// which are no longer needed.  Steps are separated by a barrier whose fences
-// cover LDS only: the outputs' global stores are never waited for.
-constexpr int kMtN = 624, kMtStep = 227, kMtRing = 1024;
+// x[k-624], x[k-623] come from a 2048-word ring in LDS.  Step t reads words
+// made in steps t-2 and t-3 only (n-624 .. n-396), so a barrier after every
+// second step suffices: between two barriers the waves are at most one step
+// apart, and the slots a step writes (n-2048 .. n-1822 mod 2048) are never
+// ones a step one behind still reads.  The barrier's fences cover LDS only:
+// the outputs' global stores are never waited for.
+constexpr int kMtN = 624, kMtStep = 227, kMtRing = 2048;
 
 __device__ __forceinline__ uint32_t mt_temper(uint32_t y) {
     y ^= y >> 11;
@@ -487,15 +511,25 @@ mt_stream_kernel(const uint32_t* __restrict__ key, int64_t count, uint32_t mask,
     for (int i = j; i < kMtN; i += 256) ring[i] = key[i];
     uint32_t c = j < kMtStep ? key[kMtN - kMtStep + j] : 0u;  // x[k - 227]
     lds_barrier();
-    for (int64_t n = kMtN; n < kMtN + count; n += kMtStep) {
+    for (int64_t n = kMtN; n < kMtN + count; n += 2 * kMtStep) {
         if (j < kMtStep) {
-            const int64_t k = n + j;
-            const uint32_t a = ring[(k - kMtN) & (kMtRing - 1)];
-            const uint32_t b = ring[(k - kMtN + 1) & (kMtRing - 1)];
-            const uint32_t y = (a & 0x80000000u) | (b & 0x7fffffffu);
-            c = c ^ (y >> 1) ^ ((b & 1u) ? 0x9908b0dfu : 0u);
-            ring[k & (kMtRing - 1)] = c;
-            if (k - kMtN < count) out[k - kMtN] = mt_temper(c) & mask;
+            // both steps read only words made before the last barrier: load
+            // all four first, so one LDS round trip serves the pair
+            uint32_t a[2], b[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int64_t k = n + h * kMtStep + j;
+                a[h] = ring[(k - kMtN) & (kMtRing - 1)];
+                b[h] = ring[(k - kMtN + 1) & (kMtRing - 1)];
+            }
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int64_t k = n + h * kMtStep + j;
+                const uint32_t y = (a[h] & 0x80000000u) | (b[h] & 0x7fffffffu);
+                c = c ^ (y >> 1) ^ ((b[h] & 1u) ? 0x9908b0dfu : 0u);
+                ring[k & (kMtRing - 1)] = c;
+                if (k - kMtN < count) out[k - kMtN] = mt_temper(c) & mask;
+            }
         }
         lds_barrier();
     }
@@ -523,22 +557,29 @@ int galop_vary_device(galop_ctx* ctx, const galop_vary_cfg* cfg, const uint32_t*
     const int64_t P = cfg->population;
     Staging& st = ctx->staging;
     if (cfg->elite_count > 0) {
-        size_t tmp_bytes = 0;
-        e = hipcub::DeviceRadixSort::SortKeysDescending(nullptr, tmp_bytes, st.d_keys.ptr,
-                                                         st.d_keys_sorted.ptr, (int)P, 0, 64, s);
-        if (e == hipSuccess) e = st.d_keys.reserve((size_t)P);
+        // d_keys holds the keys and ids in, d_keys_sorted the sorted pairs
+        e = st.d_keys.reserve((size_t)P);
         if (e == hipSuccess) e = st.d_keys_sorted.reserve((size_t)P);
+        if (e != hipSuccess) return hip_fail(ctx, e, "galop_vary_device: elite scratch");
+        uint32_t* keys_in = reinterpret_cast<uint32_t*>(st.d_keys.ptr);
+        uint32_t* ids_in = keys_in + P;
+        uint32_t* keys_out = reinterpret_cast<uint32_t*>(st.d_keys_sorted.ptr);
+        uint32_t* ids_out = keys_out + P;
+        size_t tmp_bytes = 0;
+        e = rocprim::radix_sort_pairs_desc<EliteSortConfig>(nullptr, tmp_bytes, keys_in, keys_out,
+                                                            ids_in, ids_out, (size_t)P, 0, 32, s);
         if (e == hipSuccess) e = st.d_sort_tmp.reserve(tmp_bytes);
         if (e != hipSuccess) return hip_fail(ctx, e, "galop_vary_device: elite scratch");
         hipLaunchKernelGGL(elite_keys_kernel, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, s,
-                           d_cur_fit, P, st.d_keys.ptr);
+                           d_cur_fit, P, keys_in, ids_in);
         tmp_bytes = st.d_sort_tmp.bytes();
-        e = hipcub::DeviceRadixSort::SortKeysDescending(st.d_sort_tmp.ptr, tmp_bytes, st.d_keys.ptr,
-                                                         st.d_keys_sorted.ptr, (int)P, 0, 64, s);
+        e = rocprim::radix_sort_pairs_desc<EliteSortConfig>(st.d_sort_tmp.ptr, tmp_bytes, keys_in,
+                                                            keys_out, ids_in, ids_out, (size_t)P, 0,
+                                                            32, s);
         if (e != hipSuccess) return hip_fail(ctx, e, "galop_vary_device: elite sort");
         hipLaunchKernelGGL(elite_copy_kernel, dim3((unsigned)cfg->elite_count), dim3(64), 0, s, c,
-                           st.d_keys_sorted.ptr, d_cur_instr, d_cur_len, d_cur_fit, d_next_instr,
-                           d_next_len, d_next_fit);
+                           ids_out, d_cur_instr, d_cur_len, d_cur_fit, d_next_instr, d_next_len,
+                           d_next_fit);
     }
     const int64_t n_pairs = (P - cfg->elite_count) / 2;
     if (n_pairs > 0) {
