@@ -38,11 +38,11 @@ SIMDS_PER_CU = 4               # one wave64 VALU instruction per SIMD per 4 cloc
 # (1,048,576 KiB copied: FETCH_SIZE 524,307 KiB, WRITE_SIZE 1,048,576 KiB).
 # They describe that exact population only.
 PLAIN_WORKLOAD_COUNTERS = {
-    "fetch_bytes": 321516.3 * 1024 * (1048576 / 524307.7),
-    "write_bytes": 30543.4 * 1024 * 1.0,
-    "valu_insts": 28191080292,
-    "salu_insts": 10101607500,
-    "valu_busy_pct": 81.2,
+    "fetch_bytes": 321515.9 * 1024 * (1048576 / 524307.4),
+    "write_bytes": 30430.6 * 1024 * 1.0,
+    "valu_insts": 27352885928,
+    "salu_insts": 10279931518,
+    "valu_busy_pct": 79.3,
 }
 
 
@@ -245,7 +245,7 @@ def bench_engine(ctx, population, cases, generations, seeds):
         "run_s_median": round(med, 4), "run_s": [round(w, 4) for w in walls],
         "spread": round((walls_sorted[-1] - walls_sorted[0]) / med, 3),
         "ms_per_generation": round(med * 1e3 / generations, 3),
-        "phases_last_run_s": {"init_host_draws_and_upload": round(t_init, 4),
+        "phases_last_run_s": {"init_population_device_mt19937": round(t_init, 4),
                               "eval": round(eval_ms / 1e3, 4),
                               "elites_selection_variation": round(vary_ms / 1e3, 4),
                               "stats_incl_readback": round(stats_ms / 1e3, 4)},
