@@ -47,6 +47,16 @@ DEVICE_FLAGS = [
 ]
 
 
+# Per-source flags.  kernels.hip: with one case per lane (K = 1) the LGP
+# register file is a single NR-float vector, each load of it has one
+# variable-index extract, and VectorCombine rewrites that into an element load
+# through memory; the register file then stays in memory and is placed in LDS,
+# so every register access becomes an LDS round trip.  Without the pass the
+# vector stays in VGPRs (indexed moves, as for K >= 2, whose code does not
+# change).
+SOURCE_FLAGS = {"kernels.hip": ["-mllvm", "-disable-vector-combine"]}
+
+
 def _run(cmd):
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
     out = "\n".join(l for l in r.stdout.splitlines()
@@ -63,7 +73,8 @@ def build_library(verbose=False):
 
     def compile_one(src):
         obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-        _run([HIPCC, *DEVICE_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+        _run([HIPCC, *DEVICE_FLAGS, *SOURCE_FLAGS.get(src, []), "-c", os.path.join(CSRC, src),
+              "-o", obj])
         return obj
 
     with ThreadPoolExecutor(max_workers=len(srcs)) as ex:

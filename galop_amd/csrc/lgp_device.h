@@ -366,17 +366,28 @@ __device__ __forceinline__ void load_window(const uint32_t* __restrict__ prog, i
     e.m3 = all(3);
 }
 
+// Marks the effective words of the n <= kSkipMaxWords words already loaded
+// into e (load_window).
+template <int NR, int EXT>
+__device__ __forceinline__ void mark_window(EffProgram& e, int n, int out_reg, int iprot) {
+    uint32_t live = 1u << out_reg;
+    const auto cnt = [&](int c) { return min(kWave, max(0, n - c * kWave)); };
+    // chunks past the program hold no words and leave `live` as it is
+    e.m1 = 0;
+    e.m2 = 0;
+    e.m3 = 0;
+    if (n > 3 * kWave) e.m3 = mark_chunk<NR, EXT>(e.ch3, cnt(3), out_reg, iprot, live);
+    if (n > 2 * kWave) e.m2 = mark_chunk<NR, EXT>(e.ch2, cnt(2), out_reg, iprot, live);
+    if (n > kWave) e.m1 = mark_chunk<NR, EXT>(e.ch1, cnt(1), out_reg, iprot, live);
+    e.m0 = mark_chunk<NR, EXT>(e.ch0, cnt(0), out_reg, iprot, live);
+}
+
 template <int NR, int EXT>
 __device__ __forceinline__ void mark_effective(const uint32_t* __restrict__ prog, int64_t len,
                                                int lane, int out_reg, int iprot, EffProgram& e) {
     const int n = (int)len;  // <= kSkipMaxWords
     load_window(prog, n, lane, e);
-    uint32_t live = 1u << out_reg;
-    const auto cnt = [&](int c) { return min(kWave, max(0, n - c * kWave)); };
-    e.m3 = mark_chunk<NR, EXT>(e.ch3, cnt(3), out_reg, iprot, live);
-    e.m2 = mark_chunk<NR, EXT>(e.ch2, cnt(2), out_reg, iprot, live);
-    e.m1 = mark_chunk<NR, EXT>(e.ch1, cnt(1), out_reg, iprot, live);
-    e.m0 = mark_chunk<NR, EXT>(e.ch0, cnt(0), out_reg, iprot, live);
+    mark_window<NR, EXT>(e, n, out_reg, iprot);
 }
 
 // The marked words in program order; one instance of the interpreter step.
