@@ -5,6 +5,7 @@
 #include <functional>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "../../include/galop_hip.h"
 #include "galop_internal.h"
@@ -103,6 +104,10 @@ struct Staging {
     DeviceBuf<uint64_t> d_keys_sorted;
     DeviceBuf<uint8_t> d_sort_tmp;
     DeviceBuf<uint8_t> d_small;    // small device results (generation statistics)
+    DeviceBuf<uint32_t> d_mt;      // mt19937 jump: segment windows | source streams
+    DeviceBuf<int32_t> d_mt_poly;  // mt19937 jump: set bits of t^(2^k) mod phi, k = 0..mt_poly_k
+    std::vector<int64_t> mt_poly_off;  // list k is d_mt_poly[off[k], off[k+1])
+    int mt_poly_k = -1;
     BatchBufs batch;
     void release() {
         d_words.release();
@@ -122,6 +127,10 @@ struct Staging {
         d_keys_sorted.release();
         d_sort_tmp.release();
         d_small.release();
+        d_mt.release();
+        d_mt_poly.release();
+        mt_poly_off.clear();
+        mt_poly_k = -1;
         batch.release();
     }
 };

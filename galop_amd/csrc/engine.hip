@@ -9,6 +9,7 @@
 #include "../../include/galop_engine.h"
 #include "api_internal.h"
 #include "lgp_engine.h"
+#include "mt_jump.h"
 
 using namespace galop;
 namespace ge = galop_engine;
@@ -503,10 +504,13 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
-__global__ void __launch_bounds__(256)
-mt_stream_kernel(const uint32_t* __restrict__ key, int64_t count, uint32_t mask,
-                 uint32_t* __restrict__ out, uint32_t* __restrict__ state_out) {
-    __shared__ uint32_t ring[kMtRing];
+// One workgroup of 256 threads continues the stream from the 624-word window
+// `key`: out[i] = temper(x[624 + i]) & mask for i < count (the raw x[624 + i]
+// with RAW), and, when state_out is non-null, the window after the last word.
+template <bool RAW>
+__device__ __forceinline__ void mt_run(const uint32_t* __restrict__ key, int64_t count,
+                                       uint32_t mask, uint32_t* __restrict__ out,
+                                       uint32_t* __restrict__ state_out, uint32_t* ring) {
     const int j = (int)threadIdx.x;  // blockDim.x == 256 >= kMtStep
     for (int i = j; i < kMtN; i += 256) ring[i] = key[i];
     uint32_t c = j < kMtStep ? key[kMtN - kMtStep + j] : 0u;  // x[k - 227]
@@ -528,13 +532,169 @@ mt_stream_kernel(const uint32_t* __restrict__ key, int64_t count, uint32_t mask,
                 const uint32_t y = (a[h] & 0x80000000u) | (b[h] & 0x7fffffffu);
                 c = c ^ (y >> 1) ^ ((b[h] & 1u) ? 0x9908b0dfu : 0u);
                 ring[k & (kMtRing - 1)] = c;
-                if (k - kMtN < count) out[k - kMtN] = mt_temper(c) & mask;
+                if (k - kMtN < count) out[k - kMtN] = RAW ? c : (mt_temper(c) & mask);
             }
         }
         lds_barrier();
     }
     // the state after `count` draws: x[count .. count + 623], next draw a fresh twist
-    for (int i = j; i < kMtN; i += 256) state_out[i] = ring[(count + i) & (kMtRing - 1)];
+    if (state_out)
+        for (int i = j; i < kMtN; i += 256) state_out[i] = ring[(count + i) & (kMtRing - 1)];
+}
+
+__global__ void __launch_bounds__(256)
+mt_stream_kernel(const uint32_t* __restrict__ key, int64_t count, uint32_t mask,
+                 uint32_t* __restrict__ out, uint32_t* __restrict__ state_out) {
+    __shared__ uint32_t ring[kMtRing];
+    mt_run<false>(key, count, mask, out, state_out, ring);
+}
+
+// ---- jumping the stream ahead (mt_jump.h) -------------------------------------
+// A long fill is cut into S <= 512 segments of L = 2^e draws; segment s continues the
+// stream from the window W_{sL} = x[sL .. sL + 623].  The windows come from
+// W_0 (the key) by a tree of jumps: at level m (from the top down) every known
+// window W_{qL * 2^(m+1)} is jumped 2^m segments = 2^(e+m) words ahead to make
+// W_{(q * 2^(m+1) + 2^m) L}.  A jump of 2^k words is the GF(2) correlation
+//     W'[j] = XOR over the set bits i of t^(2^k) mod phi of x[i + j]
+// over the source's next kStreamWords words, which mt_source_kernel writes.
+constexpr int kMtStreamWords = mtj::kStreamWords;  // 20,560
+constexpr int kJumpChunk = 1024;                    // poly bits per workgroup
+constexpr int kJumpChunks = (mtj::kDeg + kJumpChunk - 1) / kJumpChunk;
+
+// Source q of a level (window q * src_step): its raw stream x[0 .. 20560),
+// and its target window (q * src_step + half) cleared for the jump's XORs.
+__global__ void __launch_bounds__(256)
+mt_source_kernel(uint32_t* __restrict__ windows, int src_step, int half,
+                 uint32_t* __restrict__ streams) {
+    __shared__ uint32_t ring[kMtRing];
+    const int64_t q = blockIdx.x;
+    const uint32_t* __restrict__ w = windows + q * src_step * kMtN;
+    uint32_t* __restrict__ x = streams + q * kMtStreamWords;
+    uint32_t* __restrict__ target = windows + (q * src_step + half) * kMtN;
+    for (int i = (int)threadIdx.x; i < kMtN; i += 256) {
+        x[i] = w[i];
+        target[i] = 0u;
+    }
+    mt_run<true>(w, kMtStreamWords - kMtN, ~0u, x + kMtN, nullptr, ring);
+}
+
+// The jump polynomial's set bits, ascending, and where each 4096-bit chunk's
+// bits start in the list.
+struct JumpBits {
+    const int32_t* bits;
+    int chunk_start[kJumpChunks + 1];
+};
+
+// Target q * src_step + half from source q: blockIdx.y covers window words
+// 256y .. 256y + 255 and blockIdx.z the polynomial's bits
+// [1024z, 1024z + 1024).  The stream words that chunk needs and its set bits
+// are staged in LDS; each set bit costs a lane one LDS read and one XOR, and
+// the chunk's partial window is XORed into the target (cleared by
+// mt_source_kernel), so the result does not depend on the order.
+__global__ void __launch_bounds__(256)
+mt_jump_kernel(const uint32_t* __restrict__ streams, JumpBits jb, uint32_t* __restrict__ windows,
+               int src_step, int half) {
+    __shared__ uint32_t xs[kJumpChunk + 256];
+    __shared__ int32_t ib[kJumpChunk];
+    const int t = (int)threadIdx.x;
+    const int64_t q = blockIdx.x;
+    const int j0 = (int)blockIdx.y * 256;
+    const int c = (int)blockIdx.z;
+    const int c0 = c * kJumpChunk;
+    const int p0 = jb.chunk_start[c];
+    const int n = jb.chunk_start[c + 1] - p0;  // <= kJumpChunk
+    if (n == 0) return;                        // workgroup-uniform
+    const uint32_t* __restrict__ x = streams + q * kMtStreamWords;
+    for (int u = t; u < kJumpChunk + 256; u += 256) {
+        const int idx = c0 + j0 + u;
+        xs[u] = idx < kMtStreamWords ? x[idx] : 0u;
+    }
+    for (int u = t; u < n; u += 256) ib[u] = jb.bits[p0 + u] - c0;
+    __syncthreads();
+    uint32_t acc = 0;
+    int u = 0;
+    for (; u + 4 <= n; u += 4)
+        acc ^= xs[ib[u] + t] ^ xs[ib[u + 1] + t] ^ xs[ib[u + 2] + t] ^ xs[ib[u + 3] + t];
+    for (; u < n; ++u) acc ^= xs[ib[u] + t];
+    const int j = j0 + t;
+    if (j < kMtN && acc) atomicXor(windows + (q * src_step + half) * kMtN + j, acc);
+}
+
+// Segment s: draws [sL, min(sL + L, count)) from window s; the last segment
+// also writes the state after the fill.
+__global__ void __launch_bounds__(256)
+mt_segment_kernel(const uint32_t* __restrict__ windows, int64_t seg, int64_t count, uint32_t mask,
+                  uint32_t* __restrict__ out, uint32_t* __restrict__ state_out) {
+    __shared__ uint32_t ring[kMtRing];
+    const int64_t s = blockIdx.x;
+    const int64_t first = s * seg;
+    const int64_t n = (count - first) < seg ? (count - first) : seg;
+    mt_run<false>(windows + s * kMtN, n, mask, out + first,
+                  first + seg >= count ? state_out : nullptr, ring);
+}
+
+// Fills up to this many draws run as one stream; longer ones are jumped.
+constexpr int64_t kMtJumpMin = int64_t(1) << 21;
+constexpr int kMtMaxSegments = 512;
+
+// The segmented fill on stream s; d_key holds the key (the window W_0).
+hipError_t mt_fill_jumped(galop_ctx* ctx, const uint32_t* d_key, int64_t count, uint32_t mask,
+                          uint32_t* d_out, uint32_t* d_state, hipStream_t s) {
+    int e = 15;
+    while ((int64_t(1) << e) * kMtMaxSegments < count) ++e;
+    const int64_t L = int64_t(1) << e;
+    const int S = (int)((count + L - 1) / L);  // 64 .. 512
+    int D = 0;
+    while ((1 << D) < S) ++D;
+    const mtj::Tables& T = mtj::tables(e + D - 1);
+    Staging& st = ctx->staging;
+    // the set-bit lists of t^(2^k) mod phi, k = 0 .. e + D - 1, once per context
+    if (st.mt_poly_k < e + D - 1) {
+        const int kmax = e + D - 1;
+        std::vector<int32_t> all;
+        std::vector<int64_t> off(1, 0);
+        for (int k = 0; k <= kmax; ++k) {
+            all.insert(all.end(), T.pow2_bits[(size_t)k].begin(), T.pow2_bits[(size_t)k].end());
+            off.push_back((int64_t)all.size());
+        }
+        hipError_t e2 = st.d_mt_poly.reserve(all.size());
+        if (e2 == hipSuccess)
+            e2 = hipMemcpy(st.d_mt_poly.ptr, all.data(), all.size() * sizeof(int32_t),
+                           hipMemcpyHostToDevice);
+        if (e2 != hipSuccess) {
+            st.mt_poly_k = -1;
+            return e2;
+        }
+        st.mt_poly_off = off;
+        st.mt_poly_k = kmax;
+    }
+    const int qmax = (S + 1) / 2;
+    hipError_t err = st.d_mt.reserve((size_t)S * kMtN + (size_t)qmax * kMtStreamWords);
+    if (err != hipSuccess) return err;
+    uint32_t* windows = st.d_mt.ptr;
+    uint32_t* streams = windows + (size_t)S * kMtN;
+    err = hipMemcpyAsync(windows, d_key, kMtN * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    if (err != hipSuccess) return err;
+    for (int m = D - 1; m >= 0; --m) {
+        const int half = 1 << m, step = 2 << m;
+        const int Q = S > half ? (S - half + step - 1) / step : 0;
+        if (Q == 0) continue;
+        hipLaunchKernelGGL(mt_source_kernel, dim3((unsigned)Q), dim3(256), 0, s, windows, step,
+                           half, streams);
+        const int k = e + m;
+        const std::vector<int32_t>& bits = T.pow2_bits[(size_t)k];
+        JumpBits jb;
+        jb.bits = st.d_mt_poly.ptr + st.mt_poly_off[(size_t)k];
+        for (int c = 0, p = 0; c <= kJumpChunks; ++c) {
+            while (p < (int)bits.size() && bits[(size_t)p] < c * kJumpChunk) ++p;
+            jb.chunk_start[c] = c == kJumpChunks ? (int)bits.size() : p;
+        }
+        hipLaunchKernelGGL(mt_jump_kernel, dim3((unsigned)Q, (kMtN + 255) / 256, kJumpChunks),
+                           dim3(256), 0, s, streams, jb, windows, step, half);
+    }
+    hipLaunchKernelGGL(mt_segment_kernel, dim3((unsigned)S), dim3(256), 0, s, windows, L, count,
+                       mask, d_out, d_state);
+    return hipGetLastError();
 }
 
 }  // namespace
@@ -615,7 +775,9 @@ int galop_mt19937_fill_device(galop_ctx* ctx, const uint32_t* key, int64_t count
     uint32_t* d_state = d_key + kMtN;
     hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
     e = hipMemcpyAsync(d_key, key, kMtN * sizeof(uint32_t), hipMemcpyHostToDevice, s);
-    if (e == hipSuccess) {
+    if (e == hipSuccess && count >= kMtJumpMin) {
+        e = mt_fill_jumped(ctx, d_key, count, mask, d_out, d_state, s);
+    } else if (e == hipSuccess) {
         hipLaunchKernelGGL(mt_stream_kernel, dim3(1), dim3(256), 0, s, d_key, count, mask, d_out,
                            d_state);
         e = hipGetLastError();

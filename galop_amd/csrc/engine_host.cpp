@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "lgp_engine.h"
+#include "mt_jump.h"
 
 namespace ge = galop_engine;
 
@@ -75,6 +76,23 @@ void gv_stats(const float* fit, const uint8_t* len, int64_t P, int32_t generatio
     out[3] = len[best_idx];
     out[4] = (float)((double)len_sum / (double)P);
     out[5] = best_idx;
+}
+
+// std::mt19937 jumped ahead on the host (mt_jump.h), the checker of the
+// device fill: the window 2^k words after the window w (0, or -1 for k
+// outside 0..255).
+int32_t gv_mt_jump_pow2(const uint32_t* w, int32_t k, uint32_t* out) {
+    return mtj::jump_pow2(w, k, out) ? 0 : -1;
+}
+
+// The characteristic polynomial phi of std::mt19937 found by
+// Berlekamp-Massey: coefficients of t^0 .. t^19937, one byte each; returns
+// the degree.
+int32_t gv_mt_charpoly(uint8_t* out) {
+    const mtj::Tables& t = mtj::tables(0);
+    if (t.phi.empty()) return -1;
+    for (int i = 0; i <= mtj::kDeg; ++i) out[i] = (uint8_t)mtj::bit(t.phi, i);
+    return mtj::kDeg;
 }
 
 }  // extern "C"

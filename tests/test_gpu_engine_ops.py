@@ -94,13 +94,16 @@ def test_device_stats_match_reference(gpu_ctx):
     assert np.array_equal(np.array(got, np.float64), f["final_stats"])
 
 
-@pytest.mark.parametrize("count", [0, 1, 226, 227, 228, 623, 624, 625, 1_000_003])
+@pytest.mark.parametrize("count", [0, 1, 226, 227, 228, 623, 624, 625, 1_000_003,
+                                   2**21, 2**21 + 1, 3 * 2**23 + 17, 50_000_000])
 @pytest.mark.parametrize("seed,mask", [(42, 0xFFFFFFFF), (7, 0x87070707)])
 def test_device_mt19937_stream(gpu_ctx, count, seed, mask):
     """galop_mt19937_fill_device: the draws of std::mt19937 (numpy's MT19937
     with legacy seeding) after seeding, ANDed with the mask, and a returned
     state from which the host generator continues as if it had made the draws
-    itself."""
+    itself.  From 2^21 draws on the fill is cut into up to 512 segments whose
+    start windows come from a tree of jumps (64 to 385 segments here; the
+    last one partial except at 2^21)."""
     import torch
     from galop_amd.engine import mt19937_fill_device
     from galop_amd.evolve import host_generator
