@@ -319,6 +319,91 @@ vary_pairs_wave_kernel(ge::VaryCfg c, const uint32_t* __restrict__ cur_instr,
     }
 }
 
+// A child plan of lane j applied to the wave-held child p (crossover done);
+// returns the final length.
+template <int NW>
+__device__ __forceinline__ int wave_apply_child(WaveProg<NW>& p, const ge::ChildPlan& cp, int j,
+                                                uint32_t rmask, int lane) {
+    const int mi = __builtin_amdgcn_readlane(cp.micro_idx, j);
+    if (mi >= 0) {
+        const int field = __builtin_amdgcn_readlane(cp.micro_field, j);
+        const uint32_t r = (uint32_t)__builtin_amdgcn_readlane((int)cp.micro_r, j);
+        p.set(mi, ge::micro_apply(p.get(mi), field, r, rmask), lane);
+    }
+    const int nins = __builtin_amdgcn_readlane(cp.n_ins, j);
+#pragma unroll
+    for (int t = 0; t < ge::kPlanInserts; ++t) {
+        if (t < nins) {
+            const int pos = __builtin_amdgcn_readlane(cp.ins_pos[t], j);
+            const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)cp.ins_word[t], j);
+            p.insert_at(pos, w, lane);
+        }
+    }
+    const int len = __builtin_amdgcn_readlane(cp.len, j);
+    const int del = __builtin_amdgcn_readlane(cp.del_pos, j);
+    if (del >= 0) p.remove_at(del, len + 1, lane);
+    return len;
+}
+
+// Plannable configurations (ge::plannable): lane l makes every draw of pair
+// 64w + l (ge::plan_pair) on the vector unit, so the 64 PCG32 streams of a
+// wave run side by side instead of one after another on the scalar unit;
+// then the wave moves the words of its pairs one pair at a time, reading
+// each plan with v_readlane, as vary_pairs_wave_kernel does.
+template <int NW>
+__global__ void __launch_bounds__(256)
+vary_planned_kernel(ge::VaryCfg c, const uint32_t* __restrict__ cur_instr,
+                    const uint8_t* __restrict__ cur_len, const float* __restrict__ cur_fit,
+                    uint32_t* __restrict__ next_instr, uint8_t* __restrict__ next_len,
+                    float* __restrict__ next_fit, uint64_t base, int64_t n_pairs) {
+    const int lane = (int)(threadIdx.x & 63);
+    const int64_t w0 = ((int64_t)blockIdx.x * (blockDim.x >> 6) +
+                        __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6))) * 64;
+    if (w0 >= n_pairs) return;  // wave-uniform
+    ge::PairPlan pl;
+    if (w0 + lane < n_pairs) {
+        ge::plan_pair(w0 + lane, c, cur_len, cur_fit, base, pl);
+    } else {  // past the last pair: an empty plan, never applied
+        ge::Pcg32 none = ge::pcg32_make(0u, 0u);
+        pl.pa = pl.pb = pl.la = pl.lb = pl.cut = 0;
+        ge::plan_mutate(0, c, none, pl.a);  // length 0: no draws
+        ge::plan_mutate(0, c, none, pl.b);
+    }
+    const uint32_t rmask = (uint32_t)c.num_registers - 1u;
+    const int n = (int)((n_pairs - w0) < 64 ? (n_pairs - w0) : 64);
+    for (int j = 0; j < n; ++j) {
+        const int pa = __builtin_amdgcn_readlane(pl.pa, j);
+        const int pb = __builtin_amdgcn_readlane(pl.pb, j);
+        const int la = __builtin_amdgcn_readlane(pl.la, j);
+        const int lb = __builtin_amdgcn_readlane(pl.lb, j);
+        const int cut = __builtin_amdgcn_readlane(pl.cut, j);
+        WaveProg<NW> a, b;
+        a.load(cur_instr + (int64_t)pa * c.stride, la, lane);
+        b.load(cur_instr + (int64_t)pb * c.stride, lb, lane);
+        if (cut) {
+            // child A = a's prefix + b's suffix (length lb); child B the other way
+#pragma unroll
+            for (int q = 0; q < NW; ++q) {
+                const bool pre = 64 * q + lane < cut;
+                const uint32_t x = a.w[q], y = b.w[q];
+                a.w[q] = pre ? x : y;
+                b.w[q] = pre ? y : x;
+            }
+        }
+        const int len_a = wave_apply_child<NW>(a, pl.a, j, rmask, lane);
+        const int len_b = wave_apply_child<NW>(b, pl.b, j, rmask, lane);
+        const int64_t da = c.elite_count + 2 * (w0 + j), db = da + 1;
+        a.store(next_instr + da * c.stride, len_a, lane);
+        b.store(next_instr + db * c.stride, len_b, lane);
+        if (lane == 0) {
+            next_len[da] = (uint8_t)len_a;
+            next_len[db] = (uint8_t)len_b;
+            next_fit[da] = ge::bits_to_float(0x7fc00000u);
+            next_fit[db] = ge::bits_to_float(0x7fc00000u);
+        }
+    }
+}
+
 __global__ void init_kernel(ge::VaryCfg c, uint64_t base, uint32_t* __restrict__ instr,
                             uint8_t* __restrict__ len, float* __restrict__ fit) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -743,13 +828,22 @@ int galop_vary_device(galop_ctx* ctx, const galop_vary_cfg* cfg, const uint32_t*
     }
     const int64_t n_pairs = (P - cfg->elite_count) / 2;
     if (n_pairs > 0) {
-        // one pair per wave, four waves per workgroup; registers per lane for
-        // the longest program: ceil(stride / 64)
-        const dim3 grid((unsigned)((n_pairs + 3) / 4)), block(256);
+        // four waves per workgroup; registers per lane for the longest
+        // program: ceil(stride / 64)
+        // plannable configurations: 64 pairs per wave (drawn lane by lane);
+        // otherwise one pair per wave
+        const bool planned = ge::plannable(c);
+        const dim3 grid((unsigned)(planned ? (n_pairs + 255) / 256 : (n_pairs + 3) / 4));
+        const dim3 block(256);
 #define GALOP_VARY(NW)                                                                        \
-        hipLaunchKernelGGL(vary_pairs_wave_kernel<NW>, grid, block, 0, s, c, d_cur_instr,       \
-                           d_cur_len, d_cur_fit, d_next_instr, d_next_len, d_next_fit,          \
-                           stream_base, n_pairs)
+        if (planned)                                                                            \
+            hipLaunchKernelGGL(vary_planned_kernel<NW>, grid, block, 0, s, c, d_cur_instr,      \
+                               d_cur_len, d_cur_fit, d_next_instr, d_next_len, d_next_fit,      \
+                               stream_base, n_pairs);                                           \
+        else                                                                                    \
+            hipLaunchKernelGGL(vary_pairs_wave_kernel<NW>, grid, block, 0, s, c, d_cur_instr,   \
+                               d_cur_len, d_cur_fit, d_next_instr, d_next_len, d_next_fit,      \
+                               stream_base, n_pairs)
         const int nw = (cfg->stride + 63) / 64;
         if (nw == 1) GALOP_VARY(1);
         else if (nw == 2) GALOP_VARY(2);

@@ -34,6 +34,51 @@ void gv_vary(const ge::VaryCfg* c, const uint32_t* cur_instr, const uint8_t* cur
                       stream_base);
 }
 
+// The pairs of gv_vary in planned form (plannable configurations; returns
+// -1 otherwise): every pair's draws first (ge::plan_pair), then the words
+// moved as the plan says.  The elites are gv_vary's.
+int32_t gv_vary_planned(const ge::VaryCfg* c, const uint32_t* cur_instr, const uint8_t* cur_len,
+                        const float* cur_fit, uint32_t* next_instr, uint8_t* next_len,
+                        float* next_fit, uint64_t stream_base) {
+    if (!ge::plannable(*c)) return -1;
+    const int64_t n_pairs = (c->population - c->elite_count) / 2;
+    std::vector<ge::PairPlan> plans((size_t)n_pairs);
+    for (int64_t i = 0; i < n_pairs; ++i)
+        ge::plan_pair(i, *c, cur_len, cur_fit, stream_base, plans[(size_t)i]);
+    const uint32_t rmask = (uint32_t)c->num_registers - 1u;
+    std::vector<uint32_t> w((size_t)c->stride + ge::kPlanInserts + 1);
+    const auto child = [&](const ge::PairPlan& pl, bool first, const ge::ChildPlan& cp,
+                           int64_t dst) {
+        const uint32_t* a = cur_instr + (int64_t)(first ? pl.pa : pl.pb) * c->stride;
+        const uint32_t* b = cur_instr + (int64_t)(first ? pl.pb : pl.pa) * c->stride;
+        const int n = pl.cut ? (first ? pl.lb : pl.la) : (first ? pl.la : pl.lb);
+        for (int k = 0; k < n; ++k) w[(size_t)k] = (pl.cut && k >= pl.cut) ? b[k] : a[k];
+        int len = n;
+        if (cp.micro_idx >= 0)
+            w[(size_t)cp.micro_idx] =
+                ge::micro_apply(w[(size_t)cp.micro_idx], cp.micro_field, cp.micro_r, rmask);
+        for (int t = 0; t < cp.n_ins; ++t) {
+            for (int k = len; k > cp.ins_pos[t]; --k) w[(size_t)k] = w[(size_t)k - 1];
+            w[(size_t)cp.ins_pos[t]] = cp.ins_word[t];
+            ++len;
+        }
+        if (cp.del_pos >= 0) {
+            for (int k = cp.del_pos; k < len - 1; ++k) w[(size_t)k] = w[(size_t)k + 1];
+            --len;
+        }
+        for (int k = 0; k < len; ++k) next_instr[dst * c->stride + k] = w[(size_t)k];
+        next_len[dst] = (uint8_t)len;
+        next_fit[dst] = ge::bits_to_float(0x7fc00000u);
+    };
+    for (int64_t i = 0; i < n_pairs; ++i) {
+        const ge::PairPlan& pl = plans[(size_t)i];
+        const int64_t da = c->elite_count + 2 * i;
+        child(pl, true, pl.a, da);
+        child(pl, false, pl.b, da + 1);
+    }
+    return 0;
+}
+
 void gv_init(const ge::VaryCfg* c, uint64_t base, uint32_t* instr, uint8_t* len, float* fit) {
     for (int64_t i = 0; i < c->population; ++i) {
         ge::init_program(i, *c, base, instr);

@@ -268,6 +268,109 @@ EN_HD void vary_pair(int64_t i, const VaryCfg& c, const uint32_t* cur_instr,
     next_fit[db] = bits_to_float(0x7fc00000u);
 }
 
+// ---- the variation of a pair as a plan --------------------------------------------
+// Without effective mutation and per-instruction micro mutation, no draw of
+// vary_pair depends on a program word: the tournaments read fitness, the
+// ranges depend on lengths only, and micro_mutate draws its field and value
+// before it looks at the word.  So the draws can be made first, into a plan,
+// and the words moved afterwards; plan_pair makes exactly vary_pair's draws
+// in vary_pair's order.
+constexpr int kPlanInserts = 4;  // macro_inserts a plan can hold
+
+EN_HD bool plannable(const VaryCfg& c) {
+    return !c.effmut && !(c.micro_per_instr > 0.0f) && c.macro_inserts <= kPlanInserts;
+}
+
+struct ChildPlan {
+    int32_t len;          // final length
+    int32_t micro_idx;    // word micro-mutated after crossover, -1 for none
+    int32_t micro_field;  // micro_mutate's field draw
+    uint32_t micro_r;     // and its value draw
+    int32_t n_ins;        // inserts made, in order
+    int32_t ins_pos[kPlanInserts];
+    uint32_t ins_word[kPlanInserts];
+    int32_t del_pos;      // word deleted last, -1 for none
+};
+
+struct PairPlan {
+    int32_t pa, pb;  // parents
+    int32_t la, lb;  // their lengths
+    int32_t cut;     // crossover cut; 0 = clone
+    ChildPlan a, b;
+};
+
+// micro_mutate's effect given its draws
+EN_HD uint32_t micro_apply(uint32_t instr, int field, uint32_t r, uint32_t rmask) {
+    switch (field) {
+    case 0: return (instr & ~7u) | (r & 7u);
+    case 1: return (instr & ~(rmask << 8)) | ((r & rmask) << 8);
+    case 2: return (instr & ~(rmask << 16)) | ((r & rmask) << 16);
+    case 3: return (instr & ~(rmask << 24)) | ((r & rmask) << 24);
+    default: return instr ^ 0x80000000u;
+    }
+}
+
+// mutate's draws for a child of length len (plannable configurations)
+EN_HD void plan_mutate(int len, const VaryCfg& c, Pcg32& g, ChildPlan& p) {
+    const uint32_t rmask = (uint32_t)c.num_registers - 1u;
+    p.micro_idx = -1;
+    p.micro_field = 0;
+    p.micro_r = 0;
+    p.n_ins = 0;
+    p.del_pos = -1;
+    for (int t = 0; t < kPlanInserts; ++t) {
+        p.ins_pos[t] = 0;
+        p.ins_word[t] = 0;
+    }
+    p.len = len;
+    if (len == 0) return;
+    if (uniform01(g) < c.micro_rate) {
+        p.micro_idx = uniform_int(g, 0, len - 1);
+        p.micro_field = uniform_int(g, 0, 4);
+        p.micro_r = g.next();
+    }
+    for (int ins = 0; ins < c.macro_inserts; ++ins) {
+        if (uniform01(g) < c.insert_rate) {
+            if (len < c.stride) {
+                const int pos = c.macro_random_pos ? uniform_int(g, 0, len) : len;
+                const uint32_t word = encode_from_random(g.next(), rmask);
+                // slot n_ins, written by selects: a per-lane index into the
+                // arrays would put them in memory on the device
+                for (int t = 0; t < kPlanInserts; ++t) {
+                    if (t == p.n_ins) {
+                        p.ins_pos[t] = pos;
+                        p.ins_word[t] = word;
+                    }
+                }
+                ++p.n_ins;
+                ++len;
+            }
+        }
+    }
+    if (uniform01(g) < c.delete_rate) {
+        if (len > 1) {
+            p.del_pos = c.macro_random_pos ? uniform_int(g, 0, len - 1) : len - 1;
+            --len;
+        }
+    }
+    p.len = len;
+}
+
+// vary_pair's draws for pair i
+EN_HD void plan_pair(int64_t i, const VaryCfg& c, const uint8_t* cur_len, const float* cur_fit,
+                     uint64_t stream_base, PairPlan& p) {
+    Pcg32 g = pcg32_make(stream_base, (uint64_t)i);
+    p.pa = tournament(cur_fit, c, g);
+    p.pb = tournament(cur_fit, c, g);
+    p.la = cur_len[p.pa];
+    p.lb = cur_len[p.pb];
+    const int min_len = p.la < p.lb ? p.la : p.lb;
+    p.cut = 0;
+    if ((uniform01(g) < c.crossover_rate) && min_len >= 2) p.cut = uniform_int(g, 1, min_len - 1);
+    plan_mutate(p.cut ? p.lb : p.la, c, g, p.a);
+    plan_mutate(p.cut ? p.la : p.lb, c, g, p.b);
+}
+
 // Program i of the per-program PCG32 initializer (src/LGPEngine.cpp:52-67):
 // every word of the slot from stream (base, i).
 EN_HD void init_program(int64_t i, const VaryCfg& c, uint64_t base, uint32_t* instr) {

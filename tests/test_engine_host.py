@@ -126,3 +126,72 @@ def test_stats_match_reference():
     L.gv_stats(fit.ctypes.data, lens.ctypes.data, fit.size, int(f["final_stats"][0]),
                out.ctypes.data)
     assert np.array_equal(out, f["final_stats"])
+
+
+PLANNED_ARMS = [
+    # (stride, tournament, registers, input protect, inserts, random positions, rates)
+    (50, 3, 8, 1, 3, 1, (1.0, 0.8, 0.0, 1.0, 0.8, 0.75)),   # the standard configuration
+    (64, 4, 16, 0, 1, 1, (0.9, 0.8, 0.0, 1.0, 0.6, 0.5)),
+    (130, 2, 8, 0, 4, 1, (0.5, 0.5, 0.0, 1.0, 0.95, 0.9)),
+    (255, 3, 16, 1, 2, 0, (0.9, 1.0, 0.0, 1.0, 0.9, 0.9)),   # tail-only insertion/deletion
+    (7, 3, 8, 0, 3, 1, (1.0, 0.8, 0.0, 1.0, 0.9, 0.1)),     # programs at the stride
+]
+
+
+@pytest.mark.parametrize("arm", range(len(PLANNED_ARMS)))
+def test_planned_variation_equals_vary_pair(arm):
+    """Without effective mutation and per-instruction micro mutation every
+    draw of vary_pair depends on fitness and lengths only: making all draws
+    first (plan_pair) and moving the words afterwards must give vary_pair's
+    generation word for word, including parents of length 0 and 1."""
+    from galop_amd._lib import galop_vary_cfg
+    L = _host()
+    vp = ctypes.c_void_p
+    L.gv_vary_planned.restype = ctypes.c_int32
+    L.gv_vary_planned.argtypes = [vp, vp, vp, vp, vp, vp, vp, ctypes.c_uint64]
+    S, T, nr, ip, ins, rpos, rates = PLANNED_ARMS[arm]
+    rng = np.random.default_rng(100 + arm)
+    c = galop_vary_cfg()
+    c.population, c.stride, c.elite_count, c.tournament_size = 6000, S, 600, T
+    c.num_registers, c.input_protect, c.op_set, c.effmut = nr, ip, 0, 0
+    c.macro_inserts, c.macro_random_pos, c.starting_length, c.reserved = ins, rpos, 8, 0
+    (c.crossover_rate, c.micro_rate, c.micro_per_instr, c.effmut_rate, c.insert_rate,
+     c.delete_rate) = rates
+    n = c.population
+    lens = rng.integers(0, S + 1, n).astype(np.uint8)
+    lens[::53] = 1
+    lens[::71] = 0
+    cur = rng.integers(0, 2**32, (n, S), dtype=np.uint64).astype(np.uint32)
+    fit = rng.random(n).astype(np.float32)
+    out = []
+    for fn in (L.gv_vary, L.gv_vary_planned):
+        nxt = np.zeros((n, S), np.uint32)
+        nlen = np.zeros(n, np.uint8)
+        nfit = np.zeros(n, np.float32)
+        r = fn(ctypes.byref(c), cur.ctypes.data, lens.ctypes.data, fit.ctypes.data,
+               nxt.ctypes.data, nlen.ctypes.data, nfit.ctypes.data, 0x5EED0000 + arm)
+        assert r is None or r == 0
+        out.append((nxt, nlen, nfit))
+    (a, al, af), (b, bl, bf) = out
+    E, n_pairs = c.elite_count, (n - c.elite_count) // 2
+    sl = slice(E, E + 2 * n_pairs)
+    assert np.array_equal(al[sl], bl[sl])
+    assert np.array_equal(csr_from_dense(a[sl], al[sl]), csr_from_dense(b[sl], bl[sl]))
+    assert np.isnan(bf[sl]).all()
+    # the arms exercise every operator: some children changed length both ways
+    parents = lens[sl]
+    assert (al[sl] > 0).any() and len(set(al[sl].tolist())) > 3 and parents.size
+
+
+def test_planned_variation_refuses_unplannable():
+    from galop_amd._lib import galop_vary_cfg
+    L = _host()
+    vp = ctypes.c_void_p
+    L.gv_vary_planned.restype = ctypes.c_int32
+    L.gv_vary_planned.argtypes = [vp, vp, vp, vp, vp, vp, vp, ctypes.c_uint64]
+    c = galop_vary_cfg()
+    c.population, c.stride, c.elite_count, c.tournament_size = 10, 8, 2, 2
+    c.num_registers, c.effmut, c.macro_inserts = 8, 1, 1
+    z = np.zeros(80, np.uint32)
+    assert L.gv_vary_planned(ctypes.byref(c), z.ctypes.data, z.ctypes.data, z.ctypes.data,
+                             z.ctypes.data, z.ctypes.data, z.ctypes.data, 1) == -1

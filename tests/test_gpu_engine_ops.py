@@ -169,26 +169,38 @@ def test_device_stats_equal_host_build(gpu_ctx, kind, P):
     (20_000, 130, 2_000, "per_instr"),    # three registers, per-instruction micro mutation
     (10_000, 255, 500, "tail"),           # four registers, tail-only insertion/deletion
     (10_000, 200, 0, "effmut"),           # no elites
+    (200_000, 50, 20_000, "planned"),     # no effective mutation: draws made lane by lane
+    (30_000, 130, 3_000, "planned"),      # three registers per lane
+    (20_000, 255, 2_000, "planned_tail"), # four registers, tail-only, four insert attempts
+    (10_002, 7, 2, "planned"),            # programs at the stride, a partial last wave
 ])
 def test_large_population_two_generations_device_equals_host(gpu_ctx, P, S, E, variant):
     """Two chained generations on the device equal the host build on every
     length, word and fitness, for each register count per lane (stride up to
-    255) and each variation arm."""
+    255) and each variation arm, both of the wave-per-pair kernel and of the
+    planned kernel (configurations without effective mutation)."""
     from galop_amd._lib import galop_vary_cfg
     L = _host()
     rng = np.random.default_rng(S)
     c = galop_vary_cfg()
     c.population, c.stride, c.elite_count, c.tournament_size = P, S, E, 4
     c.num_registers, c.input_protect, c.op_set, c.effmut = 16, 1, 1, 1
-    c.macro_inserts, c.macro_random_pos, c.starting_length, c.reserved = 2, 1, 8, 0
+    c.macro_inserts, c.macro_random_pos, c.starting_length, c.reserved = 2, 1, min(8, S), 0
     (c.crossover_rate, c.micro_rate, c.micro_per_instr, c.effmut_rate, c.insert_rate,
      c.delete_rate) = (0.9, 0.8, 0.0, 0.7, 0.6, 0.5)
     if variant == "per_instr":
         c.effmut, c.micro_per_instr, c.num_registers, c.input_protect = 0, 0.05, 8, 0
     if variant == "tail":
         c.macro_random_pos, c.macro_inserts, c.insert_rate = 0, 3, 0.9
+    if variant.startswith("planned"):
+        c.effmut, c.macro_inserts, c.num_registers, c.input_protect = 0, 3, 8, 1
+    if variant == "planned_tail":
+        c.macro_random_pos, c.macro_inserts, c.insert_rate = 0, 4, 0.9
     n, S = c.population, c.stride
     lens = rng.integers(1, S + 1, n).astype(np.uint8)
+    if variant.startswith("planned"):
+        lens[::61] = 0  # empty and one-word parents: clones, no mutation draws
+        lens[::67] = 1
     cur = rng.integers(0, 2**32, (n, S), dtype=np.uint64).astype(np.uint32)
     fit = rng.random(n).astype(np.float32)
     fit[rng.integers(0, n, 5000)] = fit[0]  # ties: the index breaks them
