@@ -296,16 +296,17 @@ __device__ __forceinline__ void run_program(const uint32_t* __restrict__ prog,
 // A program of at most kSkipMaxWords words is held in kSkipChunks chunk
 // registers (word 64c + l in lane l of chunk c).  One backward pass over the
 // words on the scalar unit marks those whose result can reach the output
-// register (bit j of mask c for word 64c + j); the case tiles then run only the
-// marked words.  An unmarked word writes a register that is overwritten before
-// any marked word or the output reads it, so every case's output -- and the
-// fitness -- is bitwise that of the whole program.
+// register; each chunk's marked words are then moved, in order, to the front
+// of its register, and the case tiles run the first n_c words of each chunk.
+// An unmarked word writes a register that is overwritten before any marked
+// word or the output reads it, so every case's output -- and the fitness --
+// is bitwise that of the whole program.
 constexpr int kSkipChunks = 4;
 constexpr int64_t kSkipMaxWords = kSkipChunks * kWave;
 
 struct EffProgram {
-    uint32_t ch0, ch1, ch2, ch3;  // chunk registers
-    uint64_t m0, m1, m2, m3;      // effective-word masks (wave-uniform)
+    uint32_t ch0, ch1, ch2, ch3;  // chunk registers: the words to run first
+    int n0, n1, n2, n3;           // how many words of each chunk run (wave-uniform)
     int nchunks;
 };
 
@@ -335,16 +336,27 @@ __device__ __forceinline__ uint64_t mark_chunk(uint32_t ch, int cnt, int out_reg
     const uint32_t bits = live_bits<NR, EXT>(ch, out_reg, iprot);
     uint64_t below = cnt >= kWave ? ~0ull : ((1ull << cnt) - 1ull);  // words not yet passed
     uint64_t m = 0;
-    for (;;) {
-        const uint64_t hits = __ballot((bits & live) != 0u) & below;
-        if (!hits) break;
+    uint64_t hits = __ballot((bits & live) != 0u) & below;
+    while (hits) {
         const int i = 63 - __builtin_clzll(hits);
         const uint32_t x = wave_word(bits, i);
         live = (live & ~x) | (x >> 16);
-        m |= 1ull << i;
-        below = (1ull << i) - 1ull;
+        const uint64_t bit = 1ull << i;
+        m |= bit;
+        hits = __ballot((bits & live) != 0u) & (bit - 1ull);
     }
     return m;
+}
+
+// The words of `ch` whose bit is set in the wave-uniform mask m, moved in
+// order to lanes 0, 1, ... (lane l pushes its word to lane popcount(m below
+// l)); the other lanes push to lane 63, which receives a marked word only when
+// all 64 are marked.
+__device__ __forceinline__ uint32_t compact_chunk(uint32_t ch, uint64_t m, int lane) {
+    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+    const bool marked = (m >> lane) & 1ull;
+    return (uint32_t)__builtin_amdgcn_ds_permute((int)((marked ? below : 63u) << 2), (int)ch);
 }
 
 // Words [0, n) of `prog`, n <= kSkipMaxWords, into the chunk registers, every
@@ -356,30 +368,38 @@ __device__ __forceinline__ void load_window(const uint32_t* __restrict__ prog, i
     e.ch1 = kWave + lane < n ? prog[kWave + lane] : 0u;
     e.ch2 = 2 * kWave + lane < n ? prog[2 * kWave + lane] : 0u;
     e.ch3 = 3 * kWave + lane < n ? prog[3 * kWave + lane] : 0u;
-    const auto all = [&](int c) {
-        const int k = min(kWave, max(0, n - c * kWave));
-        return k == kWave ? ~0ull : ((1ull << k) - 1ull);
-    };
-    e.m0 = all(0);
-    e.m1 = all(1);
-    e.m2 = all(2);
-    e.m3 = all(3);
+    e.n0 = min(kWave, n);
+    e.n1 = min(kWave, max(0, n - kWave));
+    e.n2 = min(kWave, max(0, n - 2 * kWave));
+    e.n3 = max(0, n - 3 * kWave);
 }
 
 // Marks the effective words of the n <= kSkipMaxWords words already loaded
 // into e (load_window).
 template <int NR, int EXT>
-__device__ __forceinline__ void mark_window(EffProgram& e, int n, int out_reg, int iprot) {
+__device__ __forceinline__ void mark_window(EffProgram& e, int n, int out_reg, int iprot,
+                                            int lane) {
     uint32_t live = 1u << out_reg;
     const auto cnt = [&](int c) { return min(kWave, max(0, n - c * kWave)); };
     // chunks past the program hold no words and leave `live` as it is
-    e.m1 = 0;
-    e.m2 = 0;
-    e.m3 = 0;
-    if (n > 3 * kWave) e.m3 = mark_chunk<NR, EXT>(e.ch3, cnt(3), out_reg, iprot, live);
-    if (n > 2 * kWave) e.m2 = mark_chunk<NR, EXT>(e.ch2, cnt(2), out_reg, iprot, live);
-    if (n > kWave) e.m1 = mark_chunk<NR, EXT>(e.ch1, cnt(1), out_reg, iprot, live);
-    e.m0 = mark_chunk<NR, EXT>(e.ch0, cnt(0), out_reg, iprot, live);
+    if (n > 3 * kWave) {
+        const uint64_t m = mark_chunk<NR, EXT>(e.ch3, cnt(3), out_reg, iprot, live);
+        e.ch3 = compact_chunk(e.ch3, m, lane);
+        e.n3 = __popcll(m);
+    }
+    if (n > 2 * kWave) {
+        const uint64_t m = mark_chunk<NR, EXT>(e.ch2, cnt(2), out_reg, iprot, live);
+        e.ch2 = compact_chunk(e.ch2, m, lane);
+        e.n2 = __popcll(m);
+    }
+    if (n > kWave) {
+        const uint64_t m = mark_chunk<NR, EXT>(e.ch1, cnt(1), out_reg, iprot, live);
+        e.ch1 = compact_chunk(e.ch1, m, lane);
+        e.n1 = __popcll(m);
+    }
+    const uint64_t m = mark_chunk<NR, EXT>(e.ch0, cnt(0), out_reg, iprot, live);
+    e.ch0 = compact_chunk(e.ch0, m, lane);
+    e.n0 = __popcll(m);
 }
 
 template <int NR, int EXT>
@@ -387,30 +407,23 @@ __device__ __forceinline__ void mark_effective(const uint32_t* __restrict__ prog
                                                int lane, int out_reg, int iprot, EffProgram& e) {
     const int n = (int)len;  // <= kSkipMaxWords
     load_window(prog, n, lane, e);
-    mark_window<NR, EXT>(e, n, out_reg, iprot);
+    mark_window<NR, EXT>(e, n, out_reg, iprot, lane);
 }
 
-// The marked words in program order; one instance of the interpreter step.
-// The chunk in use is chosen by selects on register values (the fields are
-// copied first, so no select of their addresses can put them in memory).
+// The words to run, chunk by chunk in program order; one instance of the
+// interpreter step.  The chunk in use is chosen by selects on register values
+// (the fields are copied first, so no select of their addresses can put them
+// in memory).
 template <int NR, int K, int EXT>
 __device__ __forceinline__ void run_effective(const EffProgram& e, RegFile<NR, K>& r,
                                               ConstRegs cr, DestDecode dd) {
-    const uint32_t ch1 = e.ch1, ch2 = e.ch2, ch3 = e.ch3;
-    const uint64_t m1 = e.m1, m2 = e.m2, m3 = e.m3;
+    const uint32_t ch0 = e.ch0, ch1 = e.ch1, ch2 = e.ch2, ch3 = e.ch3;
+    const int n0 = e.n0, n1 = e.n1, n2 = e.n2, n3 = e.n3;
     const int nchunks = e.nchunks;
-    int c = 0;
-    uint64_t m = e.m0;
-    uint32_t ch = e.ch0;
-    for (;;) {
-        while (!m) {
-            if (++c >= nchunks) return;
-            m = c == 1 ? m1 : (c == 2 ? m2 : m3);
-            ch = c == 1 ? ch1 : (c == 2 ? ch2 : ch3);
-        }
-        const int j = (int)__builtin_ctzll(m);
-        m &= m - 1;
-        exec_word<NR, K, EXT>(wave_word(ch, j), r, cr, dd);
+    for (int c = 0; c < nchunks; ++c) {
+        const uint32_t ch = c == 0 ? ch0 : (c == 1 ? ch1 : (c == 2 ? ch2 : ch3));
+        const int n = c == 0 ? n0 : (c == 1 ? n1 : (c == 2 ? n2 : n3));
+        for (int j = 0; j < n; ++j) exec_word<NR, K, EXT>(wave_word(ch, j), r, cr, dd);
     }
 }
 
@@ -421,6 +434,16 @@ __device__ __forceinline__ void run_effective(const EffProgram& e, RegFile<NR, K
 template <int NR, int K>
 __device__ __forceinline__ void load_cases(const DevData& ds, int64_t first,
                                            int lane, RegFile<NR, K>& r) {
+    if constexpr (K == 1) {
+        // one case per lane: a short tile, loaded once per program, so only
+        // the dataset's own inputs are addressed (no per-register pointers
+        // or flags are kept for all NR registers)
+#pragma unroll
+        for (int i = 0; i < NR; ++i) r.set(0, i, 0.0f);
+        for (int i = 0; i < ds.n_inputs; ++i)
+            r.set(0, i, ds.inputs[(int64_t)i * ds.n_pad + first + lane]);
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
         if (i < ds.n_inputs) {
