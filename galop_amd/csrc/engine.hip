@@ -371,15 +371,19 @@ vary_planned_kernel(ge::VaryCfg c, const uint32_t* __restrict__ cur_instr,
     }
     const uint32_t rmask = (uint32_t)c.num_registers - 1u;
     const int n = (int)((n_pairs - w0) < 64 ? (n_pairs - w0) : 64);
-    for (int j = 0; j < n; ++j) {
+    // the parents of pair j + 1 are loaded while pair j is worked on
+    const auto load_parents = [&](int j, WaveProg<NW>& a, WaveProg<NW>& b) {
         const int pa = __builtin_amdgcn_readlane(pl.pa, j);
         const int pb = __builtin_amdgcn_readlane(pl.pb, j);
-        const int la = __builtin_amdgcn_readlane(pl.la, j);
-        const int lb = __builtin_amdgcn_readlane(pl.lb, j);
+        a.load(cur_instr + (int64_t)pa * c.stride, __builtin_amdgcn_readlane(pl.la, j), lane);
+        b.load(cur_instr + (int64_t)pb * c.stride, __builtin_amdgcn_readlane(pl.lb, j), lane);
+    };
+    WaveProg<NW> na, nb;
+    load_parents(0, na, nb);
+    for (int j = 0; j < n; ++j) {
+        WaveProg<NW> a = na, b = nb;
+        if (j + 1 < n) load_parents(j + 1, na, nb);
         const int cut = __builtin_amdgcn_readlane(pl.cut, j);
-        WaveProg<NW> a, b;
-        a.load(cur_instr + (int64_t)pa * c.stride, la, lane);
-        b.load(cur_instr + (int64_t)pb * c.stride, lb, lane);
         if (cut) {
             // child A = a's prefix + b's suffix (length lb); child B the other way
 #pragma unroll
