@@ -33,8 +33,12 @@ __device__ __forceinline__ void program_bounds(const int64_t* __restrict__ offse
 // The interpreter kernels ask for at least 4 waves per SIMD (at most 128
 // VGPRs).  At 8 cases per lane the unrolled FP64 evaluations of a unary word
 // otherwise take 130-140 VGPRs and 3 waves; 4 waves with a few spilled
-// dwords measured 15% faster on the plain workload.
-#define GALOP_INTERP_BOUNDS __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4)))
+// dwords measured 15% faster on the plain workload.  With 8 registers and at
+// most 4 cases per lane they ask for 5 (at most 102 VGPRs), which the 4-case
+// kernels need a few more VGPRs than to reach on their own; with 16
+// registers 5 waves would spill hundreds of bytes.
+#define GALOP_INTERP_BOUNDS \
+    __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NR == 8 && K <= 4 ? 5 : 4)))
 
 // ---------------------------------------------------------------------------
 // plain kernel

@@ -14,7 +14,8 @@
 // from exact integer arithmetic and checked by tests/test_math_constants.py):
 //   exp  x = k ln2 + r, |r| <= ln2/2, ln2 split in three (Cody-Waite);
 //        exp(r) by its Taylor series to r^12; scale by 2^k.
-//   log  x = y 2^e with y in [sqrt(1/2), sqrt(2)); j = rint((y-1) 128) picks
+//   log  x = y 2^e with y in [S, 2S), S = sqrt(1/2) cut to the high word of
+//        its double (found by integer operations on that word); j = rint((y-1) 128) picks
 //        c_j = 128/(128+j) rounded to 24 bits, so r = y c_j - 1 is exact
 //        (|r| < 0.0056); log x = e ln2 + ln(1/c_j) + log1p(r), with ln(1/c_j)
 //        from a 91-entry table and log1p(r) by its series to r^8.  No
@@ -59,7 +60,8 @@ constexpr double LN2_0 = 0x1.62e42ff000000p-1;
 constexpr double LN2_1 = -0x1.718432a200000p-35;
 constexpr double LN2_2 = 0x1.3c7673007e5edp-69;
 constexpr double INV_LN2 = 0x1.71547652b82fep+0;
-constexpr double SQRT_HALF = 0x1.6a09e667f3bcdp-1;
+// log's split point: the high word of sqrt(1/2) (0x1.6a09e667f3bcdp-1)
+constexpr uint32_t kLogSplitHi = 0x3fe6a09eu;
 
 // 1/n!, n = 0..12
 constexpr double EXP_C[13] = {
@@ -256,8 +258,7 @@ enum CoefSlot : int {
     kPio32Scaled64 = 31,
     kLn2 = 32,        // LN2_0, LN2_1, LN2_2
     kInvLn2 = 35,
-    kSqrtHalf = 36,
-    kCoefCount = 37
+    kCoefCount = 36
 };
 #define GM_COEF_INIT                                                              \
     {EXP_C[0], EXP_C[1], EXP_C[2], EXP_C[3], EXP_C[4], EXP_C[5], EXP_C[6],        \
@@ -266,7 +267,7 @@ enum CoefSlot : int {
      LOG1P_C[6],                                                                  \
      SINT_C[0], SINT_C[1], SINT_C[2], COST_C[0], COST_C[1], COST_C[2], COST_C[3],  \
      PIO32_0, PIO32_1, PIO32_2, INV_PIO32, PIO32_SCALED64,                        \
-     LN2_0, LN2_1, LN2_2, INV_LN2, SQRT_HALF}
+     LN2_0, LN2_1, LN2_2, INV_LN2}
 
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const __attribute__((address_space(4))) double* CoefPtr;
@@ -294,6 +295,16 @@ GM_HD float u2f(uint32_t u) {
     union { float f; uint32_t u; } v;
     v.u = u;
     return v.f;
+}
+GM_HD uint64_t d2u(double x) {
+    union { double d; uint64_t u; } v;
+    v.d = x;
+    return v.u;
+}
+GM_HD double u2d(uint64_t u) {
+    union { double d; uint64_t u; } v;
+    v.u = u;
+    return v.d;
 }
 
 // ---- exp -------------------------------------------------------------------
@@ -323,13 +334,15 @@ GM_HD float expf_r(float x, CoefPtr C) {
 
 // ---- log of a positive, finite, non-zero float --------------------------
 GM_HD float logf_pos(float x, CoefPtr C) {
-    int e;
-    double y = frexp((double)x, &e);  // y in [0.5, 1), 24 significant bits
-    const bool low = y < C[kSqrtHalf];
-    y = low ? 2.0 * y : y;
-    e = low ? e - 1 : e;
-    const int j = (int)rint_d((y - 1.0) * 128.0) + 37;  // 0..90
-    const double r = fma_d(y, LOG_T[2 * j], -1.0);     // exact: c_j has 24 bits
+    // x = y 2^e with y in [S, 2S), S = 0.70710659... the double with the high
+    // word of sqrt(1/2) and a zero low word: four integer operations on the
+    // high word of (double)x, which is normal for every positive float
+    const uint64_t bits = d2u((double)x);
+    const int32_t e = (int32_t)((uint32_t)(bits >> 32) - kLogSplitHi) >> 20;
+    const double y = u2d(bits - ((uint64_t)(int64_t)e << 52));  // 24 significant bits
+    // rint((y - 1) 128) + 37 in 0..90; y * 128 is exact
+    const int j = (int)rint_d(y * 128.0) - 91;
+    const double r = fma_d(y, LOG_T[2 * j], -1.0);  // exact: c_j has 24 bits
     double p = C[kLog1pC + 6];
     for (int n = 5; n >= 0; --n) p = fma_d(p, r, C[kLog1pC + n]);
     const double q = fma_d(r * r, p, r);  // log1p(r)

@@ -95,8 +95,10 @@ def test_sincos_table_identities():
 
 def test_log_table_identities():
     """c_j has 24 bits and is 128/(128+j) to 2^-24; ln(1/c_j) agrees with the
-    host C library's log within 1 ulp; every float y in [sqrt(1/2), sqrt(2))
-    gets an index in range and |y c_j - 1| < 0.0056, with y c_j exact."""
+    host C library's log within 1 ulp; every float y in [S, 2S), S the double
+    with the high word of sqrt(1/2) and a zero low word (the range the
+    device's integer split gives), gets an index in range and
+    |y c_j - 1| < 0.0056, with y c_j exact."""
     import math
     from fractions import Fraction
     t = MC.constants()["LOG_T"]
@@ -113,12 +115,16 @@ def test_log_table_identities():
         assert abs(f * (128 + j) / 128 - 1) <= Fraction(1, 1 << 24)
         ref = -math.log(t[2 * i])
         assert abs(L[i] - ref) <= np.spacing(abs(ref)) + 1e-300, j
-    # every float y in [sqrt(1/2), sqrt(2)), as the device computes the index
-    lo_bits = np.float32(np.sqrt(0.5)).view(np.uint32)
-    hi_bits = np.float32(np.sqrt(2.0)).view(np.uint32)
+    # every float y in [S, 2S), as the device computes the index
+    S = np.array([0x3FE6A09E00000000], np.uint64).view(np.float64)[0]
+    assert S <= np.sqrt(0.5) < S * (1 + 2**-20)
+    lo_bits = np.float32(S).view(np.uint32)
+    hi_bits = np.float32(2 * S).view(np.uint32)
     y = np.arange(lo_bits - 2, hi_bits + 2, dtype=np.uint32).view(np.float32).astype(np.float64)
-    y = y[(y >= np.sqrt(0.5)) & (y < np.sqrt(2.0))]
-    j = np.rint((y - 1.0) * 128.0).astype(np.int64)
+    y = y[(y >= S) & (y < 2 * S)]
+    assert y.size == (1 << 23) + 1 or y.size == 1 << 23
+    j = np.rint(y * 128.0).astype(np.int64) - 128
+    assert np.array_equal(j, np.rint((y - 1.0) * 128.0).astype(np.int64))
     assert j.min() == lo and j.max() == hi
     cj = c[j - lo]
     r = y * cj - 1.0
