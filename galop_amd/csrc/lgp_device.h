@@ -74,6 +74,32 @@ __device__ __forceinline__ ConstRegs load_const_regs(const float* consts, int la
 __device__ __forceinline__ float op_div(float a, float b) {
     return (b != 0.0f) ? a / b : 1.0f;  // src/ISA.h:224 protected division
 }
+
+// Division by a wave-uniform constant b, finite and non-zero, for which the
+// reciprocal is worked out once per instruction word: r = 1/b to about
+// 2^-52 (one Newton step on the hardware reciprocal).  Per case
+// q0 = a r and q = q0 + r (a - b q0) are within about 2^-52 of a/b,
+// relative.  The exact quotient of two floats is never a rounding midpoint of
+// a float result and lies at least 2^-50 (relative) from every one -- a/b - m
+// is a non-zero integer over b times a power of two -- so q rounded to float
+// is the IEEE quotient a / b.  For a = 0, +-inf or NaN, q0 itself (exact,
+// with the sign of a / b) is the IEEE result.
+struct DivConst {
+    double b, r;
+};
+__device__ __forceinline__ DivConst div_const_setup(float b) {
+    DivConst dc;
+    dc.b = (double)b;
+    const double r0 = __builtin_amdgcn_rcp(dc.b);
+    dc.r = __builtin_fma(r0, __builtin_fma(-dc.b, r0, 1.0), r0);
+    return dc;
+}
+__device__ __forceinline__ float div_by_const(float a, DivConst dc) {
+    const double ad = (double)a;
+    const double q0 = ad * dc.r;
+    const double q = __builtin_fma(dc.r, __builtin_fma(-dc.b, q0, ad), q0);
+    return (float)((__builtin_isfinite(a) && a != 0.0f) ? q : q0);
+}
 // Unary ops on src2 (src/ISA.h:225-226 and the EXT8 table), evaluated in FP64
 // and rounded once (lgp_math.h).
 __device__ __forceinline__ float op_sin(float b, galop_math::CoefPtr C) {
@@ -253,11 +279,38 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
         }                                                                 \
     } while (0)
 
+// Protected division: a finite non-zero constant divisor takes div_by_const
+// (its reciprocal once per word), any other the IEEE division per case.
+#define GALOP_DIV()                                                       \
+    do {                                                                  \
+        float av[K], cv[K];                                               \
+        _Pragma("unroll") for (int k = 0; k < K; ++k) av[k] = r.get(k, s1);   \
+        if (kc) {                                                         \
+            const float b = wave_bcast(cr.pool, s2);                      \
+            if (b != 0.0f && __builtin_isfinite(b)) {                     \
+                const DivConst dc = div_const_setup(b);                   \
+                _Pragma("unroll") for (int k = 0; k < K; ++k)             \
+                    cv[k] = div_by_const(av[k], dc);                      \
+            } else {                                                      \
+                _Pragma("unroll") for (int k = 0; k < K; ++k)             \
+                    cv[k] = op_div(av[k], b);                             \
+            }                                                             \
+            _Pragma("unroll") for (int k = 0; k < K; ++k)                 \
+                __asm__("" : "+v"(cv[k]));                                \
+        } else {                                                          \
+            float bv[K];                                                  \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) bv[k] = r.get(k, s2); \
+            _Pragma("unroll") for (int k = 0; k < K; ++k)                 \
+                cv[k] = op_div(av[k], bv[k]);                             \
+        }                                                                 \
+        _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]);    \
+    } while (0)
+
     switch (op) {
     case 0: GALOP_BIN(a + b); break;
     case 1: GALOP_BIN(a - b); break;
     case 2: GALOP_BIN(a * b); break;
-    case 3: GALOP_BIN(op_div(a, b)); break;
+    case 3: GALOP_DIV(); break;
     case 4: GALOP_SINCOS(0, 0); break;
     case 5: GALOP_SINCOS(1, 1); break;
     case 6:
@@ -270,6 +323,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
         break;
     }
 #undef GALOP_BIN
+#undef GALOP_DIV
 #undef GALOP_UNARY
 #undef GALOP_SINCOS
 }
