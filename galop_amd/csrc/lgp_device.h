@@ -186,6 +186,10 @@ __device__ __forceinline__ DestDecode dest_decode(int nr, int iprot) {
     return dd;
 }
 
+#ifndef GALOP_ROW_GROUP
+#define GALOP_ROW_GROUP 2
+#endif
+
 template <int NR, int K, int EXT>
 __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                                           ConstRegs cr, DestDecode dd) {
@@ -194,6 +198,9 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
     const int s1 = (int)((w >> 16) & (uint32_t)(NR - 1));
     const int s2 = (int)((w >> 24) & (uint32_t)(NR - 1));
     const bool kc = (w >> 31) != 0u;
+    // sin, cos and log read a table row per case; the rows of this many cases
+    // are loaded before any of them is used (a power of two dividing K)
+    constexpr int kRowGroup = K < GALOP_ROW_GROUP ? K : GALOP_ROW_GROUP;
 
 // A constant operand stays one scalar value for all K cases: the compute is
 // written out in both branches, so it is never copied into K vector values.
@@ -272,8 +279,66 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                                 ? galop_math::sincos_ph(bv[k], WANT_COS, C) \
                                 : galop_math::sincos_cw(bv[k], WANT_COS, C); \
             } else {                                                      \
+                /* the table rows of kRowGroup cases are loaded before */ \
+                /* any of them is used */                                 \
+                _Pragma("unroll") for (int g = 0; g < K; g += kRowGroup) {  \
+                    double t[kRowGroup], ts[kRowGroup], tc[kRowGroup];    \
+                    _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) { \
+                        uint32_t j;                                       \
+                        t[i] = galop_math::sincos_reduce(bv[g + i], WANT_COS, j, C); \
+                        ts[i] = galop_math::SINCOS_T[2 * j];              \
+                        tc[i] = galop_math::SINCOS_T[2 * j + 1];          \
+                    }                                                     \
+                    _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) \
+                        cv[g + i] = galop_math::sincos_finish(bv[g + i], WANT_COS, t[i], \
+                                                              ts[i], tc[i], C); \
+                }                                                         \
+            }                                                             \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]); \
+        }                                                                 \
+    } while (0)
+
+// protected log (EXT8 opcode 7): like GALOP_UNARY, with the table rows of
+// kRowGroup cases loaded before any is used.  The arithmetic runs on |b| for
+// every case (log_reduce keeps the row in range for any bits), and 0,
+// infinities and NaN take their fixed results at the end, as in plogf_r.
+#define GALOP_PLOG()                                                      \
+    do {                                                                  \
+        if (kc) {                                                         \
+            const float v = wave_bcast(cr.table, 3 * 16 + s2);            \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, v);    \
+        } else {                                                          \
+            const galop_math::CoefPtr C = galop_math::coef_table();       \
+            float bv[K], cv[K];                                           \
+            bool sp = true;                                               \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
+                bv[k] = r.get(k, s2);                                     \
+                sp = sp && galop_math::plogf_special(bv[k]);              \
+            }                                                             \
+            if (__all(sp)) {                                              \
                 _Pragma("unroll") for (int k = 0; k < K; ++k)             \
-                    cv[k] = galop_math::sincos_cw(bv[k], WANT_COS, C);    \
+                    cv[k] = galop_math::plogf_special_value(bv[k]);       \
+            } else {                                                      \
+                _Pragma("unroll") for (int g = 0; g < K; g += kRowGroup) {  \
+                    double y[kRowGroup], tcj[kRowGroup], tln[kRowGroup];  \
+                    int32_t e[kRowGroup];                                 \
+                    _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) { \
+                        uint32_t row91;                                   \
+                        y[i] = galop_math::log_reduce(__builtin_fabsf(bv[g + i]), e[i], \
+                                                      row91, C);          \
+                        /* row91 - 91 is in 0..90: the mask only tells */ \
+                        /* the compiler so */                              \
+                        const uint32_t row = (row91 - 91u) & 127u;        \
+                        tcj[i] = galop_math::LOG_T[2 * row];              \
+                        tln[i] = galop_math::LOG_T[2 * row + 1];          \
+                    }                                                     \
+                    _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) { \
+                        const float b = bv[g + i];                        \
+                        const float v = galop_math::log_finish(y[i], e[i], tcj[i], tln[i], C); \
+                        cv[g + i] = galop_math::plogf_special(b)          \
+                                        ? galop_math::plogf_special_value(b) : v; \
+                    }                                                     \
+                }                                                         \
             }                                                             \
             _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]); \
         }                                                                 \
@@ -318,7 +383,11 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
         else GALOP_BIN((a < b) ? 1.0f : 0.0f);
         break;
     default:
+#ifdef XOLDPLOG
         if constexpr (EXT) GALOP_UNARY(3, op_plog);
+#else
+        if constexpr (EXT) GALOP_PLOG();
+#endif
         else GALOP_BIN((a > b) ? 1.0f : 0.0f);
         break;
     }
@@ -326,6 +395,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
 #undef GALOP_DIV
 #undef GALOP_UNARY
 #undef GALOP_SINCOS
+#undef GALOP_PLOG
 }
 
 // Run `len` words starting at `prog` on the K register vectors.  The next

@@ -25,14 +25,16 @@
 //        behind the prefetch of the next program chunk.)
 //   sin, cos
 //        x = k pi/32 + t, |t| <= pi/64: |x| < 2^25 by Cody-Waite with pi/32
-//        split in three; |x| >= 2^25 by Payne-Hanek -- the 24-bit mantissa
+//        split in three and k = rint(x 32/pi) rounded by adding 1.5 2^52
+//        (the low word of the sum is k mod 2^32); |x| >= 2^25 by Payne-Hanek -- the 24-bit mantissa
 //        times a 128-bit window of the binary expansion of 2/pi gives k mod 64
 //        and a 122-bit fraction.  Then sin x = S_j cos t + C_j sin t with
 //        (S_j, C_j) = (sin, cos)(j pi/32) from a 64-entry table, j = k mod 64
 //        (cos x is j + 16), and Taylor series of sin t to t^7 and cos t - 1
 //        to t^8.
-// Everything is IEEE double add/mul/fma/div, rint, ldexp, frexp and
-// conversions, so the same source compiled for the host gives the same bits.
+// Everything is IEEE double add/mul/fma, integer operations on the bits,
+// ldexp and conversions, so the same source compiled for the host gives the
+// same bits (NaN payloads aside).
 #pragma once
 #include <stdint.h>
 
@@ -60,6 +62,10 @@ constexpr double LN2_0 = 0x1.62e42ff000000p-1;
 constexpr double LN2_1 = -0x1.718432a200000p-35;
 constexpr double LN2_2 = 0x1.3c7673007e5edp-69;
 constexpr double INV_LN2 = 0x1.71547652b82fep+0;
+// Rounding by addition: for |v| < 2^51, fl(v + 1.5 2^52) is 1.5 2^52 plus v
+// rounded to the nearest integer (ties to even), and the low word of that
+// double is the integer modulo 2^32.
+constexpr double ROUND_MAGIC = 0x1.8000000000000p+52;
 // log's split point: the high word of sqrt(1/2) (0x1.6a09e667f3bcdp-1)
 constexpr uint32_t kLogSplitHi = 0x3fe6a09eu;
 
@@ -258,7 +264,8 @@ enum CoefSlot : int {
     kPio32Scaled64 = 31,
     kLn2 = 32,        // LN2_0, LN2_1, LN2_2
     kInvLn2 = 35,
-    kCoefCount = 36
+    kRoundMagic = 36,
+    kCoefCount = 37
 };
 #define GM_COEF_INIT                                                              \
     {EXP_C[0], EXP_C[1], EXP_C[2], EXP_C[3], EXP_C[4], EXP_C[5], EXP_C[6],        \
@@ -267,7 +274,7 @@ enum CoefSlot : int {
      LOG1P_C[6],                                                                  \
      SINT_C[0], SINT_C[1], SINT_C[2], COST_C[0], COST_C[1], COST_C[2], COST_C[3],  \
      PIO32_0, PIO32_1, PIO32_2, INV_PIO32, PIO32_SCALED64,                        \
-     LN2_0, LN2_1, LN2_2, INV_LN2}
+     LN2_0, LN2_1, LN2_2, INV_LN2, ROUND_MAGIC}
 
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const __attribute__((address_space(4))) double* CoefPtr;
@@ -333,22 +340,42 @@ GM_HD float expf_r(float x, CoefPtr C) {
 }
 
 // ---- log of a positive, finite, non-zero float --------------------------
-GM_HD float logf_pos(float x, CoefPtr C) {
-    // x = y 2^e with y in [S, 2S), S = 0.70710659... the double with the high
-    // word of sqrt(1/2) and a zero low word: four integer operations on the
-    // high word of (double)x, which is normal for every positive float
+// In two steps, so that an interpreter word can issue the table loads of all
+// its cases before it uses any: log_reduce splits x and picks the table row,
+// log_finish takes the row (c_j, ln(1/c_j)).
+//
+// x = y 2^e with y in [S, 2S), S = 0.70710659... the double with the high
+// word of sqrt(1/2) and a zero low word: integer operations on the high word
+// hi of (double)x, which is normal for every positive float.  With
+// d = hi - hi(S), e = d >> 20 and y's high word hi - (d & 0xfff00000) =
+// hi(S) + (d & 0xfffff): y lies in [S, 2S) whatever the bits of x, so the row
+// below is in range for every argument, 0, infinities and NaN included.  The
+// row is rint((y - 1) 128) + 37 in 0..90; returned here as rint(y 128) in
+// 91..181 (y 128 is exact, so rounding it by addition is rint).
+GM_HD double log_reduce(float x, int32_t& e, uint32_t& row91, CoefPtr C) {
     const uint64_t bits = d2u((double)x);
-    const int32_t e = (int32_t)((uint32_t)(bits >> 32) - kLogSplitHi) >> 20;
-    const double y = u2d(bits - ((uint64_t)(int64_t)e << 52));  // 24 significant bits
-    // rint((y - 1) 128) + 37 in 0..90; y * 128 is exact
-    const int j = (int)rint_d(y * 128.0) - 91;
-    const double r = fma_d(y, LOG_T[2 * j], -1.0);  // exact: c_j has 24 bits
+    const uint32_t hi = (uint32_t)(bits >> 32);
+    const uint32_t dh = hi - kLogSplitHi;
+    e = (int32_t)dh >> 20;
+    const double y = u2d(((uint64_t)(hi - (dh & 0xfff00000u)) << 32) | (bits & 0xffffffffu));
+    row91 = (uint32_t)d2u(y * 128.0 + C[kRoundMagic]);
+    return y;
+}
+GM_HD float log_finish(double y, int32_t e, double cj, double lncj, CoefPtr C) {
+    const double r = fma_d(y, cj, -1.0);  // exact: c_j has 24 bits
     double p = C[kLog1pC + 6];
     for (int n = 5; n >= 0; --n) p = fma_d(p, r, C[kLog1pC + n]);
     const double q = fma_d(r * r, p, r);  // log1p(r)
     const double ed = (double)e;
     // e ln2 + ln(1/c_j) + log1p(r); e * LN2_0 is exact
-    return (float)(fma_d(ed, C[kLn2], LOG_T[2 * j + 1]) + fma_d(ed, C[kLn2 + 1], q));
+    return (float)(fma_d(ed, C[kLn2], lncj) + fma_d(ed, C[kLn2 + 1], q));
+}
+GM_HD float logf_pos(float x, CoefPtr C) {
+    int32_t e;
+    uint32_t row91;
+    const double y = log_reduce(x, e, row91, C);
+    const uint32_t j = (row91 - 91u) & 127u;  // 0..90; the mask only bounds it
+    return log_finish(y, e, LOG_T[2 * j], LOG_T[2 * j + 1], C);
 }
 
 // protected log of the interpreter's EXT8 table: b != 0 ? log|b| : 0
@@ -363,12 +390,13 @@ GM_HD float plogf_special_value(float b) {
     return (b != b) ? b : (a == 0.0f ? 0.0f : a);
 }
 
-// Without branches: the logarithm is taken of 1 in place of 0, infinity and
-// NaN, whose results (0, +inf, NaN) are selected at the end.
+// Without branches: the arithmetic runs on every |b| (log_reduce keeps the
+// table row in range for any bits), and the results of 0, infinity and NaN
+// (0, +inf, NaN) are selected at the end.
 GM_HD float plogf_r(float b, CoefPtr C) {
     const float a = u2f(f2u(b) & 0x7fffffffu);
     const bool regular = a > 0.0f && a < u2f(0x7f800000u);  // false for 0, inf, NaN
-    const float r = logf_pos(regular ? a : 1.0f, C);
+    const float r = logf_pos(a, C);
     const float special = (b != b) ? b : (a == 0.0f ? 0.0f : a);
     return regular ? r : special;
 }
@@ -376,10 +404,15 @@ GM_HD float plogf_r(float b, CoefPtr C) {
 // ---- sin / cos ---------------------------------------------------------------
 // x = k pi/32 + t, |t| <= pi/64 (plus rounding), j = k mod 64.
 
-// Finite arguments with |x| >= 2^25 take the Payne-Hanek reduction.
+// Finite arguments with |x| >= 2^25 take the Payne-Hanek reduction: with the
+// sign shifted out, the bits of |x| lie in [2^25, inf) as [0x98 << 24, 0xff << 24).
 GM_HD bool sincos_needs_ph(float x) {
+#ifdef XOLDCLASS
     const uint32_t ex = (f2u(x) >> 23) & 0xffu;
     return ex >= 127u + 25u && ex != 255u;
+#else
+    return (f2u(x) << 1) - 0x98000000u < 0x67000000u;
+#endif
 }
 
 // Payne-Hanek for finite |x| >= 2^25: |x| = M 2^E with E = ex - 150 >= 2.
@@ -424,13 +457,13 @@ GM_HD double reduce_ph(float x, int& j, CoefPtr C) {
 }
 
 // sin(j pi/32 + t) = S_j cos t + C_j sin t = S_j + (S_j (cos t - 1) + C_j sin t)
-GM_HD double sin_from_reduced(double t, int j, CoefPtr C) {
-    const double s = SINCOS_T[2 * j];
-    const double c = SINCOS_T[2 * j + 1];
+// from the table row (s, c) = (S_j, C_j)
+GM_HD double sin_from_row(double t, double s, double c, CoefPtr C) {
     const double t2 = t * t;
     // sin t = t + t^3 (s3 + t^2 (s5 + t^2 s7)); the innermost step is a
-    // multiply and an add, since one VOP3 instruction cannot read two SGPR
-    // pairs on gfx950
+    // multiply and an add: one VOP3 instruction on gfx950 reads at most one
+    // SGPR pair, and an FMA of two coefficients would keep one of them in a
+    // VGPR pair for the whole kernel
     double ps = t2 * C[kSinT + 2] + C[kSinT + 1];
     ps = fma_d(ps, t2, C[kSinT]);
     const double sin_t = fma_d(t * t2, ps, t);
@@ -441,30 +474,52 @@ GM_HD double sin_from_reduced(double t, int j, CoefPtr C) {
     const double cm1 = pc * t2;
     return s + fma_d(c, sin_t, s * cm1);
 }
+GM_HD double sin_from_reduced(double t, int j, CoefPtr C) {
+    return sin_from_row(t, SINCOS_T[2 * j], SINCOS_T[2 * j + 1], C);
+}
+
+// Cody-Waite for an x that does not need Payne-Hanek: x = k pi/32 + t, and the
+// table row j = (k + 16 want_cos) mod 64 (cos x = sin(x + pi/2)).  k =
+// rint(x 32/pi) is rounded by addition (|k| < 2^28.4), so the row is the low
+// word of that sum: no conversion, and for an infinite or NaN x, whose t is
+// NaN, the row is still in 0..63.
+GM_HD double sincos_reduce(float x, int want_cos, uint32_t& j, CoefPtr C) {
+    const double xd = (double)x;
+    const double kb = xd * C[kInvPio32] + C[kRoundMagic];
+    const double k = kb - C[kRoundMagic];
+    j = ((uint32_t)d2u(kb) + 16u * (uint32_t)want_cos) & 63u;
+    double t = fma_d(-k, C[kPio32], xd);
+    t = fma_d(-k, C[kPio32 + 1], t);
+    return fma_d(-k, C[kPio32 + 2], t);
+}
+
+// The rounded result from the reduced argument and its row: infinities and
+// NaN come out NaN, cos(+-0) is 1, sin(+0) is +0, and sin(-0), the one case
+// the arithmetic gets wrong (t = -0 + 0 = +0), keeps its argument.
+GM_HD float sincos_finish(float x, int want_cos, double t, double s, double c, CoefPtr C) {
+    const float r = (float)sin_from_row(t, s, c, C);
+    return (want_cos || x != 0.0f) ? r : x;
+}
 
 // sin (want_cos = 0) or cos (1) of an x that does not need Payne-Hanek, by
 // Cody-Waite, without branches: the K cases of an interpreter word run
-// interleaved.  Infinities and NaN are reduced as 0 and give NaN at the end;
-// sin keeps the sign of a zero argument.
+// interleaved.
 GM_HD float sincos_cw(float x, int want_cos, CoefPtr C) {
-    const float ax = u2f(f2u(x) & 0x7fffffffu);
-    const bool fin = ax < u2f(0x7f800000u);  // false for inf, NaN
-    const double xd = fin ? (double)x : 0.0;
-    const double k = rint_d(xd * C[kInvPio32]);  // |k| < 2^28.4
-    double t = fma_d(-k, C[kPio32], xd);
-    t = fma_d(-k, C[kPio32 + 1], t);
-    t = fma_d(-k, C[kPio32 + 2], t);
-    const int j = (((int)k) + 16 * want_cos) & 63;  // cos x = sin(x + pi/2)
-    const float r = (float)sin_from_reduced(t, j, C);
-    const float zero_case = want_cos ? r : x;
-    return !fin ? x - x : (x == 0.0f ? zero_case : r);
+    uint32_t j;
+    const double t = sincos_reduce(x, want_cos, j, C);
+    return sincos_finish(x, want_cos, t, SINCOS_T[2 * j], SINCOS_T[2 * j + 1], C);
 }
 
 // Arguments whose sin / cos is fixed without arithmetic: zeros (sin keeps the
-// sign, cos is 1) and infinities / NaN (NaN).
+// sign, cos is 1) and infinities / NaN (NaN).  With the sign shifted out, the
+// bits are 0 or at least those of inf, 0xff << 24.
 GM_HD bool sincos_special(float x) {
+#ifdef XOLDCLASS
     const float ax = u2f(f2u(x) & 0x7fffffffu);
     return x == 0.0f || !(ax < u2f(0x7f800000u));
+#else
+    return (f2u(x) << 1) - 1u >= 0xfeffffffu;
+#endif
 }
 GM_HD float sincos_special_value(float x, int want_cos) {
     return x == 0.0f ? (want_cos ? 1.0f : x) : x - x;

@@ -192,6 +192,8 @@ def constants():
         "PIO32_SCALED64": float(Fraction(pi, 1 << (PREC + 5)) / (1 << 64)),
         "LN2": split(ln2, PREC, 32),
         "INV_LN2": float(Fraction(1 << PREC, ln2)),
+        # 1.5 2^52: adding it rounds |v| < 2^51 to an integer held in the low word
+        "ROUND_MAGIC": float(3 << 51),
         "TWO_OVER_PI_WORDS": two_over_pi_words(12),
         "EXP_POLY": taylor("exp", 13),
         "SINT_POLY": taylor("sin", 4)[1:],   # -1/3!, 1/5!, -1/7!

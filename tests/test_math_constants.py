@@ -49,6 +49,7 @@ def test_constants_match_derivation():
     assert h["INV_PIO32"] == c["INV_PIO32"]
     assert h["PIO32_SCALED64"] == c["PIO32_SCALED64"]
     assert h["INV_LN2"] == c["INV_LN2"]
+    assert h["ROUND_MAGIC"] == c["ROUND_MAGIC"] == 1.5 * 2.0 ** 52
     assert h["EXP_C"] == c["EXP_POLY"]
     assert h["SINT_C"] == c["SINT_POLY"]
     assert h["COST_C"] == c["COST_POLY"]
