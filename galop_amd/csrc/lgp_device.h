@@ -189,6 +189,9 @@ __device__ __forceinline__ DestDecode dest_decode(int nr, int iprot) {
 #ifndef GALOP_ROW_GROUP
 #define GALOP_ROW_GROUP 2
 #endif
+// Ends a group of table-row cases: the scheduler moves no instruction across
+// it, so one group's loads and arithmetic are not spread into the next.
+#define GALOP_GROUP_FENCE() __builtin_amdgcn_sched_barrier(0)
 
 template <int NR, int K, int EXT>
 __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
@@ -292,6 +295,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                     _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) \
                         cv[g + i] = galop_math::sincos_finish(bv[g + i], WANT_COS, t[i], \
                                                               ts[i], tc[i], C); \
+                    GALOP_GROUP_FENCE();                                  \
                 }                                                         \
             }                                                             \
             _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]); \
@@ -338,6 +342,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                         cv[g + i] = galop_math::plogf_special(b)          \
                                         ? galop_math::plogf_special_value(b) : v; \
                     }                                                     \
+                    GALOP_GROUP_FENCE();                                  \
                 }                                                         \
             }                                                             \
             _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]); \
@@ -396,6 +401,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
 #undef GALOP_UNARY
 #undef GALOP_SINCOS
 #undef GALOP_PLOG
+#undef GALOP_GROUP_FENCE
 }
 
 // Run `len` words starting at `prog` on the K register vectors.  The next
@@ -571,7 +577,12 @@ __device__ __forceinline__ void load_cases(const DevData& ds, int64_t first,
 #pragma unroll
     for (int i = 0; i < NR; ++i) {
         if (i < ds.n_inputs) {
+            // the column base is wave-uniform: kept in SGPRs for this tile,
+            // the loads take it with one lane-offset VGPR shared by all
+            // columns (hoisted out of the program loop, the per-column
+            // 64-bit lane pointers were kept in VGPRs, and spilled)
             const float* __restrict__ col = ds.inputs + (int64_t)i * ds.n_pad + first;
+            __asm__ volatile("" : "+s"(col));
 #pragma unroll
             for (int k = 0; k < K; ++k) r.set(k, i, col[k * kWave + lane]);
         } else {

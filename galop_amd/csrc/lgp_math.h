@@ -18,7 +18,7 @@
 //        its double (found by integer operations on that word); j = rint((y-1) 128) picks
 //        c_j = 128/(128+j) rounded to 24 bits, so r = y c_j - 1 is exact
 //        (|r| < 0.0056); log x = e ln2 + ln(1/c_j) + log1p(r), with ln(1/c_j)
-//        from a 91-entry table and log1p(r) by its series to r^8.  No
+//        from a 91-entry table and log1p(r) by its series to r^7.  No
 //        division.  (The same table method for exp, 2^(j/64) with a
 //        degree-6 series, measured slower in the interpreter than the
 //        table-free Taylor form: its per-lane table load stalls the wave
@@ -79,8 +79,8 @@ constexpr double EXP_C[13] = {
 constexpr double SINT_C[3] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13};
 // (-1)^k / (2k)!, k = 1..4
 constexpr double COST_C[4] = {-0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-16};
-// (-1)^(n+1)/n, n = 2..8  (log1p r = r + r^2 sum_n (-1)^(n+1)/n r^(n-2))
-constexpr double LOG1P_C[7] = {-0x1.0000000000000p-1, 0x1.5555555555555p-2, -0x1.0000000000000p-2, 0x1.999999999999ap-3, -0x1.5555555555555p-3, 0x1.2492492492492p-3, -0x1.0000000000000p-3};
+// (-1)^(n+1)/n, n = 2..7  (log1p r = r + r^2 sum_n (-1)^(n+1)/n r^(n-2))
+constexpr double LOG1P_C[6] = {-0x1.0000000000000p-1, 0x1.5555555555555p-2, -0x1.0000000000000p-2, 0x1.999999999999ap-3, -0x1.5555555555555p-3, 0x1.2492492492492p-3};
 // bits 2^-1 .. 2^-256 of 2/pi (word 0 first)
 GM_CONST uint32_t TWO_OVER_PI_W[8] = {0xA2F9836Eu, 0x4E441529u, 0xFC2757D1u, 0xF534DDC0u,
                                       0xDB629599u, 0x3C439041u, 0xFE5163ABu, 0xDEBBC561u};
@@ -256,22 +256,21 @@ GM_CONST double LOG_T[182] = {
 // gfx950 takes no 64-bit literal), about 60 VGPRs that cost occupancy.
 enum CoefSlot : int {
     kExpC = 0,        // EXP_C[0..12]
-    kLog1pC = 13,     // LOG1P_C[0..6]
-    kSinT = 20,       // SINT_C[0..2]
-    kCosT = 23,       // COST_C[0..3]
-    kPio32 = 27,      // PIO32_0, PIO32_1, PIO32_2
-    kInvPio32 = 30,
-    kPio32Scaled64 = 31,
-    kLn2 = 32,        // LN2_0, LN2_1, LN2_2
-    kInvLn2 = 35,
-    kRoundMagic = 36,
-    kCoefCount = 37
+    kLog1pC = 13,     // LOG1P_C[0..5]
+    kSinT = 19,       // SINT_C[0..2]
+    kCosT = 22,       // COST_C[0..3]
+    kPio32 = 26,      // PIO32_0, PIO32_1, PIO32_2
+    kInvPio32 = 29,
+    kPio32Scaled64 = 30,
+    kLn2 = 31,        // LN2_0, LN2_1, LN2_2
+    kInvLn2 = 34,
+    kRoundMagic = 35,
+    kCoefCount = 36
 };
 #define GM_COEF_INIT                                                              \
     {EXP_C[0], EXP_C[1], EXP_C[2], EXP_C[3], EXP_C[4], EXP_C[5], EXP_C[6],        \
      EXP_C[7], EXP_C[8], EXP_C[9], EXP_C[10], EXP_C[11], EXP_C[12],               \
      LOG1P_C[0], LOG1P_C[1], LOG1P_C[2], LOG1P_C[3], LOG1P_C[4], LOG1P_C[5],      \
-     LOG1P_C[6],                                                                  \
      SINT_C[0], SINT_C[1], SINT_C[2], COST_C[0], COST_C[1], COST_C[2], COST_C[3],  \
      PIO32_0, PIO32_1, PIO32_2, INV_PIO32, PIO32_SCALED64,                        \
      LN2_0, LN2_1, LN2_2, INV_LN2, ROUND_MAGIC}
@@ -363,8 +362,9 @@ GM_HD double log_reduce(float x, int32_t& e, uint32_t& row91, CoefPtr C) {
 }
 GM_HD float log_finish(double y, int32_t e, double cj, double lncj, CoefPtr C) {
     const double r = fma_d(y, cj, -1.0);  // exact: c_j has 24 bits
-    double p = C[kLog1pC + 6];
-    for (int n = 5; n >= 0; --n) p = fma_d(p, r, C[kLog1pC + n]);
+    // log1p(r) to r^7: the rest is below |r|^8 / 8 < 1.2e-19 for |r| < 0.0056
+    double p = C[kLog1pC + 5];
+    for (int n = 4; n >= 0; --n) p = fma_d(p, r, C[kLog1pC + n]);
     const double q = fma_d(r * r, p, r);  // log1p(r)
     const double ed = (double)e;
     // e ln2 + ln(1/c_j) + log1p(r); e * LN2_0 is exact
@@ -456,8 +456,9 @@ GM_HD double reduce_ph(float x, int& j, CoefPtr C) {
     return t;
 }
 
-// sin(j pi/32 + t) = S_j cos t + C_j sin t = S_j + (S_j (cos t - 1) + C_j sin t)
-// from the table row (s, c) = (S_j, C_j)
+// sin(j pi/32 + t) = S_j cos t + C_j sin t from the table row (s, c) =
+// (S_j, C_j).  With sin t = t + t^3 ps and cos t - 1 = t^2 pc this is
+// (s + c t) + t^2 (s pc + c t ps): five operations after the two series.
 GM_HD double sin_from_row(double t, double s, double c, CoefPtr C) {
     const double t2 = t * t;
     // sin t = t + t^3 (s3 + t^2 (s5 + t^2 s7)); the innermost step is a
@@ -466,13 +467,12 @@ GM_HD double sin_from_row(double t, double s, double c, CoefPtr C) {
     // VGPR pair for the whole kernel
     double ps = t2 * C[kSinT + 2] + C[kSinT + 1];
     ps = fma_d(ps, t2, C[kSinT]);
-    const double sin_t = fma_d(t * t2, ps, t);
     // cos t - 1 = t^2 (c2 + t^2 (c4 + t^2 (c6 + t^2 c8)))
     double pc = t2 * C[kCosT + 3] + C[kCosT + 2];
     pc = fma_d(pc, t2, C[kCosT + 1]);
     pc = fma_d(pc, t2, C[kCosT]);
-    const double cm1 = pc * t2;
-    return s + fma_d(c, sin_t, s * cm1);
+    const double ct = c * t;
+    return fma_d(t2, fma_d(ct, ps, s * pc), fma_d(c, t, s));
 }
 GM_HD double sin_from_reduced(double t, int j, CoefPtr C) {
     return sin_from_row(t, SINCOS_T[2 * j], SINCOS_T[2 * j + 1], C);

@@ -199,8 +199,8 @@ def constants():
         "SINT_POLY": taylor("sin", 4)[1:],   # -1/3!, 1/5!, -1/7!
         "COST_POLY": taylor("cos", 5)[1:],   # -1/2!, 1/4!, -1/6!, 1/8!
         "SINCOS_T": sincos_table(),
-        # log1p(r) = r + r^2 (-1/2 + r/3 - r^2/4 + ... - r^6/8)
-        "LOG1P_POLY": [float(Fraction((-1) ** (n + 1), n)) for n in range(2, 9)],
+        # log1p(r) = r + r^2 (-1/2 + r/3 - r^2/4 + ... + r^5/7)
+        "LOG1P_POLY": [float(Fraction((-1) ** (n + 1), n)) for n in range(2, 8)],
         "LOG_T": log_table(),
     }
     return c
