@@ -118,6 +118,61 @@ def test_unary_op_ulp(gpu_ctx, op_set, op):
           f"to correctly rounded on {(d_exact == 0).mean():.4%}")
 
 
+@pytest.mark.parametrize("op_set,op", [(0, 4), (0, 5), (1, 6), (1, 7)])
+def test_unary_special_lanes_on_fast_path(gpu_ctx, op_set, op):
+    """Zeros, infinities, NaN and subnormals mixed into words whose other
+    arguments are ordinary and below 2^25, so the word takes the branch-free
+    path (no Payne-Hanek lane): sin/cos there get no special-value selects
+    but the one for sin(-0).  Per case (one case per lane): bitwise the host
+    build (NaNs as one value; sin(-0) = -0, cos(+-0) = 1) and within 1 ulp
+    of the correctly rounded value.  At 8 cases per lane (the table rows of
+    two cases loaded ahead) through the fitness: with the host build's
+    outputs as targets, on the same mix without inf and NaN, the FP64 SSE
+    is exactly 0 at K = 1 and K = 8."""
+    rng = np.random.default_rng(40 + op)
+    n = 1 << 14
+    x = rng.uniform(-10, 10, n).astype(np.float32)
+    big = [-3.0e7, -104.0, 88.0] if op == 6 else [-3.0e7, 3.0e7, 33554430.0, -104.0, 88.0]
+    finite = np.array([0.0, -0.0, 1e-45, -1e-45, 1.17549435e-38] + big, np.float32)
+    nonfinite = np.array([np.inf, -np.inf, np.nan], np.float32)
+    specials = np.concatenate([finite, nonfinite])
+    pos = rng.choice(n, 6 * specials.size, replace=False)
+    x[pos] = np.tile(specials, 6)
+    isa = _isa(8, op_set)
+    word = np.array([op], np.uint32)  # r0 = op(r0), register operand
+    offsets = np.array([0, 1], np.int64)
+    oisa = O.isa_from(isa)
+    with O.unary_math(math_host_path()):
+        host = O.run_program(oisa, word, x)
+    with np.errstate(all="ignore"):
+        fn = {4: np.sin, 5: np.cos, 6: np.exp,
+              7: lambda v: np.where(v != 0, np.log(np.abs(v)), 0.0)}[op]
+        exact = fn(x.astype(np.float64)).astype(np.float32)
+    out = gpu_cases(_dds(gpu_ctx, isa, x, np.zeros_like(x)), word, offsets)[0]
+    assert same_bits(out, host).all(), "device differs from its host build"
+    assert ulp_distance(out, exact).max() <= 1
+    if op == 4:
+        z = x == 0
+        assert z.sum() == 12 and np.array_equal(np.signbit(out[z]), np.signbit(x[z]))
+    if op == 5:
+        assert (out[x == 0] == 1.0).all()
+    # 8 cases per lane: the same mix without inf and NaN, host outputs as targets
+    xf = x.copy()
+    bad = ~np.isfinite(xf)
+    xf[bad] = np.tile(finite, 6)[: bad.sum()]
+    with O.unary_math(math_host_path()):
+        yf = O.run_program(oisa, word, xf)
+    assert np.isfinite(yf).all()
+    dds = _dds(gpu_ctx, isa, xf, yf)
+    try:
+        for k in (1, 8):
+            gpu_ctx.set_tuning(k, 0)
+            fit, sse = gpu_fitness(dds, word, offsets)
+            assert sse[0] == 0.0, f"K={k}: SSE {sse[0]!r} against the host build's outputs"
+    finally:
+        gpu_ctx.set_tuning(0, 0)
+
+
 @pytest.mark.parametrize("nr", [8, 16])
 @pytest.mark.parametrize("op_set", [0, 1])
 def test_unary_on_every_constant(gpu_ctx, nr, op_set):
