@@ -339,8 +339,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                     _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) { \
                         const float b = bv[g + i];                        \
                         const float v = galop_math::log_finish(y[i], e[i], tcj[i], tln[i], C); \
-                        cv[g + i] = galop_math::plogf_special(b)          \
-                                        ? galop_math::plogf_special_value(b) : v; \
+                        cv[g + i] = galop_math::plogf_special(b) ? __builtin_fabsf(b) : v; \
                     }                                                     \
                     GALOP_GROUP_FENCE();                                  \
                 }                                                         \

@@ -379,26 +379,22 @@ GM_HD float logf_pos(float x, CoefPtr C) {
 }
 
 // protected log of the interpreter's EXT8 table: b != 0 ? log|b| : 0
-// Arguments whose protected log is fixed without arithmetic: 0 -> 0,
-// +-inf -> +inf, NaN -> NaN.
+// Arguments whose protected log is fixed without arithmetic: 0 -> +0,
+// +-inf -> +inf, NaN -> NaN.  Each of these results is |b| itself.
 GM_HD bool plogf_special(float b) {
     const float a = u2f(f2u(b) & 0x7fffffffu);
     return !(a > 0.0f && a < u2f(0x7f800000u));
 }
-GM_HD float plogf_special_value(float b) {
-    const float a = u2f(f2u(b) & 0x7fffffffu);
-    return (b != b) ? b : (a == 0.0f ? 0.0f : a);
-}
+GM_HD float plogf_special_value(float b) { return u2f(f2u(b) & 0x7fffffffu); }
 
 // Without branches: the arithmetic runs on every |b| (log_reduce keeps the
-// table row in range for any bits), and the results of 0, infinity and NaN
-// (0, +inf, NaN) are selected at the end.
+// table row in range for any bits), and 0, infinity and NaN take |b| (+0,
+// +inf, NaN) at the end.
 GM_HD float plogf_r(float b, CoefPtr C) {
     const float a = u2f(f2u(b) & 0x7fffffffu);
     const bool regular = a > 0.0f && a < u2f(0x7f800000u);  // false for 0, inf, NaN
     const float r = logf_pos(a, C);
-    const float special = (b != b) ? b : (a == 0.0f ? 0.0f : a);
-    return regular ? r : special;
+    return regular ? r : a;
 }
 
 // ---- sin / cos ---------------------------------------------------------------
