@@ -38,11 +38,11 @@ SIMDS_PER_CU = 4               # one wave64 VALU instruction per SIMD per 4 cloc
 # (1,048,576 KiB copied: FETCH_SIZE 524,307 KiB, WRITE_SIZE 1,048,576 KiB).
 # They describe that exact population only.
 PLAIN_WORKLOAD_COUNTERS = {
-    "fetch_bytes": 320861.4 * 1024 * (1048576 / 524308.2),
-    "write_bytes": 25797.3 * 1024 * 1.0,
+    "fetch_bytes": 320867.3 * 1024 * (1048576 / 524307.4),
+    "write_bytes": 25671.9 * 1024 * 1.0,
     "valu_insts": 25649597039,
     "salu_insts": 7761334077,
-    "valu_busy_pct": 85.9,
+    "valu_busy_pct": 86.8,
 }
 
 

@@ -187,7 +187,7 @@ __device__ __forceinline__ DestDecode dest_decode(int nr, int iprot) {
 }
 
 #ifndef GALOP_ROW_GROUP
-#define GALOP_ROW_GROUP 2
+#define GALOP_ROW_GROUP 4
 #endif
 // Ends a group of table-row cases: the scheduler moves no instruction across
 // it, so one group's loads and arithmetic are not spread into the next.
