@@ -387,11 +387,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
         else GALOP_BIN((a < b) ? 1.0f : 0.0f);
         break;
     default:
-#ifdef XOLDPLOG
-        if constexpr (EXT) GALOP_UNARY(3, op_plog);
-#else
         if constexpr (EXT) GALOP_PLOG();
-#endif
         else GALOP_BIN((a > b) ? 1.0f : 0.0f);
         break;
     }

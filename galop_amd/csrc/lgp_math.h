@@ -403,12 +403,7 @@ GM_HD float plogf_r(float b, CoefPtr C) {
 // Finite arguments with |x| >= 2^25 take the Payne-Hanek reduction: with the
 // sign shifted out, the bits of |x| lie in [2^25, inf) as [0x98 << 24, 0xff << 24).
 GM_HD bool sincos_needs_ph(float x) {
-#ifdef XOLDCLASS
-    const uint32_t ex = (f2u(x) >> 23) & 0xffu;
-    return ex >= 127u + 25u && ex != 255u;
-#else
     return (f2u(x) << 1) - 0x98000000u < 0x67000000u;
-#endif
 }
 
 // Payne-Hanek for finite |x| >= 2^25: |x| = M 2^E with E = ex - 150 >= 2.
@@ -510,12 +505,7 @@ GM_HD float sincos_cw(float x, int want_cos, CoefPtr C) {
 // sign, cos is 1) and infinities / NaN (NaN).  With the sign shifted out, the
 // bits are 0 or at least those of inf, 0xff << 24.
 GM_HD bool sincos_special(float x) {
-#ifdef XOLDCLASS
-    const float ax = u2f(f2u(x) & 0x7fffffffu);
-    return x == 0.0f || !(ax < u2f(0x7f800000u));
-#else
     return (f2u(x) << 1) - 1u >= 0xfeffffffu;
-#endif
 }
 GM_HD float sincos_special_value(float x, int want_cos) {
     return x == 0.0f ? (want_cos ? 1.0f : x) : x - x;

@@ -138,6 +138,75 @@ double go_program_sse(const go_isa* isa, const uint32_t* prog, int64_t len,
     return sse;
 }
 
+/* ---- planted mistakes (tests only) --------------------------------------
+ * A test sets one of these to show that its checks of the device-order sum
+ * and of the attribution probes fail when the code is wrong in that way; 0
+ * restores the real code. */
+static int g_fault = GO_FAULT_NONE;
+void go_plant_fault(int32_t code) { g_fault = code; }
+
+/* ---- the GPU evaluator's summation order ---------------------------------
+ * The same per-case FP32 outputs, summed the way the gfx950 kernels sum them
+ * (galop_amd/csrc/lgp_device.h, LaneAcc / wave_sum_f64 / kCaseBlock): cases
+ * in blocks of 16384; in a block, lane l of 64 takes cases l, l+64, ... in
+ * increasing order into a compensated FP32 sum (TwoSum of out - y, exact
+ * square by FMA, Neumaier step; |err| >= 2^48 and non-finite errors go to an
+ * FP64 side sum); the 64 lane totals (s + c) + big are added in the fixed
+ * tree of a wave's shuffle-down reduction; block partials are added in block
+ * order.  This is a second summation of the same FP64-rounded squares, not a
+ * different function: it agrees with go_program_sse to about 1e-15 relative
+ * and is used where the checker must reproduce the device's bits (fixtures
+ * of whole evolutionary runs). */
+#define GO_LANES 64
+#define GO_CASE_BLOCK 16384
+
+typedef struct { float s, c; double big; } go_lane_acc;
+
+static void lane_add(go_lane_acc* a, float out, float y) {
+    const float d = out - y;
+    if (fabsf(d) < 0x1p48f) {
+        const float bb = d - out;
+        const float dlo = (out - (d - bb)) + (-y - bb);
+        const float p = d * d;
+        const float plo = fmaf(d, d, -p);
+        const float t = a->s + p;
+        const float z = t - a->s;
+        if (g_fault != GO_FAULT_DEVSUM_UNCOMPENSATED)
+            a->c += ((a->s - (t - z)) + (p - z)) + (plo + 2.0f * d * dlo);
+        a->s = t;
+    } else {
+        const double dd = (double)out - (double)y;
+        a->big += dd * dd;
+    }
+}
+
+double go_sse_device_order(const float* outs, const float* targets, int64_t n_cases) {
+    double sse = 0.0;
+    for (int64_t c0 = 0; c0 < n_cases; c0 += GO_CASE_BLOCK) {
+        const int64_t c1 = (c0 + GO_CASE_BLOCK) < n_cases ? (c0 + GO_CASE_BLOCK) : n_cases;
+        double v[GO_LANES];
+        for (int l = 0; l < GO_LANES; ++l) {
+            go_lane_acc a = {0.0f, 0.0f, 0.0};
+            for (int64_t c = c0 + l; c < c1; c += GO_LANES) lane_add(&a, outs[c], targets[c]);
+            v[l] = ((double)a.s + (double)a.c) + a.big;
+        }
+        for (int off = GO_LANES / 2; off > 0; off >>= 1)
+            for (int l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
+        sse += v[0];
+    }
+    return sse;
+}
+
+double go_program_sse_device_order(const go_isa* isa, const uint32_t* prog, int64_t len,
+                                   const float* inputs, const float* targets,
+                                   int32_t n_inputs, int64_t n_cases) {
+    float* outs = (float*)malloc((size_t)(n_cases > 0 ? n_cases : 1) * sizeof(float));
+    go_run_program(isa, prog, len, inputs, n_inputs, n_cases, outs);
+    const double sse = go_sse_device_order(outs, targets, n_cases);
+    free(outs);
+    return sse;
+}
+
 /* src/Evaluator.cpp:88-92 */
 float go_mse_to_fitness(float mse) {
     if (!isfinite(mse)) return 0.0f;
@@ -221,6 +290,21 @@ void go_eval_dense(const go_isa* isa, const uint32_t* instructions,
     }
 }
 
+/* go_eval_dense with the GPU evaluator's summation order. */
+void go_eval_dense_device_order(const go_isa* isa, const uint32_t* instructions,
+                                const uint8_t* lengths, int32_t stride, int64_t n_programs,
+                                const float* inputs, const float* targets, int32_t n_inputs,
+                                int64_t n_cases, float* fitness_out, double* sse_out) {
+    const double ss_tot = isa->fitness == GO_FITNESS_R2 ? go_ss_tot(targets, n_cases) : 0.0;
+    for (int64_t i = 0; i < n_programs; ++i) {
+        const double sse = go_program_sse_device_order(
+            isa, instructions + (size_t)i * (size_t)stride, (int64_t)lengths[i], inputs, targets,
+            n_inputs, n_cases);
+        if (sse_out) sse_out[i] = sse;
+        fitness_out[i] = go_fitness_of_sse(isa, sse, n_cases, ss_tot);
+    }
+}
+
 /* ---- src/Introns.h:28-57: backward liveness from the output register --- */
 int64_t go_effective_mask(const go_isa* isa, const uint32_t* prog, int64_t len,
                           uint8_t* mask) {
@@ -260,7 +344,9 @@ static double apply_op_f64(int op_set, int op, double a, double b) {
     case 0: return a + b;
     case 1: return a - b;
     case 2: return a * b;
-    case 3: return (b != 0.0) ? a / b : 1.0;
+    case 3:
+        if (g_fault == GO_FAULT_DUAL_UNPROTECTED_DIV) return a / b;
+        return (b != 0.0) ? a / b : 1.0;
     case 4: return sin(b);
     case 5: return cos(b);
     case 6:
@@ -294,7 +380,8 @@ void go_dual_precision_fitness(const go_isa* isa, const uint32_t* prog,
             const float fb = k2 ? isa->constants[s2] : fr[s2];
             const double db = k2 ? (double)isa->constants[s2] : dr[s2];
             fr[d] = go_apply_op(isa->op_set, op, fr[s1], fb);
-            dr[d] = apply_op_f64(isa->op_set, op, dr[s1], db);
+            dr[d] = g_fault == GO_FAULT_DUAL_LEG_FLOAT ? (double)fr[d]
+                                                        : apply_op_f64(isa->op_set, op, dr[s1], db);
         }
         const double t = (double)targets[c];
         const double ef = (double)fr[o] - t;
@@ -349,7 +436,10 @@ float go_perturbed_fitness(const go_isa* isa, const uint32_t* prog, int64_t len,
             const float b = go_src2_is_const(w) ? isa->constants[go_get_src2(isa, w)]
                                                 : r[go_get_src2(isa, w)];
             float v = go_apply_op(isa->op_set, op, a, b);
-            if (op_is_unary(isa->op_set, op)) {
+            const int perturb = g_fault == GO_FAULT_PERTURB_ALL_OPS ? 1
+                                : g_fault == GO_FAULT_PERTURB_DROPPED ? 0
+                                : op_is_unary(isa->op_set, op);
+            if (perturb) {
                 const uint32_t h = mix32(seed * 0x9E3779B97F4A7C15ULL + (uint64_t)c * 1000003ULL +
                                          (uint64_t)pc);
                 v = nudge(v, (int)(h % (uint32_t)(2 * ulps + 1)) - ulps);

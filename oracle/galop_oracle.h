@@ -101,6 +101,19 @@ void go_eval_dense(const go_isa* isa, const uint32_t* instructions,
                    const float* inputs, const float* targets, int32_t n_inputs,
                    int64_t n_cases, float* fitness_out, int32_t threads);
 
+/* The same SSE summed in the order of the GPU kernels (lane-strided
+ * compensated FP32 partials, a fixed FP64 64-lane tree, 16384-case blocks):
+ * for checks that must reproduce the device's bits.  go_sse_device_order
+ * sums given per-case outputs. */
+double go_sse_device_order(const float* outs, const float* targets, int64_t n_cases);
+double go_program_sse_device_order(const go_isa* isa, const uint32_t* prog, int64_t len,
+                                   const float* inputs, const float* targets,
+                                   int32_t n_inputs, int64_t n_cases);
+void go_eval_dense_device_order(const go_isa* isa, const uint32_t* instructions,
+                                const uint8_t* lengths, int32_t stride, int64_t n_programs,
+                                const float* inputs, const float* targets, int32_t n_inputs,
+                                int64_t n_cases, float* fitness_out, double* sse_out);
+
 /* ---- intron analysis (src/Introns.h:28-57) ---- */
 int64_t go_effective_mask(const go_isa* isa, const uint32_t* prog, int64_t len,
                           uint8_t* mask);
@@ -119,6 +132,17 @@ void go_dual_precision_fitness(const go_isa* isa, const uint32_t* prog,
 float go_perturbed_fitness(const go_isa* isa, const uint32_t* prog, int64_t len,
                            const float* inputs, const float* targets, int32_t n_inputs,
                            int64_t n_cases, int32_t ulps, uint64_t seed);
+
+/* Planted mistakes for the tests of the probes and of the device-order sum
+ * (test hook; the checker's own checks must fail under each).  0 restores the
+ * real code. */
+#define GO_FAULT_NONE 0
+#define GO_FAULT_DUAL_UNPROTECTED_DIV 1 /* double leg: a / b without protection  */
+#define GO_FAULT_DUAL_LEG_FLOAT 2       /* double leg: float registers instead   */
+#define GO_FAULT_PERTURB_DROPPED 3      /* perturbation never applied            */
+#define GO_FAULT_PERTURB_ALL_OPS 4      /* every op perturbed, not the unary ones */
+#define GO_FAULT_DEVSUM_UNCOMPENSATED 5 /* device-order sum: compensation dropped */
+void go_plant_fault(int32_t code);
 
 #ifdef __cplusplus
 }

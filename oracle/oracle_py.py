@@ -70,6 +70,14 @@ def lib():
         L.go_perturbed_fitness.argtypes = [pisa, vp, i64, vp, vp, i32, i64, i32, ctypes.c_uint64]
         L.go_set_unary_math.restype = None
         L.go_set_unary_math.argtypes = [vp, vp, vp, vp]
+        L.go_plant_fault.restype = None
+        L.go_plant_fault.argtypes = [i32]
+        L.go_sse_device_order.restype = ctypes.c_double
+        L.go_sse_device_order.argtypes = [vp, vp, i64]
+        L.go_program_sse_device_order.restype = ctypes.c_double
+        L.go_program_sse_device_order.argtypes = [pisa, vp, i64, vp, vp, i32, i64]
+        L.go_eval_dense_device_order.restype = None
+        L.go_eval_dense_device_order.argtypes = [pisa, vp, vp, i32, i64, vp, vp, i32, i64, vp, vp]
         _lib = L
     return _lib
 
@@ -205,16 +213,58 @@ def libm_sensitivity(isa, prog, inputs, targets, ulps=1, seeds=8):
     return base, worst
 
 
-def attributed(isa, prog, inputs, targets, tol_rel):
-    """The reference's chaos rule (float vs double registers,
-    tests/cuda_diff.cu:162-181) or libm sensitivity at 1 ulp: True when the
-    program's fitness is not defined to tol_rel by its arithmetic."""
+def attribution(isa, prog, inputs, targets, tol_rel):
+    """Why a program's fitness is not defined to tol_rel by its arithmetic, if
+    it is not.  Two rules, checked in this order:
+
+    * "float_vs_double": the reference's chaos rule (tests/cuda_diff.cu:162-181)
+      -- the fitness with double registers differs from the fitness with float
+      registers by more than tol_rel (division protected in both legs);
+    * "libm_1ulp": moving every sin/cos/exp/log result by up to one ulp moves
+      the fitness by more than tol_rel (two C libraries that differ by one ulp
+      per operation cannot agree to tol_rel on it).
+
+    Returns {"attributed", "reason", "float_double_gap_rel",
+    "ulp_sensitivity_rel"}; the gaps are relative to the float-register
+    fitness.  Fitness is finite by construction (non-finite -> 0,
+    src/Evaluator.cpp:88-92), so every gap is a finite number."""
     ff, fd = dual_precision_fitness(isa, prog, inputs, targets)
+    base, worst = libm_sensitivity(isa, prog, inputs, targets)
+    if not (np.isfinite(ff) and np.isfinite(fd) and np.isfinite(base) and np.isfinite(worst)):
+        raise ValueError("attribution: a probe returned a non-finite fitness")
     scale = max(abs(ff), 1e-30)
-    if abs(fd - ff) > tol_rel * scale:
-        return True
-    _, worst = libm_sensitivity(isa, prog, inputs, targets)
-    return worst > tol_rel * scale
+    gap = abs(fd - ff) / scale
+    sens = worst / scale
+    reason = None
+    if gap > tol_rel:
+        reason = "float_vs_double"
+    elif sens > tol_rel:
+        reason = "libm_1ulp"
+    return {"attributed": reason is not None, "reason": reason,
+            "float_double_gap_rel": float(gap), "ulp_sensitivity_rel": float(sens)}
+
+
+def attributed(isa, prog, inputs, targets, tol_rel):
+    """True when attribution() finds a reason (see there)."""
+    return attribution(isa, prog, inputs, targets, tol_rel)["attributed"]
+
+
+class planted_fault:
+    """Context manager for the oracle's tests: plant one of the GO_FAULT_*
+    mistakes (oracle/galop_oracle.h) in the attribution probes."""
+
+    (NONE, DUAL_UNPROTECTED_DIV, DUAL_LEG_FLOAT, PERTURB_DROPPED, PERTURB_ALL_OPS,
+     DEVSUM_UNCOMPENSATED) = range(6)
+
+    def __init__(self, code):
+        self.code = code
+
+    def __enter__(self):
+        lib().go_plant_fault(self.code)
+        return self
+
+    def __exit__(self, *a):
+        lib().go_plant_fault(0)
 
 
 class unary_math:
@@ -235,6 +285,35 @@ class unary_math:
 
     def __exit__(self, *a):
         lib().go_set_unary_math(None, None, None, None)
+
+
+def sse_device_order(outputs, targets):
+    """SSE of per-case outputs in the GPU kernels' summation order
+    (go_sse_device_order)."""
+    o = np.ascontiguousarray(outputs, np.float32)
+    y = np.ascontiguousarray(targets, np.float32)
+    return float(lib().go_sse_device_order(_p(o), _p(y), y.size))
+
+
+def eval_csr_device_order(isa, words, offsets, inputs, targets):
+    """eval_csr with the GPU kernels' summation order: (fitness, sse)."""
+    words = np.ascontiguousarray(words, np.uint32)
+    offsets = np.asarray(offsets, np.int64)
+    x = np.ascontiguousarray(inputs, np.float32)
+    if x.ndim == 1:
+        x = x[:, None]
+    y = np.ascontiguousarray(targets, np.float32)
+    n = offsets.size - 1
+    sse = np.empty(n, np.float64)
+    fit = np.empty(n, np.float32)
+    sst = ss_tot(y) if isa.fitness == FITNESS_R2 else 0.0
+    for i in range(n):
+        p = words[offsets[i]:offsets[i + 1]]
+        sse[i] = lib().go_program_sse_device_order(ctypes.byref(isa), _p(p), p.size, _p(x), _p(y),
+                                                   x.shape[1], y.size)
+        fit[i] = (r2_to_fitness(sse[i], sst) if isa.fitness == FITNESS_R2
+                  else lib().go_sse_to_fitness(sse[i], y.size))
+    return fit, sse
 
 
 def ss_tot(targets):

@@ -1,9 +1,19 @@
 // ref_hip_run.cpp -- the reference engine (LGPEngine::evolve_sr) on the
-// engine-only regression configuration, with either its own CpuEvaluator or
-// this repository's HipEvaluator behind EvalBackend (the drop-in).  Built by
-// oracle/Makefile (target `ref`) from the reference sources where they lie,
-// with -DPOPULATION_SIZE_OVERRIDE=1000 -DMAX_PROGRAM_SIZE_OVERRIDE=64
-// -DTOURNAMENT_SIZE_OVERRIDE=4 -DMAX_GENERATIONS_OVERRIDE=50; run only by tests.
+// engine-only regression configuration, with one of three evaluators behind
+// EvalBackend:
+//   cpu            the reference's own CpuEvaluator;
+//   devbits        this repository's checker with the GPU kernels' unary
+//                  functions (lgp_math.h built for the host) and summation
+//                  order (go_eval_dense_device_order): the GPU evaluator's
+//                  fitness bits, computed on the CPU.  The evolve_c1 fixtures
+//                  are written with it, in the container that holds the
+//                  reference; nothing of the reference goes to a GPU machine;
+//   hip, hip-effective
+//                  HipEvaluator, the drop-in itself (needs a GPU; kept so the
+//                  adapter is compiled inside the reference tree).
+// Built by oracle/Makefile (target `ref`) from the reference sources where
+// they lie, with -DPOPULATION_SIZE_OVERRIDE=1000 -DMAX_PROGRAM_SIZE_OVERRIDE=64
+// -DTOURNAMENT_SIZE_OVERRIDE=4 -DMAX_GENERATIONS_OVERRIDE=50; test use only.
 //
 // Dataset: x ~ U[-1,1] (std::mt19937 seed 0), 20 cases; y = outputs of a
 // random 20-instruction program over {+,-,*,/} on 8 registers drawn from the
@@ -15,7 +25,7 @@
 // dataset's N cases) and the final generation: .words.u32 (dense, stride
 // MAX_PROGRAM_SIZE), .lengths.u8, .fitness.f32.
 //
-// usage: ref_hip_run <cpu|hip|hip-effective> <seed> <history.csv> [prefix]
+// usage: ref_hip_run <cpu|devbits|hip|hip-effective> <seed> <history.csv> [prefix]
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -30,6 +40,37 @@
 #include "Interpreter.h"
 #include "LGPConfig.h"
 #include "LGPEngine.h"
+
+extern "C" {
+#include "galop_oracle.h"
+float gm_sinf(float);
+float gm_cosf(float);
+float gm_expf(float);
+float gm_plogf(float);
+}
+
+// The GPU evaluator's fitness bits on the CPU (see the header comment).
+class DeviceBitsEvaluator : public EvalBackend {
+public:
+    DeviceBitsEvaluator() {
+        std::memset(&isa_, 0, sizeof(isa_));
+        isa_.num_registers = LGPConfig::NUM_REGISTERS;
+        isa_.op_set = GO_OPS_REF8;
+        isa_.input_protect = LGPConfig::INPUT_PROTECT ? 1 : 0;
+        for (int i = 0; i < LGPConfig::NUM_CONSTANTS; ++i) isa_.constants[i] = LGPConfig::CONSTANTS[i];
+        go_set_unary_math(gm_sinf, gm_cosf, gm_expf, gm_plogf);
+    }
+    ~DeviceBitsEvaluator() override { go_set_unary_math(nullptr, nullptr, nullptr, nullptr); }
+    void evaluate_population(const uint32_t* instructions, const uint8_t* lengths,
+                             float* fitness_out, int n_programs, const Dataset& ds) override {
+        go_eval_dense_device_order(&isa_, instructions, lengths, LGPConfig::MAX_PROGRAM_SIZE,
+                                   n_programs, ds.inputs.data(), ds.targets.data(), ds.num_inputs,
+                                   ds.N, fitness_out, nullptr);
+    }
+
+private:
+    go_isa isa_;
+};
 
 static Dataset make_dataset() {
     std::mt19937 rng(0);
@@ -72,7 +113,7 @@ static bool write_raw(const std::string& path, const T* p, size_t n) {
 
 int main(int argc, char** argv) {
     if (argc != 4 && argc != 5) {
-        std::fprintf(stderr, "usage: %s <cpu|hip|hip-effective> <seed> <history.csv> [prefix]\n",
+        std::fprintf(stderr, "usage: %s <cpu|devbits|hip|hip-effective> <seed> <history.csv> [prefix]\n",
                      argv[0]);
         return 2;
     }
@@ -80,6 +121,7 @@ int main(int argc, char** argv) {
     const uint32_t seed = (uint32_t)std::stoul(argv[2]);
     std::unique_ptr<EvalBackend> backend;
     if (mode == "cpu") backend = std::make_unique<CpuEvaluator>();
+    else if (mode == "devbits") backend = std::make_unique<DeviceBitsEvaluator>();
     else if (mode == "hip") backend = std::make_unique<HipEvaluator>(HipEvaluator::Layout::Packed);
     else if (mode == "hip-effective")
         backend = std::make_unique<HipEvaluator>(HipEvaluator::Layout::PackedEffective);

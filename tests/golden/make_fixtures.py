@@ -23,10 +23,12 @@ Fixtures:
                      the whole LGPEngine::evolve_sr run of the engine-only
                      regression configuration (1,000 programs, max length 64,
                      tournament 4, 50 generations; oracle/ref_hip_run.cpp)
-                     with this repository's HipEvaluator behind EvalBackend:
-                     the dataset, the per-generation history and the final
-                     generation.  Needs the GPU (--evolve-only --out DIR runs
-                     just this part, from an already built oracle/_ref)
+                     with the GPU evaluator's fitness bits computed on the CPU
+                     behind EvalBackend (mode `devbits`: the checker with the
+                     device's unary functions and summation order): the
+                     dataset, the per-generation history and the final
+                     generation.  Runs here, on the CPU; --evolve-only writes
+                     just these, from an already built oracle/_ref
   engine_seed42.npz  LGPEngine(42).init_evolution() population and the
                      population after evolve_sr on datasets/nguyen1_train.csv
                      (the cuda_diff populations, tests/cuda_diff.cu:57-73,
@@ -129,7 +131,7 @@ def vary(tool, seed, csv, out_name):
     print("wrote", out_name)
 
 
-def evolve(seed, out_dir, mode="hip"):
+def evolve(seed, out_dir, mode="devbits"):
     with tempfile.TemporaryDirectory() as tmp:
         prefix = os.path.join(tmp, "d")
         csv = os.path.join(tmp, "history.csv")
@@ -162,7 +164,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--ref", default="/root/reference")
     ap.add_argument("--evolve-only", action="store_true",
-                    help="only the evolve_c1 fixtures (needs the GPU and oracle/_ref)")
+                    help="only the evolve_c1 fixtures (from an already built oracle/_ref)")
     ap.add_argument("--out", default=HERE)
     args = ap.parse_args()
     if args.evolve_only:
@@ -181,6 +183,8 @@ def main():
     csv = os.path.join(args.ref, "datasets", "nguyen1_train.csv")
     vary("ref_dump_r8", 7, csv, "vary_default.npz")
     vary("ref_dump_em", 8, csv, "vary_em.npz")
+    for seed in (3, 11):
+        evolve(seed, args.out)
     return 0
 
 

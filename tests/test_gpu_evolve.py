@@ -3,20 +3,17 @@
 galop_generation_stats_device) against the reference engine's own loop,
 LGPEngine::evolve_sr, on the engine-only regression configuration (1,000
 programs, max length 64, tournament 4, 50 generations, 20 cases), with the
-same fitness evaluator (this repository's HipEvaluator behind EvalBackend):
+same fitness bits (the GPU evaluator's):
 
-* evolve_c1_s<seed>.npz (written by tests/golden/make_fixtures.py from the
-  reference engine): every generation's best and mean fitness (bitwise),
-  best length and mean length, and the final generation's programs, lengths
-  and fitness, all equal;
-* where oracle/_ref/ref_hip_run is built, the same against a live run of
-  the reference engine on another seed;
+* evolve_c1_s<seed>.npz (written on the CPU by tests/golden/make_fixtures.py
+  from the reference engine with the GPU evaluator's fitness bits, see
+  tests/test_oracle_evolve_fixtures.py): every generation's best and mean
+  fitness (bitwise), best length and mean length, and the final generation's
+  programs, lengths and fitness, all equal;
 * the parallel initializer of the effective-mutation build gives
   vary_em.npz's initial population from the engine seed alone.
 """
 import os
-import subprocess
-import tempfile
 
 import numpy as np
 import pytest
@@ -25,7 +22,6 @@ from conftest import REPO, load_fixture
 
 pytestmark = pytest.mark.gpu
 
-DRIVER = os.path.join(REPO, "oracle", "_ref", "ref_hip_run")
 GENERATIONS = 50
 
 
@@ -66,33 +62,6 @@ def test_device_loop_matches_reference_engine_fixture(gpu_ctx, seed):
         pytest.skip(f"{name} not written yet (tests/golden/make_fixtures.py --evolve-only)")
     ref = load_fixture(name)
     hist, pop = _run_device(gpu_ctx, ref["inputs"], ref["targets"], seed)
-    _check(hist, pop, ref)
-
-
-@pytest.mark.skipif(not os.path.exists(DRIVER), reason="ref_hip_run needs the reference sources")
-def test_device_loop_matches_live_reference_engine(gpu_ctx):
-    seed = 5
-    with tempfile.TemporaryDirectory() as tmp:
-        csv, prefix = os.path.join(tmp, "h.csv"), os.path.join(tmp, "d")
-        r = subprocess.run([DRIVER, "hip", str(seed), csv, prefix], capture_output=True,
-                           text=True, timeout=300, env=dict(os.environ, OMP_NUM_THREADS="4"))
-        print(r.stdout, r.stderr)
-        assert r.returncode == 0
-        rows = [line.strip().split(",") for line in open(csv).readlines()[1:]]
-        lengths = np.fromfile(prefix + ".lengths.u8", np.uint8)
-        dense = np.fromfile(prefix + ".words.u32", np.uint32).reshape(lengths.size, -1)
-        ref = dict(
-            generation=np.array([int(x[0]) for x in rows]),
-            best_fitness=np.array([float(x[1]) for x in rows], np.float32),
-            mean_fitness=np.array([float(x[2]) for x in rows], np.float32),
-            best_length=np.array([int(x[3]) for x in rows]),
-            mean_length_text=np.array([x[4] for x in rows]),
-            final_lengths=lengths,
-            final_fitness=np.fromfile(prefix + ".fitness.f32", np.float32),
-            final_words=np.concatenate([dense[i, :lengths[i]] for i in range(lengths.size)]))
-        inputs = np.fromfile(prefix + ".inputs.f32", np.float32)
-        targets = np.fromfile(prefix + ".targets.f32", np.float32)
-    hist, pop = _run_device(gpu_ctx, inputs, targets, seed)
     _check(hist, pop, ref)
 
 
