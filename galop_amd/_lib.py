@@ -26,6 +26,7 @@ FITNESS_R2 = 1
 KERNEL_PLAIN = 0
 KERNEL_BATCHED = 1
 KERNEL_EFFECTIVE = 2
+LAYOUT_DENSE = 0
 LAYOUT_PACKED = 1
 LAYOUT_EFFECTIVE = 2
 BATCH_STRIP_INTRONS = 1
@@ -77,6 +78,10 @@ SIGNATURES = {
     "galop_last_error": (ctypes.c_char_p, [_vp]),
     "galop_ctx_set_tuning": (ctypes.c_int, [_vp, _i32, _i32]),
     "galop_ctx_stream": (_vp, [_vp]),
+    "galop_ctx_phase_seconds": (ctypes.c_int, [_vp, _pf64, _pf64, _pf64, _pf64, _pf64]),
+    "galop_ctx_last_footprint": (ctypes.c_int, [_vp, _pi64, _pi64]),
+    "galop_device_count": (_i32, []),
+    "galop_device_info": (ctypes.c_int, [_i32, ctypes.c_char_p, _i32, _pi32, _pi32]),
     "galop_dataset_create": (ctypes.c_int, [_vp, ctypes.POINTER(galop_isa), _vp, _vp, _i64, _i32,
                                             ctypes.POINTER(_vp)]),
     "galop_dataset_destroy": (None, [_vp]),

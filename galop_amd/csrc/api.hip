@@ -108,6 +108,13 @@ int galop_ctx_create(int32_t device, galop_ctx** out) {
         delete ctx;
         return hip_fail(nullptr, e, "hipStreamCreateWithFlags");
     }
+    for (auto& ev : ctx->ev) {
+        if (e == hipSuccess) e = hipEventCreate(&ev);
+    }
+    if (e != hipSuccess) {
+        galop_ctx_destroy(ctx);
+        return hip_fail(nullptr, e, "hipEventCreate");
+    }
     *out = ctx;
     return GALOP_OK;
 }
@@ -117,8 +124,53 @@ void galop_ctx_destroy(galop_ctx* ctx) {
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
     ctx->staging.release();
+    for (auto& ev : ctx->ev)
+        if (ev) hipEventDestroy(ev);
     hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+int galop_ctx_phase_seconds(const galop_ctx* ctx, double* pack, double* alloc, double* h2d,
+                            double* kernel, double* d2h) {
+    if (!ctx) return set_error(nullptr, GALOP_ERR_INVALID, "galop_ctx_phase_seconds: NULL ctx");
+    if (pack) *pack = ctx->pack_s;
+    if (alloc) *alloc = ctx->alloc_s;
+    if (h2d) *h2d = ctx->h2d_s;
+    if (kernel) *kernel = ctx->kernel_s;
+    if (d2h) *d2h = ctx->d2h_s;
+    return GALOP_OK;
+}
+
+int galop_ctx_last_footprint(const galop_ctx* ctx, int64_t* words, int64_t* effective_words) {
+    if (!ctx) return set_error(nullptr, GALOP_ERR_INVALID, "galop_ctx_last_footprint: NULL ctx");
+    if (words) *words = ctx->fp_words;
+    if (effective_words) *effective_words = ctx->fp_effective;
+    return GALOP_OK;
+}
+
+int32_t galop_device_count(void) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess) return 0;
+    return count;
+}
+
+int galop_device_info(int32_t device, char* name, int32_t name_len, int32_t* n_cu,
+                      int32_t* clock_khz) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+        return set_error(nullptr, GALOP_ERR_NODEVICE, "galop_device_info: no HIP device");
+    if (device < 0 || device >= count)
+        return set_error(nullptr, GALOP_ERR_INVALID, "galop_device_info: bad device index");
+    hipDeviceProp_t prop;
+    const hipError_t e = hipGetDeviceProperties(&prop, device);
+    if (e != hipSuccess) return hip_fail(nullptr, e, "hipGetDeviceProperties");
+    if (name && name_len > 0) {
+        std::strncpy(name, prop.gcnArchName, (size_t)name_len - 1);
+        name[name_len - 1] = '\0';
+    }
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    if (clock_khz) *clock_khz = prop.clockRate;
+    return GALOP_OK;
 }
 
 int galop_ctx_set_tuning(galop_ctx* ctx, int32_t cases_per_lane, int32_t waves_per_block) {
@@ -297,38 +349,98 @@ static hipError_t split_scratch(galop_ctx* ctx, const galop_dataset* ds, int64_t
     return e;
 }
 
-static int eval_host_csr(galop_ctx* ctx, const galop_dataset* ds, int64_t n, int64_t total_words,
-                         bool strip, float* fitness_out, double* sse_out) {
+using SteadyClock = std::chrono::steady_clock;
+
+static double seconds_since(SteadyClock::time_point t0) {
+    return std::chrono::duration<double>(SteadyClock::now() - t0).count();
+}
+
+static double event_seconds(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.0f;
+    return hipEventElapsedTime(&ms, a, b) == hipSuccess ? ms / 1e3 : 0.0;
+}
+
+// How the staged programs are laid out for eval_staged.
+struct StagedLayout {
+    int32_t dense_stride = 0;  // > 0: n slots of dense_stride words + u8 lengths
+    bool strip = false;        // CSR: remove introns on the device first
+    bool count = false;        // CSR: count the effective words (footprint)
+};
+
+// The programs are in the pinned staging buffers (h_words with h_offsets, or
+// the dense slots with h_lengths): upload, evaluate, download; phase times
+// and the footprint go to the context.  Shared by the dense and CSR calls.
+static int eval_staged(galop_ctx* ctx, const galop_dataset* ds, int64_t n, int64_t upload_words,
+                       StagedLayout lay, float* fitness_out, double* sse_out) {
     Staging& st = ctx->staging;
     hipStream_t s = ctx->stream;
-    GALOP_HIP(ctx, st.d_words.reserve((size_t)std::max<int64_t>(total_words, 1)));
-    GALOP_HIP(ctx, st.d_offsets.reserve((size_t)n + 1));
+    const auto t_alloc = SteadyClock::now();
+    GALOP_HIP(ctx, st.d_words.reserve((size_t)std::max<int64_t>(upload_words, 1)));
+    if (lay.dense_stride > 0) GALOP_HIP(ctx, st.d_lengths.reserve((size_t)n));
+    else GALOP_HIP(ctx, st.d_offsets.reserve((size_t)n + 1));
     GALOP_HIP(ctx, st.d_fitness.reserve((size_t)std::max<int64_t>(n, 1)));
     GALOP_HIP(ctx, st.h_fitness.reserve((size_t)std::max<int64_t>(n, 1)));
+    GALOP_HIP(ctx, st.h_small.reserve(1));
     if (sse_out) {
         GALOP_HIP(ctx, st.d_sse.reserve((size_t)std::max<int64_t>(n, 1)));
         GALOP_HIP(ctx, st.h_sse.reserve((size_t)std::max<int64_t>(n, 1)));
     }
-    GALOP_HIP(ctx, hipMemcpyAsync(st.d_words.ptr, st.h_words.ptr, (size_t)total_words * sizeof(uint32_t),
-                                  hipMemcpyHostToDevice, s));
-    GALOP_HIP(ctx, hipMemcpyAsync(st.d_offsets.ptr, st.h_offsets.ptr, (size_t)(n + 1) * sizeof(int64_t),
-                                  hipMemcpyHostToDevice, s));
-    const uint32_t* w = st.d_words.ptr;
-    const int64_t* o = st.d_offsets.ptr;
-    if (strip) {
-        const int rc = strip_into_staging(ctx, ds, w, o, n, total_words, s, &w, &o);
-        if (rc != GALOP_OK) return rc;
-    }
     double* partials = nullptr;
     GALOP_HIP(ctx, split_scratch(ctx, ds, n, &partials));
-    GALOP_HIP(ctx, launch_plain(ds->dev, ds->isa.num_registers, ctx->tuning, w, o, n, st.d_fitness.ptr,
-                                sse_out ? st.d_sse.ptr : nullptr, partials, ctx->n_cu, s));
+    if (lay.strip || lay.count) {
+        GALOP_HIP(ctx, st.d_counts.reserve((size_t)n + 1));
+        GALOP_HIP(ctx, st.d_scan_tmp.reserve((size_t)scan_temp_elems(n)));
+        GALOP_HIP(ctx, st.d_eff_offsets.reserve((size_t)n + 1));
+        if (lay.strip) GALOP_HIP(ctx, st.d_eff_words.reserve((size_t)std::max<int64_t>(upload_words, 1)));
+    }
+    ctx->alloc_s += seconds_since(t_alloc);
+
+    GALOP_HIP(ctx, hipEventRecord(ctx->ev[0], s));
+    GALOP_HIP(ctx, hipMemcpyAsync(st.d_words.ptr, st.h_words.ptr, (size_t)upload_words * sizeof(uint32_t),
+                                  hipMemcpyHostToDevice, s));
+    if (lay.dense_stride > 0)
+        GALOP_HIP(ctx, hipMemcpyAsync(st.d_lengths.ptr, st.h_lengths.ptr, (size_t)n,
+                                      hipMemcpyHostToDevice, s));
+    else
+        GALOP_HIP(ctx, hipMemcpyAsync(st.d_offsets.ptr, st.h_offsets.ptr, (size_t)(n + 1) * sizeof(int64_t),
+                                      hipMemcpyHostToDevice, s));
+    GALOP_HIP(ctx, hipEventRecord(ctx->ev[1], s));
+    const uint32_t* w = st.d_words.ptr;
+    const int64_t* o = st.d_offsets.ptr;
+    const galop_isa& isa = ds->isa;
+    if (lay.strip) {
+        const int rc = strip_into_staging(ctx, ds, w, o, n, upload_words, s, &w, &o);
+        if (rc != GALOP_OK) return rc;
+    } else if (lay.count) {
+        GALOP_HIP(ctx, launch_effective_counts(isa.num_registers, isa.op_set, isa.input_protect, w, o,
+                                               n, st.d_counts.ptr, s));
+        GALOP_HIP(ctx, launch_exclusive_scan(st.d_counts.ptr, n, st.d_eff_offsets.ptr,
+                                             st.d_scan_tmp.ptr, s));
+    }
+    GALOP_HIP(ctx, hipEventRecord(ctx->ev[2], s));
+    if (lay.dense_stride > 0)
+        GALOP_HIP(ctx, launch_plain(ds->dev, isa.num_registers, ctx->tuning, w, nullptr, n,
+                                    st.d_fitness.ptr, sse_out ? st.d_sse.ptr : nullptr, partials,
+                                    ctx->n_cu, s, st.d_lengths.ptr, lay.dense_stride));
+    else
+        GALOP_HIP(ctx, launch_plain(ds->dev, isa.num_registers, ctx->tuning, w, o, n, st.d_fitness.ptr,
+                                    sse_out ? st.d_sse.ptr : nullptr, partials, ctx->n_cu, s));
+    GALOP_HIP(ctx, hipEventRecord(ctx->ev[3], s));
     GALOP_HIP(ctx, hipMemcpyAsync(st.h_fitness.ptr, st.d_fitness.ptr, (size_t)n * sizeof(float),
                                   hipMemcpyDeviceToHost, s));
     if (sse_out)
         GALOP_HIP(ctx, hipMemcpyAsync(st.h_sse.ptr, st.d_sse.ptr, (size_t)n * sizeof(double),
                                       hipMemcpyDeviceToHost, s));
+    if (lay.strip || lay.count)
+        GALOP_HIP(ctx, hipMemcpyAsync(st.h_small.ptr, st.d_eff_offsets.ptr + n, sizeof(int64_t),
+                                      hipMemcpyDeviceToHost, s));
+    GALOP_HIP(ctx, hipEventRecord(ctx->ev[4], s));
     GALOP_HIP(ctx, hipStreamSynchronize(s));
+    ctx->h2d_s += event_seconds(ctx->ev[0], ctx->ev[1]);
+    ctx->pack_s += event_seconds(ctx->ev[1], ctx->ev[2]);
+    ctx->kernel_s += event_seconds(ctx->ev[2], ctx->ev[3]);
+    ctx->d2h_s += event_seconds(ctx->ev[3], ctx->ev[4]);
+    if (lay.strip || lay.count) ctx->fp_effective = st.h_small.ptr[0];
     std::memcpy(fitness_out, st.h_fitness.ptr, (size_t)n * sizeof(float));
     if (sse_out) std::memcpy(sse_out, st.h_sse.ptr, (size_t)n * sizeof(double));
     return GALOP_OK;
@@ -342,31 +454,56 @@ int galop_eval_dense(galop_ctx* ctx, const galop_dataset* ds, const uint32_t* in
     if (n_programs == 0) return GALOP_OK;
     if (!instructions || !lengths || !fitness_out)
         return set_error(ctx, GALOP_ERR_INVALID, "galop_eval_dense: NULL buffer");
-    if (layout != GALOP_LAYOUT_PACKED && layout != GALOP_LAYOUT_EFFECTIVE)
-        return set_error(ctx, GALOP_ERR_INVALID, "galop_eval_dense: layout must be PACKED or EFFECTIVE");
+    if (layout != GALOP_LAYOUT_DENSE && layout != GALOP_LAYOUT_PACKED &&
+        layout != GALOP_LAYOUT_EFFECTIVE)
+        return set_error(ctx, GALOP_ERR_INVALID,
+                         "galop_eval_dense: layout must be DENSE, PACKED or EFFECTIVE");
     if (stride <= 0) return set_error(ctx, GALOP_ERR_INVALID, "galop_eval_dense: stride <= 0");
+    if (layout == GALOP_LAYOUT_DENSE && stride > 255)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_eval_dense: DENSE layout needs stride <= 255");
     for (int64_t i = 0; i < n_programs; ++i)
         if ((int)lengths[i] > stride)
             return set_error(ctx, GALOP_ERR_INVALID,
                              "galop_eval_dense: lengths[" + std::to_string(i) + "] > stride");
     GALOP_HIP(ctx, hipSetDevice(ctx->device));
     Staging& st = ctx->staging;
-    GALOP_HIP(ctx, st.h_offsets.reserve((size_t)n_programs + 1));
-    // two-phase pack (src/EvaluatorCuda.cu:335-372): prefix of lengths, then
-    // parallel copy of each program's active words
-    int64_t* off = st.h_offsets.ptr;
-    off[0] = 0;
-    for (int64_t i = 0; i < n_programs; ++i) off[i + 1] = off[i] + lengths[i];
-    const int64_t total = off[n_programs];
-    GALOP_HIP(ctx, st.h_words.reserve((size_t)std::max<int64_t>(total, 1)));
-    uint32_t* hw = st.h_words.ptr;
-    parallel_chunks(n_programs, 1 << 14, [&](int64_t lo, int64_t hi) {
-        for (int64_t i = lo; i < hi; ++i)
-            std::memcpy(hw + off[i], instructions + (size_t)i * (size_t)stride,
-                        (size_t)lengths[i] * sizeof(uint32_t));
-    });
-    return eval_host_csr(ctx, ds, n_programs, total, layout == GALOP_LAYOUT_EFFECTIVE, fitness_out,
-                         nullptr);
+    const auto t_pack = SteadyClock::now();
+    StagedLayout lay;
+    int64_t upload = 0, total = 0;
+    if (layout == GALOP_LAYOUT_DENSE) {
+        // the slots as they are (src/EvaluatorCuda.cu:323-333): one copy into
+        // pinned staging, no packing
+        upload = n_programs * (int64_t)stride;
+        GALOP_HIP(ctx, st.h_words.reserve((size_t)upload));
+        GALOP_HIP(ctx, st.h_lengths.reserve((size_t)n_programs));
+        parallel_chunks(upload, int64_t(1) << 22, [&](int64_t lo, int64_t hi) {
+            std::memcpy(st.h_words.ptr + lo, instructions + lo, (size_t)(hi - lo) * sizeof(uint32_t));
+        });
+        std::memcpy(st.h_lengths.ptr, lengths, (size_t)n_programs);
+        for (int64_t i = 0; i < n_programs; ++i) total += lengths[i];
+        lay.dense_stride = stride;
+    } else {
+        GALOP_HIP(ctx, st.h_offsets.reserve((size_t)n_programs + 1));
+        // two-phase pack (src/EvaluatorCuda.cu:335-372): prefix of lengths, then
+        // parallel copy of each program's active words
+        int64_t* off = st.h_offsets.ptr;
+        off[0] = 0;
+        for (int64_t i = 0; i < n_programs; ++i) off[i + 1] = off[i] + lengths[i];
+        total = upload = off[n_programs];
+        GALOP_HIP(ctx, st.h_words.reserve((size_t)std::max<int64_t>(total, 1)));
+        uint32_t* hw = st.h_words.ptr;
+        parallel_chunks(n_programs, 1 << 14, [&](int64_t lo, int64_t hi) {
+            for (int64_t i = lo; i < hi; ++i)
+                std::memcpy(hw + off[i], instructions + (size_t)i * (size_t)stride,
+                            (size_t)lengths[i] * sizeof(uint32_t));
+        });
+        lay.strip = layout == GALOP_LAYOUT_EFFECTIVE;
+        lay.count = !lay.strip;
+    }
+    ctx->pack_s += seconds_since(t_pack);
+    ctx->fp_words = total;
+    ctx->fp_effective = 0;
+    return eval_staged(ctx, ds, n_programs, upload, lay, fitness_out, nullptr);
 }
 
 int galop_eval_csr(galop_ctx* ctx, const galop_dataset* ds, const uint32_t* words,
@@ -385,9 +522,14 @@ int galop_eval_csr(galop_ctx* ctx, const galop_dataset* ds, const uint32_t* word
     Staging& st = ctx->staging;
     GALOP_HIP(ctx, st.h_offsets.reserve((size_t)n_programs + 1));
     GALOP_HIP(ctx, st.h_words.reserve((size_t)std::max<int64_t>(total, 1)));
+    const auto t_pack = SteadyClock::now();
     for (int64_t i = 0; i <= n_programs; ++i) st.h_offsets.ptr[i] = offsets[i] - first;
-    if (total > 0) std::memcpy(st.h_words.ptr, words + first, (size_t)total * sizeof(uint32_t));
-    return eval_host_csr(ctx, ds, n_programs, total, false, fitness_out, sse_out);
+    if (total > 0)
+        parallel_chunks(total, int64_t(1) << 22, [&](int64_t lo, int64_t hi) {
+            std::memcpy(st.h_words.ptr + lo, words + first + lo, (size_t)(hi - lo) * sizeof(uint32_t));
+        });
+    ctx->pack_s += seconds_since(t_pack);
+    return eval_staged(ctx, ds, n_programs, total, StagedLayout{}, fitness_out, sse_out);
 }
 
 int galop_eval_csr_device(galop_ctx* ctx, const galop_dataset* ds, const uint32_t* d_words,

@@ -100,6 +100,9 @@ struct Staging {
     DeviceBuf<int64_t> d_eff_offsets;
     DeviceBuf<uint32_t> d_eff_words;
     DeviceBuf<double> d_partials;  // case-split block partials
+    DeviceBuf<uint8_t> d_lengths;  // dense layout: program lengths
+    PinnedBuf<uint8_t> h_lengths;
+    PinnedBuf<int64_t> h_small;    // small read-backs (effective word total)
     DeviceBuf<uint64_t> d_keys;    // elite selection: keys, sorted keys
     DeviceBuf<uint64_t> d_keys_sorted;
     DeviceBuf<uint8_t> d_sort_tmp;
@@ -123,6 +126,9 @@ struct Staging {
         d_eff_offsets.release();
         d_eff_words.release();
         d_partials.release();
+        d_lengths.release();
+        h_lengths.release();
+        h_small.release();
         d_keys.release();
         d_keys_sorted.release();
         d_sort_tmp.release();
@@ -152,6 +158,11 @@ struct galop_ctx {
     galop::Staging staging;
     std::mutex err_mu;
     std::string last_error;
+    // phases of the synchronous calls: events around H2D | device packing |
+    // kernel | D2H, and the summed seconds of each phase
+    hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    double pack_s = 0.0, alloc_s = 0.0, h2d_s = 0.0, kernel_s = 0.0, d2h_s = 0.0;
+    int64_t fp_words = 0, fp_effective = 0;  // last galop_eval_dense footprint
 };
 
 struct galop_dataset {

@@ -289,17 +289,34 @@ class BatchPlan:
 
 class HipEvaluator:
     """EvalBackend (src/EvalBackend.h:8-19) over the HIP library, with the
-    reference CudaEvaluator's layout choice (src/EvaluatorCuda.h:24):
-    'packed' (CSR of raw programs) or 'effective' (introns stripped, on the
-    device).  The device dataset is cached by Dataset object identity, like
-    CudaEvaluator::upload_dataset (src/EvaluatorCuda.cu:309-312)."""
+    reference CudaEvaluator's choices (src/EvaluatorCuda.h:11-24): strategy
+    'warp' (warp_block_threads / 64 waves per workgroup) or 'block' (one wave
+    per workgroup), layout 'dense' (the slots as they are), 'packed' (CSR of
+    raw programs) or 'effective' (introns stripped, on the device); fitness is
+    bitwise the same in every combination.  The device dataset is cached by
+    Dataset object identity, like CudaEvaluator::upload_dataset
+    (src/EvaluatorCuda.cu:309-312); phase_seconds() and footprint_log mirror
+    src/EvaluatorCuda.h:36-51."""
 
-    def __init__(self, ctx=None, isa=None, layout="packed", device=0):
+    def __init__(self, ctx=None, isa=None, layout="packed", device=0, strategy="warp",
+                 warp_block_threads=256):
         self.ctx = ctx or Context(device)
         self.isa = isa or Isa()
-        self.layout = {"packed": _lib.LAYOUT_PACKED, "effective": _lib.LAYOUT_EFFECTIVE}[layout]
+        self.layout = {"dense": _lib.LAYOUT_DENSE, "packed": _lib.LAYOUT_PACKED,
+                       "effective": _lib.LAYOUT_EFFECTIVE}[layout]
+        waves = 1 if strategy == "block" else min(4, max(1, warp_block_threads // 64))
+        if strategy not in ("warp", "block"):
+            raise ValueError("strategy must be 'warp' or 'block'")
+        self.ctx.set_tuning(0, waves)
+        self.footprint_log = []
         self._ds_key = None
         self._dds = None
+
+    def phase_seconds(self):
+        """{pack, alloc, h2d, kernel, d2h} seconds summed over the calls."""
+        vals = [ctypes.c_double() for _ in range(5)]
+        _lib.check(self.ctx.lib.galop_ctx_phase_seconds(self.ctx.handle, *[ctypes.byref(v) for v in vals]))
+        return dict(zip(("pack", "alloc", "h2d", "kernel", "d2h"), (v.value for v in vals)))
 
     def _device_dataset(self, ds):
         if self._ds_key is not ds:
@@ -314,6 +331,10 @@ class HipEvaluator:
         stride = instr.shape[1] if instr.ndim == 2 else instr.size // max(len(lengths), 1)
         instr = instr.reshape(-1, stride)[:n_programs]
         fitness_out[:n_programs] = eval_dense(dds, instr, lengths, self.layout)
+        words, eff = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(self.ctx.lib.galop_ctx_last_footprint(self.ctx.handle, ctypes.byref(words),
+                                                         ctypes.byref(eff)))
+        self.footprint_log.append((words.value, eff.value))
 
 
 class Server:

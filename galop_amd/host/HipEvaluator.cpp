@@ -1,6 +1,7 @@
 // HipEvaluator.cpp -- EvalBackend adapter over libgalop_hip.so.
 #include "HipEvaluator.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -32,12 +33,20 @@ galop_isa HipEvaluator::engine_isa() {
     return isa;
 }
 
-HipEvaluator::HipEvaluator(Layout layout, int device) : HipEvaluator(engine_isa(), layout, device) {}
+HipEvaluator::HipEvaluator(Strategy strategy, int warp_block_threads, Layout layout)
+    : HipEvaluator(engine_isa(), strategy, warp_block_threads, layout, 0) {}
 
-HipEvaluator::HipEvaluator(const galop_isa& isa, Layout layout, int device)
-    : isa_(isa), layout_(layout), stride_(kStride) {
-    const int rc = galop_ctx_create(device, &ctx_);
+HipEvaluator::HipEvaluator(const galop_isa& isa, Strategy strategy, int warp_block_threads,
+                           Layout layout, int device)
+    : isa_(isa), strategy_(strategy), layout_(layout), stride_(kStride) {
+    int rc = galop_ctx_create(device, &ctx_);
     if (rc != GALOP_OK) fail(rc, "galop_ctx_create");
+    // waves per workgroup: Block = 1; Warp = threads / 64, within [1, 4]
+    const int waves = strategy_ == Strategy::Block
+                          ? 1
+                          : std::min(4, std::max(1, warp_block_threads / 64));
+    rc = galop_ctx_set_tuning(ctx_, 0, waves);
+    if (rc != GALOP_OK) fail(rc, "galop_ctx_set_tuning");
 }
 
 HipEvaluator::~HipEvaluator() {
@@ -48,6 +57,12 @@ HipEvaluator::~HipEvaluator() {
 void HipEvaluator::fail(int code, const char* what) const {
     std::fprintf(stderr, "HIP evaluator error %d in %s: %s\n", code, what, galop_last_error(ctx_));
     std::exit(code);
+}
+
+double HipEvaluator::phase(int i) const {
+    double t[5] = {0, 0, 0, 0, 0};
+    galop_ctx_phase_seconds(ctx_, &t[0], &t[1], &t[2], &t[3], &t[4]);
+    return t[i];
 }
 
 void HipEvaluator::evaluate_population(const uint32_t* instructions, const uint8_t* lengths,
@@ -61,11 +76,16 @@ void HipEvaluator::evaluate_population(const uint32_t* instructions, const uint8
         if (rc != GALOP_OK) fail(rc, "galop_dataset_create");
         uploaded_ds_ = &ds;
     }
-    const int layout =
-        layout_ == Layout::PackedEffective ? GALOP_LAYOUT_EFFECTIVE : GALOP_LAYOUT_PACKED;
-    const int rc = galop_eval_dense(ctx_, ds_, instructions, lengths, stride_, n_programs,
-                                    fitness_out, layout);
+    const int layout = layout_ == Layout::Dense    ? GALOP_LAYOUT_DENSE
+                       : layout_ == Layout::Packed ? GALOP_LAYOUT_PACKED
+                                                   : GALOP_LAYOUT_EFFECTIVE;
+    int rc = galop_eval_dense(ctx_, ds_, instructions, lengths, stride_, n_programs, fitness_out,
+                              layout);
     if (rc != GALOP_OK) fail(rc, "galop_eval_dense");
+    int64_t words = 0, eff = 0;
+    rc = galop_ctx_last_footprint(ctx_, &words, &eff);
+    if (rc != GALOP_OK) fail(rc, "galop_ctx_last_footprint");
+    footprint_log_.emplace_back((long long)words, (long long)eff);
     eval_seconds_ +=
         std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }

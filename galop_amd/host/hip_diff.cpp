@@ -1,9 +1,10 @@
 // hip_diff.cpp -- drives HipEvaluator through the EvalBackend interface the
 // way the engine does (LGPEngine::evaluate_range, src/LGPEngine.cpp:132-144:
 // pointers offset past the elites, a count smaller than the population) and
-// the way cuda_diff does (tests/cuda_diff.cu:83-87,184-203: fresh backend per
-// layout, bitwise layout identity).  Writes the fitness arrays for the test to
-// compare with the oracle.
+// the way cuda_diff does (tests/cuda_diff.cu:83-87,184-203: a fresh backend
+// for each strategy x layout, bitwise identity across them), and checks the
+// footprint log and the phase timers.  Writes the fitness arrays for the test
+// to compare with the oracle.
 //
 // usage: hip_diff <dense.u32> <lengths.u8> <inputs.f32> <targets.f32>
 //                 <n_inputs> <elite_count> <out_prefix>
@@ -67,12 +68,28 @@ int main(int argc, char** argv) {
     std::memcpy(ds.inputs.data(), x.data(), x.size() * sizeof(float));
     std::memcpy(ds.targets.data(), y.data(), y.size() * sizeof(float));
 
+    // the strategy x layout grid of tests/cuda_diff.cu:185-189
+    struct Variant {
+        const char* name;
+        HipEvaluator::Strategy strategy;
+        HipEvaluator::Layout layout;
+    };
+    const Variant variants[] = {
+        {"warp/dense", HipEvaluator::Strategy::Warp, HipEvaluator::Layout::Dense},
+        {"warp/packed", HipEvaluator::Strategy::Warp, HipEvaluator::Layout::Packed},
+        {"warp/effect", HipEvaluator::Strategy::Warp, HipEvaluator::Layout::PackedEffective},
+        {"block/dense", HipEvaluator::Strategy::Block, HipEvaluator::Layout::Dense},
+        {"block/packed", HipEvaluator::Strategy::Block, HipEvaluator::Layout::Packed},
+    };
     bool ok = true;
-    std::vector<float> by_layout[2];
-    for (int li = 0; li < 2; ++li) {
-        std::unique_ptr<EvalBackend> backend = std::make_unique<HipEvaluator>(
-            li == 0 ? HipEvaluator::Layout::Packed : HipEvaluator::Layout::PackedEffective);
-        const int stride = static_cast<HipEvaluator*>(backend.get())->program_stride();
+    std::vector<float> first;
+    long long sum_l = 0;
+    for (int i = 0; i < n; ++i) sum_l += lengths[i];
+    for (const Variant& v : variants) {
+        auto eval = std::make_unique<HipEvaluator>(v.strategy, 256, v.layout);
+        HipEvaluator* hp = eval.get();
+        std::unique_ptr<EvalBackend> backend = std::move(eval);
+        const int stride = hp->program_stride();
         if ((size_t)n * stride != dense.size()) {
             std::fprintf(stderr, "dense buffer is not n x %d words\n", stride);
             return 2;
@@ -86,16 +103,32 @@ int main(int argc, char** argv) {
         backend->evaluate_population(dense.data() + (size_t)elites * stride, lengths.data() + elites,
                                      win.data() + elites, n - elites, ds);
         const bool same = std::memcmp(fit.data(), win.data(), (size_t)n * sizeof(float)) == 0;
-        std::printf("[%s] window [%d, %d) %s\n", li == 0 ? "packed" : "effective", elites, n,
-                    same ? "BITWISE IDENTICAL" : "DIFFERS");
-        ok &= same;
-        by_layout[li] = fit;
+        const auto& log = hp->footprint_log();
+        const bool fp_ok = log.size() == 2 && log[0].first == sum_l &&
+                           (v.layout == HipEvaluator::Layout::Dense
+                                ? log[0].second == 0
+                                : (log[0].second > 0 && log[0].second <= sum_l));
+        const double split = hp->pack_seconds() + hp->alloc_seconds() + hp->h2d_seconds() +
+                             hp->kernel_seconds() + hp->d2h_seconds();
+        const bool timers_ok = hp->kernel_seconds() > 0.0 && split <= hp->eval_seconds() * 1.01;
+        std::printf("[%-12s] window [%d, %d) %s; footprint {%lld, %lld}%s; eval %.4fs = pack %.4f "
+                    "alloc %.4f h2d %.4f kernel %.4f d2h %.4f + other%s\n",
+                    v.name, elites, n, same ? "BITWISE IDENTICAL" : "DIFFERS", log[0].first,
+                    log[0].second, fp_ok ? "" : " WRONG", hp->eval_seconds(), hp->pack_seconds(),
+                    hp->alloc_seconds(), hp->h2d_seconds(), hp->kernel_seconds(),
+                    hp->d2h_seconds(), timers_ok ? "" : " (timers WRONG)");
+        ok &= same && fp_ok && timers_ok;
+        if (first.empty()) {
+            first = fit;
+        } else {
+            const bool eq = std::memcmp(first.data(), fit.data(), (size_t)n * 4) == 0;
+            std::printf("[%-12s] == warp/dense: %s\n", v.name, eq ? "BITWISE IDENTICAL" : "DIFFERS");
+            ok &= eq;
+        }
+        if (v.layout != HipEvaluator::Layout::Dense && v.strategy == HipEvaluator::Strategy::Warp)
+            write_all(out + (v.layout == HipEvaluator::Layout::Packed ? ".packed.f32" : ".effective.f32"),
+                      fit);
     }
-    const bool same = std::memcmp(by_layout[0].data(), by_layout[1].data(), (size_t)n * 4) == 0;
-    std::printf("[packed==effective] %s\n", same ? "BITWISE IDENTICAL" : "DIFFERS");
-    ok &= same;
-    write_all(out + ".packed.f32", by_layout[0]);
-    write_all(out + ".effective.f32", by_layout[1]);
     std::printf(ok ? "PASS\n" : "FAIL\n");
     return ok ? 0 : 1;
 }

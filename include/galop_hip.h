@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define GALOP_ABI_VERSION 3
+#define GALOP_ABI_VERSION 4
 
 /* ---- status codes ---- */
 #define GALOP_OK            0
@@ -58,7 +58,10 @@ extern "C" {
                                     most 256 words; longer ones run whole.
                                     Fitness bitwise that of PLAIN.          */
 
-/* ---- dense layouts (CudaEvaluator::Layout, src/EvaluatorCuda.h:24) ---- */
+/* ---- layouts of the engine's dense buffer (CudaEvaluator::Layout,
+ * src/EvaluatorCuda.h:17-24) ---- */
+#define GALOP_LAYOUT_DENSE     0 /* the P x stride slots as the engine holds
+                                    them, u8 lengths (galop_eval_dense only) */
 #define GALOP_LAYOUT_PACKED    1 /* CSR of the raw programs                 */
 #define GALOP_LAYOUT_EFFECTIVE 2 /* CSR of intron-stripped programs         */
 
@@ -101,6 +104,27 @@ const char* galop_last_error(const galop_ctx* ctx); /* ctx may be NULL */
 int         galop_ctx_set_tuning(galop_ctx* ctx, int32_t cases_per_lane,
                                  int32_t waves_per_block);
 void*       galop_ctx_stream(galop_ctx* ctx);
+/* Seconds spent, summed over every synchronous call on this context
+ * (galop_eval_dense, galop_eval_csr), in the five phases of
+ * CudaEvaluator::{pack,alloc,h2d,kernel,d2h}_seconds (src/EvaluatorCuda.h:36-45):
+ * host packing (plus, for the effective layout, the device intron removal),
+ * staging growth, host-to-device copy, kernel, fitness download.  Any output
+ * may be NULL. */
+int         galop_ctx_phase_seconds(const galop_ctx* ctx, double* pack, double* alloc,
+                                    double* h2d, double* kernel, double* d2h);
+/* Footprint of the last galop_eval_dense call on this context, as one entry
+ * of CudaEvaluator::footprint_log (src/EvaluatorCuda.h:47-51): the words of
+ * the programs (sum of lengths) and, for the packed layouts, how many of them
+ * are effective (src/Introns.h:28-46; counted on the device); 0 for DENSE. */
+int         galop_ctx_last_footprint(const galop_ctx* ctx, int64_t* words,
+                                     int64_t* effective_words);
+
+/* ---- devices (for the drop-in's Is_GPU_present, src/cuda_errors.h:17-39) */
+int32_t     galop_device_count(void); /* 0 when no HIP device is visible   */
+/* Name (e.g. "gfx950"), compute units and peak clock of a device; name is
+ * NUL-terminated within name_len.  Outputs may be NULL. */
+int         galop_device_info(int32_t device, char* name, int32_t name_len,
+                              int32_t* n_cu, int32_t* clock_khz);
 
 /* ---- datasets ------------------------------------------------------------
  * inputs: case-major AoS inputs[c*n_inputs+i] (src/Dataset.h:22-40, unpadded
@@ -117,8 +141,10 @@ int64_t galop_dataset_device_bytes(const galop_dataset* ds);
 /* ---- synchronous host-buffer entry points --------------------------------
  * galop_eval_dense: EvalBackend::evaluate_population (src/EvalBackend.h:11-17)
  * with the dense program-major buffer of the engine: program i at
- * instructions + i*stride, lengths[i] words.  layout = GALOP_LAYOUT_PACKED or
- * GALOP_LAYOUT_EFFECTIVE (src/EvaluatorCuda.cu:335-422).  Blocking. */
+ * instructions + i*stride, lengths[i] <= stride words.  layout =
+ * GALOP_LAYOUT_DENSE (the slots as they are, stride <= 255),
+ * GALOP_LAYOUT_PACKED or GALOP_LAYOUT_EFFECTIVE (src/EvaluatorCuda.cu:335-422).
+ * Fitness is bitwise the same in every layout.  Blocking. */
 int galop_eval_dense(galop_ctx* ctx, const galop_dataset* ds,
                      const uint32_t* instructions, const uint8_t* lengths,
                      int32_t stride, int64_t n_programs, float* fitness_out,
@@ -199,9 +225,10 @@ int  galop_batch_autotune(galop_batch* b, const uint32_t* d_words,
 /* ---- serving: many evolution runs, continuous batching -----------------
  * A server keeps datasets and populations resident, admits requests while
  * their device bytes fit under hbm_budget_bytes, and overlaps admitted
- * requests on n_streams HIP streams.  galop_submit copies the population to
- * device memory and returns at once; fitness_out/sse_out must stay valid
- * until galop_wait reports completion. */
+ * requests on n_streams HIP streams.  galop_submit returns at once; the
+ * population is read later, when the request is admitted, so words, offsets,
+ * fitness_out and sse_out must all stay valid and unchanged until galop_wait
+ * (or galop_poll) reports the request done. */
 int  galop_server_create(galop_ctx* ctx, int64_t hbm_budget_bytes,
                          int32_t n_streams, galop_server** out);
 void galop_server_destroy(galop_server* s);

@@ -122,9 +122,10 @@ int main(int argc, char** argv) {
     std::unique_ptr<EvalBackend> backend;
     if (mode == "cpu") backend = std::make_unique<CpuEvaluator>();
     else if (mode == "devbits") backend = std::make_unique<DeviceBitsEvaluator>();
-    else if (mode == "hip") backend = std::make_unique<HipEvaluator>(HipEvaluator::Layout::Packed);
+    else if (mode == "hip") backend = std::make_unique<HipEvaluator>();
     else if (mode == "hip-effective")
-        backend = std::make_unique<HipEvaluator>(HipEvaluator::Layout::PackedEffective);
+        backend = std::make_unique<HipEvaluator>(HipEvaluator::Strategy::Warp, 256,
+                                                 HipEvaluator::Layout::PackedEffective);
     else return 2;
     const Dataset ds = make_dataset();
     LGPEngine engine(seed, std::move(backend));

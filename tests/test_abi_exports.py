@@ -40,7 +40,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_without_device():
     from galop_amd import _lib
-    assert _lib.load().galop_abi_version() == 3
+    assert _lib.load().galop_abi_version() == 4
 
 
 def test_null_arguments_rejected_without_device():
@@ -51,6 +51,9 @@ def test_null_arguments_rejected_without_device():
     assert lib.galop_eval_csr(None, None, None, None, 0, None, None) == _lib.GALOP_ERR_INVALID
     assert b"NULL" in lib.galop_last_error(None)
     assert lib.galop_fill_random_words_device(None, 5, 8, 0, None) == _lib.GALOP_ERR_INVALID
+    assert lib.galop_ctx_phase_seconds(None, None, None, None, None, None) == _lib.GALOP_ERR_INVALID
+    assert lib.galop_ctx_last_footprint(None, None, None) == _lib.GALOP_ERR_INVALID
+    assert lib.galop_device_count() == 0  # no device visible in this container
 
 
 def test_oracle_is_not_reachable_from_the_product():

@@ -261,8 +261,10 @@ def test_fitness_vs_oracle(gpu_ctx, nr, op_set, ip, N, k, P, lo, hi):
 @pytest.mark.parametrize("tag", ["init", "evolved"])
 def test_engine_population_dense_layouts(gpu_ctx, tag):
     """cuda_diff on the LGPEngine(42) populations: EvalBackend-shaped call,
-    packed and effective layouts vs the reference CpuEvaluator fitness, and
-    bitwise identity packed == effective (tests/cuda_diff.cu:184-235)."""
+    the dense, packed and effective layouts and both strategies vs the
+    reference CpuEvaluator fitness, bitwise identity across them
+    (tests/cuda_diff.cu:184-235), the footprint log (effective words as the
+    oracle's intron analysis counts them) and the phase timers."""
     from galop_amd import Dataset, HipEvaluator, Isa
     f = load_fixture("engine_seed42.npz")
     offs, words, lengths = f[f"{tag}_offsets"], f[f"{tag}_words"], f[f"{tag}_lengths"]
@@ -272,12 +274,25 @@ def test_engine_population_dense_layouts(gpu_ctx, tag):
         dense[i, :lengths[i]] = words[offs[i]:offs[i + 1]]
     ds = Dataset(f["inputs"], f["targets"])
     got = {}
-    for layout in ("packed", "effective"):
-        ev = HipEvaluator(ctx=gpu_ctx, isa=Isa(8), layout=layout)
+    for layout, strategy in (("packed", "warp"), ("effective", "warp"), ("dense", "warp"),
+                             ("dense", "block")):
+        ev = HipEvaluator(ctx=gpu_ctx, isa=Isa(8), layout=layout, strategy=strategy)
         out = np.full(lengths.size, np.nan, np.float32)
         ev.evaluate_population(dense, lengths, out, lengths.size, ds)
-        got[layout] = out
-    assert same_bits(got["packed"], got["effective"]).all()
+        got[layout, strategy] = out
+        (words_l, eff_l), = ev.footprint_log
+        assert words_l == int(lengths.sum())
+        assert (eff_l == 0) if layout == "dense" else (0 < eff_l <= words_l)
+        if layout != "dense":
+            assert eff_l == sum(O.effective_mask(O.make_isa(8), words[offs[i]:offs[i + 1]])[1]
+                                for i in range(lengths.size))
+        ph = ev.phase_seconds()
+        assert ph["kernel"] > 0 and all(v >= 0 for v in ph.values())
+    gpu_ctx.set_tuning(0, 0)
+    base = got["packed", "warp"]
+    for key, out in got.items():
+        assert same_bits(base, out).all(), key
+    got["packed"] = base
     ref = f[f"{tag}_fitness"]
     r = rel_err(got["packed"], ref)
     oisa = O.make_isa(8)
