@@ -305,6 +305,24 @@ void go_eval_dense_device_order(const go_isa* isa, const uint32_t* instructions,
     }
 }
 
+/* go_eval_csr with the GPU evaluator's summation order (OpenMP over programs). */
+void go_eval_csr_device_order(const go_isa* isa, const uint32_t* words, const int64_t* offsets,
+                              int64_t n_programs, const float* inputs, const float* targets,
+                              int32_t n_inputs, int64_t n_cases, float* fitness_out,
+                              double* sse_out, int32_t threads) {
+    const int nt = pick_threads(threads);
+    (void)nt;
+    const double ss_tot = isa->fitness == GO_FITNESS_R2 ? go_ss_tot(targets, n_cases) : 0.0;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
+    for (int64_t i = 0; i < n_programs; ++i) {
+        const double sse = go_program_sse_device_order(isa, words + offsets[i],
+                                                       offsets[i + 1] - offsets[i], inputs,
+                                                       targets, n_inputs, n_cases);
+        if (sse_out) sse_out[i] = sse;
+        fitness_out[i] = go_fitness_of_sse(isa, sse, n_cases, ss_tot);
+    }
+}
+
 /* ---- src/Introns.h:28-57: backward liveness from the output register --- */
 int64_t go_effective_mask(const go_isa* isa, const uint32_t* prog, int64_t len,
                           uint8_t* mask) {

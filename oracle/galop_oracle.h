@@ -109,6 +109,10 @@ double go_sse_device_order(const float* outs, const float* targets, int64_t n_ca
 double go_program_sse_device_order(const go_isa* isa, const uint32_t* prog, int64_t len,
                                    const float* inputs, const float* targets,
                                    int32_t n_inputs, int64_t n_cases);
+void go_eval_csr_device_order(const go_isa* isa, const uint32_t* words, const int64_t* offsets,
+                              int64_t n_programs, const float* inputs, const float* targets,
+                              int32_t n_inputs, int64_t n_cases, float* fitness_out,
+                              double* sse_out, int32_t threads);
 void go_eval_dense_device_order(const go_isa* isa, const uint32_t* instructions,
                                 const uint8_t* lengths, int32_t stride, int64_t n_programs,
                                 const float* inputs, const float* targets, int32_t n_inputs,

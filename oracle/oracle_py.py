@@ -76,6 +76,8 @@ def lib():
         L.go_sse_device_order.argtypes = [vp, vp, i64]
         L.go_program_sse_device_order.restype = ctypes.c_double
         L.go_program_sse_device_order.argtypes = [pisa, vp, i64, vp, vp, i32, i64]
+        L.go_eval_csr_device_order.restype = None
+        L.go_eval_csr_device_order.argtypes = [pisa, vp, vp, i64, vp, vp, i32, i64, vp, vp, i32]
         L.go_eval_dense_device_order.restype = None
         L.go_eval_dense_device_order.argtypes = [pisa, vp, vp, i32, i64, vp, vp, i32, i64, vp, vp]
         _lib = L
@@ -295,10 +297,10 @@ def sse_device_order(outputs, targets):
     return float(lib().go_sse_device_order(_p(o), _p(y), y.size))
 
 
-def eval_csr_device_order(isa, words, offsets, inputs, targets):
+def eval_csr_device_order(isa, words, offsets, inputs, targets, threads=0):
     """eval_csr with the GPU kernels' summation order: (fitness, sse)."""
     words = np.ascontiguousarray(words, np.uint32)
-    offsets = np.asarray(offsets, np.int64)
+    offsets = np.ascontiguousarray(offsets, np.int64)
     x = np.ascontiguousarray(inputs, np.float32)
     if x.ndim == 1:
         x = x[:, None]
@@ -306,13 +308,8 @@ def eval_csr_device_order(isa, words, offsets, inputs, targets):
     n = offsets.size - 1
     sse = np.empty(n, np.float64)
     fit = np.empty(n, np.float32)
-    sst = ss_tot(y) if isa.fitness == FITNESS_R2 else 0.0
-    for i in range(n):
-        p = words[offsets[i]:offsets[i + 1]]
-        sse[i] = lib().go_program_sse_device_order(ctypes.byref(isa), _p(p), p.size, _p(x), _p(y),
-                                                   x.shape[1], y.size)
-        fit[i] = (r2_to_fitness(sse[i], sst) if isa.fitness == FITNESS_R2
-                  else lib().go_sse_to_fitness(sse[i], y.size))
+    lib().go_eval_csr_device_order(ctypes.byref(isa), _p(words), _p(offsets), n, _p(x), _p(y),
+                                   x.shape[1], y.size, _p(fit), _p(sse), int(threads))
     return fit, sse
 
 
