@@ -89,6 +89,8 @@ SIGNATURES = {
     "galop_eval_dense": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i64, _vp, _i32]),
     "galop_eval_csr": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp]),
     "galop_eval_csr_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _i32, _vp]),
+    "galop_eval_csr_device_noalloc": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _i32,
+                                                     _vp]),
     "galop_eval_dense_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i32, _i64, _vp, _vp, _i32,
                                                _vp]),
     "galop_run_cases_device": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp]),

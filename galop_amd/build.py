@@ -31,7 +31,8 @@ BUILD = os.path.join(REPO, "_build")
 LIB = os.path.join(PKG, "libgalop_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels.hip", "api.hip", "batch.hip", "server.hip", "workload.hip", "engine.hip"]
+SOURCES = ["kernels.hip", "api.hip", "batch.hip", "server.hip", "workload.hip", "engine.hip",
+           "noalloc.hip"]
 
 DEVICE_FLAGS = [
     "--offload-arch=gfx950",

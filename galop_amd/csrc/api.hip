@@ -115,6 +115,12 @@ int galop_ctx_create(int32_t device, galop_ctx** out) {
         galop_ctx_destroy(ctx);
         return hip_fail(nullptr, e, "hipEventCreate");
     }
+    const int rc = noalloc_scratch_init(ctx);  // with `device` current
+    if (rc != GALOP_OK) {
+        const std::string msg = ctx->last_error;
+        galop_ctx_destroy(ctx);
+        return set_error(nullptr, rc, msg);
+    }
     *out = ctx;
     return GALOP_OK;
 }

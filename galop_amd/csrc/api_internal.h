@@ -141,6 +141,13 @@ struct Staging {
     }
 };
 
+// Device scratch of galop_eval_csr_device_noalloc (noalloc.hip): the batched
+// path's counters (1 KiB) and order array, or the case-split partials.
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+constexpr int64_t kNoAllocMaxBatchedPrograms = (int64_t)((kNoAllocScratchBytes - 1024) / 4);
+// Resolves this device's scratch address (once per device) into ctx.
+int noalloc_scratch_init(galop_ctx* ctx);
+
 int set_error(galop_ctx* ctx, int code, const std::string& msg);
 int hip_fail(galop_ctx* ctx, hipError_t e, const char* what);
 void parallel_chunks(int64_t n, int64_t min_per_thread,
@@ -163,6 +170,7 @@ struct galop_ctx {
     hipEvent_t ev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double pack_s = 0.0, alloc_s = 0.0, h2d_s = 0.0, kernel_s = 0.0, d2h_s = 0.0;
     int64_t fp_words = 0, fp_effective = 0;  // last galop_eval_dense footprint
+    void* noalloc_scratch = nullptr;  // this device's `scratch` (noalloc.hip)
 };
 
 struct galop_dataset {

@@ -162,6 +162,21 @@ int galop_eval_csr_device(galop_ctx* ctx, const galop_dataset* ds,
                           const uint32_t* d_words, const int64_t* d_offsets,
                           int64_t n_programs, float* d_fitness, double* d_sse,
                           int32_t kernel, void* stream);
+/* galop_eval_csr_device without any device allocation.  No counterpart in
+ * the reference: it is for callers that cannot allocate -- code under HIP
+ * graph capture, tight serving loops, hosts that forbid allocation after
+ * start-up.  Same arguments, kernels and bits as galop_eval_csr_device; the
+ * kernels' scratch (the batched path's work queue, the case-split partials
+ * of small populations) is a fixed 32 MiB device array per GPU, held until
+ * the process ends.  Limits: GALOP_KERNEL_BATCHED takes at most 8,388,352
+ * programs per call (GALOP_ERR_INVALID beyond that); a small population whose
+ * case-split partials would not fit runs one wave per program instead (same
+ * bits).  Every call on one GPU shares that array: calls that overlap on one
+ * GPU (other streams, other threads, other contexts) race. */
+int galop_eval_csr_device_noalloc(galop_ctx* ctx, const galop_dataset* ds,
+                                  const uint32_t* d_words, const int64_t* d_offsets,
+                                  int64_t n_programs, float* d_fitness, double* d_sse,
+                                  int32_t kernel, void* stream);
 /* evaluate_kernel (src/EvaluatorCuda.cu:163-190) on a device-resident
  * population in the engine's dense layout: program i is
  * d_instructions[i*stride, i*stride + d_lengths[i]), stride 1..255 (a length
