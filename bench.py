@@ -335,8 +335,10 @@ def main():
                     help="skip the 64-run serving mix (p50/p99 latency)")
     ap.add_argument("--serving-only", action="store_true")
     ap.add_argument("--serving-runs", type=int, default=64)
-    ap.add_argument("--serving-generations", type=int, default=3)
-    ap.add_argument("--serving-pop-scale", type=float, default=1.0 / 64)
+    ap.add_argument("--serving-generations", type=int, default=1,
+                    help="generations per run under Poisson arrivals, after the all-at-once pass")
+    ap.add_argument("--serving-pop-scale", type=float, default=1.0 / 8)
+    ap.add_argument("--serving-streams", type=int, default=4)
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="run one device copy of this many bytes first (HBM counter calibration)")
     ap.add_argument("--no-uneven", action="store_true",
@@ -392,7 +394,7 @@ def main():
     if args.serving_only:
         from galop_amd.serving_mix import run_mix
         res = run_mix(ctx, n_runs=args.serving_runs, generations=args.serving_generations,
-                      pop_scale=args.serving_pop_scale, log=log)
+                      pop_scale=args.serving_pop_scale, n_streams=args.serving_streams, log=log)
         print(json.dumps({"metric": "serving mix: aggregate GPop/s, p50/p99 eval latency",
                           "value": round(res["gpops"], 3), "unit": "GPop/s", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "f32",
@@ -595,7 +597,7 @@ def main():
         from galop_amd.serving_mix import run_mix
         log("[bench] serving mix")
         res = run_mix(ctx, n_runs=args.serving_runs, generations=args.serving_generations,
-                      pop_scale=args.serving_pop_scale, log=log)
+                      pop_scale=args.serving_pop_scale, n_streams=args.serving_streams, log=log)
         log(f"[bench] serving: {res}")
         line["serving"] = res
     print(json.dumps(line), flush=True)

@@ -7,5 +7,5 @@ synthetic-workload helpers used by the benchmark.
 """
 from ._lib import GalopError, load  # noqa: F401
 from .evaluator import (BatchPlan, Context, Dataset, DeviceDataset, HipEvaluator, Server,  # noqa: F401,E501
-                        Isa, eval_csr, eval_csr_device, eval_dense, eval_dense_device,
+                        Isa, Population, eval_csr, eval_csr_device, eval_dense, eval_dense_device,
                         run_cases_device, strip_introns_device)

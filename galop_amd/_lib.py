@@ -103,6 +103,14 @@ SIGNATURES = {
     "galop_server_create": (ctypes.c_int, [_vp, _i64, _i32, ctypes.POINTER(_vp)]),
     "galop_server_destroy": (None, [_vp]),
     "galop_submit": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _pi64]),
+    "galop_submit_resident": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _pi64]),
+    "galop_population_create": (ctypes.c_int, [_vp, _i64, _i64, ctypes.POINTER(_vp)]),
+    "galop_population_destroy": (ctypes.c_int, [_vp]),
+    "galop_population_upload": (ctypes.c_int, [_vp, _vp, _vp, _i64]),
+    "galop_population_device": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.POINTER(_vp),
+                                               ctypes.POINTER(_vp), ctypes.POINTER(_vp)]),
+    "galop_request_release": (ctypes.c_int, [_vp, _i64]),
+    "galop_server_rounds": (ctypes.c_int, [_vp, _pi64]),
     "galop_poll": (ctypes.c_int, [_vp, _i64, _pi32]),
     "galop_wait": (ctypes.c_int, [_vp, _i64, ctypes.c_double]),
     "galop_request_times": (ctypes.c_int, [_vp, _i64, _pf64, _pf64]),

@@ -109,4 +109,29 @@ hipError_t launch_batched(const DevData& ds, int num_registers, BatchTuning t, i
                           const int64_t* offsets, int64_t n, float* fitness,
                           double* sse, hipStream_t stream);
 
+// Serving: one work queue per round over the programs of many requests.
+// A request's population, dataset and outputs, as the kernels read them.
+struct ServeDesc {
+    DevData ds;
+    const uint32_t* words;   // program p is words[offsets[p], offsets[p+1])
+    const int64_t* offsets;
+    float* fitness;
+    double* sse;             // may be null
+    int64_t n;               // programs
+    int64_t tiles;           // case tiles per program (work = length x tiles)
+};
+constexpr int kServeBuckets = 128;
+// Bits of the request index in a work-queue entry (the rest is the program).
+constexpr int kServeProgramBits = 40;
+// Orders the programs of requests [0, n_req) of `desc` (host copy h_desc for
+// the sizes) into `order`, heaviest work first (length x case tiles, 4 buckets
+// per octave), entries (request << 40) | program; counters holds
+// 3 * kServeBuckets int32.
+hipError_t launch_serve_order(const ServeDesc* desc, const ServeDesc* h_desc, int n_req,
+                              int32_t* counters, int64_t* order, hipStream_t stream);
+// One wave per entry of `order` (n entries), all requests of one
+// (num_registers, cases per lane, op table) configuration.
+hipError_t launch_serve_eval(const ServeDesc* desc, int num_registers, int cases_per_lane,
+                             int op_set, const int64_t* order, int64_t n, hipStream_t stream);
+
 }  // namespace galop

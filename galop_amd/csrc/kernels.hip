@@ -728,6 +728,160 @@ batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ o
     }
 }
 
+// ---------------------------------------------------------------------------
+// serving rounds: the programs of every request of a round in one work queue
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int work_bucket(int64_t work) {
+    if (work < 16) return (int)(work < 0 ? 0 : work);
+    const int lg = 63 - __clzll((unsigned long long)work);  // >= 4
+    const int frac = (int)((work >> (lg - 2)) & 3);
+    const int b = 16 + (lg - 4) * 4 + frac;
+    return b > kServeBuckets - 1 ? kServeBuckets - 1 : b;
+}
+
+// counters: [0, B) program counts per bucket, [B, 2B) bucket starts, [2B, 3B)
+// places taken so far in each bucket (B = kServeBuckets).
+__global__ void serve_reset_kernel(int32_t* counters) {
+    for (int i = (int)threadIdx.x; i < 3 * kServeBuckets; i += (int)blockDim.x) counters[i] = 0;
+}
+
+__global__ void __launch_bounds__(256)
+serve_histogram_kernel(const ServeDesc* __restrict__ desc, int r, int32_t* __restrict__ counts) {
+    __shared__ int32_t local[kServeBuckets];
+    for (int i = (int)threadIdx.x; i < kServeBuckets; i += (int)blockDim.x) local[i] = 0;
+    __syncthreads();
+    const int64_t* __restrict__ off = desc[r].offsets;
+    const int64_t n = desc[r].n, tiles = desc[r].tiles;
+    for (int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; p < n;
+         p += (int64_t)gridDim.x * blockDim.x)
+        atomicAdd(&local[work_bucket((off[p + 1] - off[p]) * tiles)], 1);
+    __syncthreads();
+    for (int i = (int)threadIdx.x; i < kServeBuckets; i += (int)blockDim.x)
+        if (local[i]) atomicAdd(&counts[i], local[i]);
+}
+
+// Start of each bucket in the queue, heaviest bucket first.
+__global__ void serve_bucket_start_kernel(const int32_t* __restrict__ counts,
+                                          int32_t* __restrict__ cursor) {
+    if (threadIdx.x == 0) {
+        int32_t run = 0;
+        for (int b = kServeBuckets - 1; b >= 0; --b) {
+            cursor[b] = run;
+            run += counts[b];
+        }
+    }
+}
+
+// Queue entries of request r: each block ranks its tile of programs in LDS and
+// reserves one range per bucket with a single global atomic.
+__global__ void __launch_bounds__(kScatterThreads)
+serve_scatter_kernel(const ServeDesc* __restrict__ desc, int r, const int32_t* __restrict__ start,
+                     int32_t* __restrict__ taken, int64_t* __restrict__ order) {
+    __shared__ int32_t cnt[kServeBuckets];
+    __shared__ int32_t base[kServeBuckets];
+    const int t = (int)threadIdx.x;
+    for (int i = t; i < kServeBuckets; i += kScatterThreads) cnt[i] = 0;
+    __syncthreads();
+    const int64_t* __restrict__ off = desc[r].offsets;
+    const int64_t n = desc[r].n, tiles = desc[r].tiles;
+    const int64_t first = (int64_t)blockIdx.x * kScatterTile;
+    int bucket[kScatterItems], rank[kScatterItems];
+#pragma unroll
+    for (int it = 0; it < kScatterItems; ++it) {
+        const int64_t p = first + (int64_t)it * kScatterThreads + t;
+        bucket[it] = 0;
+        rank[it] = 0;
+        if (p < n) {
+            bucket[it] = work_bucket((off[p + 1] - off[p]) * tiles);
+            rank[it] = atomicAdd(&cnt[bucket[it]], 1);
+        }
+    }
+    __syncthreads();
+    for (int i = t; i < kServeBuckets; i += kScatterThreads)
+        if (cnt[i]) base[i] = start[i] + atomicAdd(&taken[i], cnt[i]);
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < kScatterItems; ++it) {
+        const int64_t p = first + (int64_t)it * kScatterThreads + t;
+        if (p < n)
+            order[(int64_t)base[bucket[it]] + rank[it]] = ((int64_t)r << kServeProgramBits) | p;
+    }
+}
+
+// One wave per queue entry: the request's descriptor is read with the
+// wave-uniform request index, then the program runs exactly as in
+// plain_kernel (same bits).
+template <int NR, int K, int EXT>
+__global__ void GALOP_INTERP_BOUNDS
+serve_kernel(const ServeDesc* __restrict__ desc, const int64_t* __restrict__ order,
+             int64_t first, int64_t n) {
+    const int lane = (int)(threadIdx.x & (kWave - 1));
+    const int wave_in_block = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int64_t w = first + (int64_t)blockIdx.x * (int64_t)(blockDim.x >> 6) + wave_in_block;
+    if (w >= n) return;  // wave-uniform
+    const int64_t e = readlane_i64(order[w], 0);
+    const int r = (int)(e >> kServeProgramBits);
+    const int64_t p = e & ((int64_t(1) << kServeProgramBits) - 1);
+    const ServeDesc* __restrict__ d = desc + r;
+    const DevData ds = d->ds;
+    const ConstRegs cr = load_const_regs(ds.consts, lane);
+    const int64_t lo = d->offsets[p], hi = d->offsets[p + 1];
+    const double sse = eval_program_wave<NR, K, EXT>(d->words + lo, hi - lo, ds, lane, cr, nullptr);
+    if (lane == 0) {
+        d->fitness[p] = fitness_of_sse(sse, ds);
+        if (d->sse) d->sse[p] = sse;
+    }
+}
+
+hipError_t launch_serve_order(const ServeDesc* desc, const ServeDesc* h_desc, int n_req,
+                              int32_t* counters, int64_t* order, hipStream_t stream) {
+    hipLaunchKernelGGL(serve_reset_kernel, dim3(1), dim3(256), 0, stream, counters);
+    for (int r = 0; r < n_req; ++r) {
+        const int64_t n = h_desc[r].n;
+        if (n <= 0) continue;
+        const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+        hipLaunchKernelGGL(serve_histogram_kernel, dim3(grid), dim3(256), 0, stream, desc, r,
+                           counters);
+    }
+    hipLaunchKernelGGL(serve_bucket_start_kernel, dim3(1), dim3(64), 0, stream, counters,
+                       counters + kServeBuckets);
+    for (int r = 0; r < n_req; ++r) {
+        const int64_t n = h_desc[r].n;
+        if (n <= 0) continue;
+        hipLaunchKernelGGL(serve_scatter_kernel, dim3((unsigned)((n + kScatterTile - 1) / kScatterTile)),
+                           dim3(kScatterThreads), 0, stream, desc, r, counters + kServeBuckets,
+                           counters + 2 * kServeBuckets, order);
+    }
+    return hipGetLastError();
+}
+
+template <int NR, int K, int EXT>
+static hipError_t launch_serve_t(const ServeDesc* desc, const int64_t* order, int64_t n,
+                                 hipStream_t stream) {
+    constexpr int wpb = 4;
+    for (int64_t first = 0; first < n; first += kLaunchWindow) {
+        const int64_t cnt = (n - first) < kLaunchWindow ? (n - first) : kLaunchWindow;
+        hipLaunchKernelGGL((serve_kernel<NR, K, EXT>), dim3((unsigned)((cnt + wpb - 1) / wpb)),
+                           dim3(wpb * kWave), 0, stream, desc, order, first, first + cnt);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+hipError_t launch_serve_eval(const ServeDesc* desc, int num_registers, int cases_per_lane,
+                             int op_set, const int64_t* order, int64_t n, hipStream_t stream) {
+    if (n <= 0) return hipSuccess;
+#define GALOP_CASE(NRV, KV)                                                                 \
+    if (num_registers == NRV && cases_per_lane == KV)                                       \
+        return op_set ? launch_serve_t<NRV, KV, 1>(desc, order, n, stream)                  \
+                      : launch_serve_t<NRV, KV, 0>(desc, order, n, stream);
+    GALOP_CASE(8, 1) GALOP_CASE(8, 2) GALOP_CASE(8, 4) GALOP_CASE(8, 8)
+    GALOP_CASE(16, 1) GALOP_CASE(16, 2) GALOP_CASE(16, 4)
+#undef GALOP_CASE
+    return hipErrorInvalidValue;
+}
+
 BatchTuning resolve_batch_tuning(const DevData& ds, int num_registers, BatchTuning t, int n_cu) {
     (void)n_cu;
     if (t.cases_per_lane <= 0) {

@@ -1,15 +1,21 @@
 // server.hip -- serving many concurrent evolution runs (C ABI).
 //
 // Clients (one per evolution run, any thread) submit generation-evaluation
-// requests: a dataset handle (resident in HBM) and a host CSR population.  A
-// scheduler thread admits requests in arrival order into S stream slots while
-// the device bytes of resident datasets plus every slot's buffers stay under
-// the HBM budget; an admitted request is copied to the device, evaluated by
-// the persistent batched kernel with 1/S of the chip's worker waves, and its
-// fitness copied back, all on the slot's stream, so up to S requests execute
-// on the device at once and a new request starts as soon as a slot and memory
-// free up.  Fitness of a program does not depend on which requests ran beside
-// it: the per-program reduction order is fixed (see lgp_device.h).
+// requests: a dataset handle (resident in HBM) and either a population kept
+// resident in HBM by the server (galop_population, submitted by handle, no
+// copy) or a host CSR population (copied in when the request is admitted).
+//
+// A scheduler thread runs up to S rounds at once, one per HIP stream.  When a
+// stream is free it forms a round from the waiting requests, in arrival order,
+// as long as the HBM budget holds the round: resident populations, the
+// datasets of requests in flight, and every stream's grow-only work buffers.
+// A round merges the programs of all its requests into one work queue, ordered
+// heaviest first (program length x case tiles, across requests), and
+// evaluates it with one wave per entry, so requests of any size share the
+// whole chip, and the waves of concurrent rounds interleave as they retire.
+// Requests that arrive while rounds run join the next round as soon as a
+// stream and memory free up.  A program's fitness does not depend on the
+// round it ran in: the per-program reduction order is fixed (lgp_device.h).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,7 +27,6 @@
 #include <string>
 #include <thread>
 #include <unordered_map>
-#include <unordered_set>
 #include <vector>
 
 #include "api_internal.h"
@@ -33,59 +38,107 @@ namespace {
 
 enum class ReqState { Queued, Running, Done };
 
+constexpr int kMaxRoundRequests = 1024;
+
 struct Request {
     int64_t id = 0;
     const galop_dataset* ds = nullptr;
-    const uint32_t* words = nullptr;
+    galop_population* pop = nullptr;  // resident population, or
+    const uint32_t* words = nullptr;  // a host CSR population
     const int64_t* offsets = nullptr;
     int64_t n = 0;
     int64_t total_words = 0;
-    float* fitness_out = nullptr;
+    float* fitness_out = nullptr;  // host outputs, may be null for resident requests
     double* sse_out = nullptr;
     ReqState state = ReqState::Queued;
     int status = GALOP_OK;
     Clock::time_point t_submit, t_done;
     double device_s = 0.0;
+    int k = 0;  // cases per lane: requests of one round share (registers, k, op table)
+    // placement in its round
+    int64_t word_base = 0, prog_base = 0, out_base = 0;
 };
 
+// One stream's round and its grow-only buffers.
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t ev_start = nullptr, ev_end = nullptr;
-    DeviceBuf<uint32_t> d_words;
+    DeviceBuf<int64_t> order;      // work queue entries
+    DeviceBuf<int32_t> counters;   // bucket counters
+    DeviceBuf<ServeDesc> desc;
+    PinnedBuf<ServeDesc> h_desc;
+    DeviceBuf<uint32_t> d_words;   // host requests' programs
     DeviceBuf<int64_t> d_offsets;
-    DeviceBuf<float> d_fitness;
+    DeviceBuf<float> d_fitness;    // host requests' outputs
     DeviceBuf<double> d_sse;
-    BatchBufs batch;
     PinnedBuf<uint32_t> h_words;
     PinnedBuf<int64_t> h_offsets;
-    PinnedBuf<float> h_fitness;
+    PinnedBuf<float> h_fitness;    // outputs of every request that wants them on the host
     PinnedBuf<double> h_sse;
-    Request* req = nullptr;  // in flight
-    int64_t capacity_bytes() const {
-        return (int64_t)(d_words.bytes() + d_offsets.bytes() + d_fitness.bytes() + d_sse.bytes() +
-                         batch.small.bytes() + batch.order.bytes());
+    std::vector<Request*> reqs;    // the round in flight (empty: the slot is free)
+    // device bytes after the buffers grow for a round of this size
+    int64_t bytes_after(const struct RoundSize& z) const;
+    int64_t device_bytes() const {
+        return (int64_t)(order.bytes() + counters.bytes() + desc.bytes() + d_words.bytes() +
+                         d_offsets.bytes() + d_fitness.bytes() + d_sse.bytes());
     }
     void release_device() {
+        order.release();
+        counters.release();
+        desc.release();
         d_words.release();
         d_offsets.release();
         d_fitness.release();
         d_sse.release();
-        batch.release();
     }
 };
 
-// Device bytes a request needs in a slot (words, offsets, fitness, sse,
-// work-queue order), with the 25% growth headroom of DeviceBuf::reserve.
-int64_t request_bytes(int64_t n, int64_t words) {
-    auto grown = [](int64_t elems, int64_t size) {
-        elems = std::max<int64_t>(elems, 1);
-        return (elems + elems / 4) * size;
-    };
-    return grown(words, 4) + grown(n + 1, 8) + grown(n, 4) + grown(n, 8) +
-           std::max<int64_t>(n, 1) * 4 + (2 * kBuckets + 64) * 4;
+// Element counts of a round's device buffers.
+struct RoundSize {
+    int64_t programs = 0, requests = 0;                 // all requests
+    int64_t host_words = 0, host_programs = 0;          // host requests
+    int64_t host_offsets = 0;                           // sum of (n + 1)
+    void add(int64_t n, int64_t words, bool host) {
+        programs += n;
+        requests += 1;
+        if (host) {
+            host_words += words;
+            host_programs += n;
+            host_offsets += n + 1;
+        }
+    }
+};
+
+// Bytes of a buffer of `cap` elements after DeviceBuf::reserve(need): kept
+// when large enough, else reallocated with 25% headroom.
+int64_t grown_bytes(size_t cap, int64_t need, size_t elem) {
+    need = std::max<int64_t>(need, 1);
+    if ((int64_t)cap >= need) return (int64_t)(cap * elem);
+    return (need + need / 4) * (int64_t)elem;
+}
+
+int64_t Slot::bytes_after(const RoundSize& z) const {
+    return grown_bytes(order.cap, z.programs, 8) + grown_bytes(counters.cap, 3 * kServeBuckets, 4) +
+           grown_bytes(desc.cap, z.requests, sizeof(ServeDesc)) +
+           grown_bytes(d_words.cap, z.host_words, 4) + grown_bytes(d_offsets.cap, z.host_offsets, 8) +
+           grown_bytes(d_fitness.cap, z.host_programs, 4) + grown_bytes(d_sse.cap, z.host_programs, 8);
 }
 
 }  // namespace
+
+struct galop_population {
+    galop_server* srv = nullptr;
+    int64_t max_programs = 0, max_words = 0;
+    int64_t n_uploaded = -1;  // programs of the last upload (-1: written on the device)
+    DeviceBuf<uint32_t> words;
+    DeviceBuf<int64_t> offsets;
+    DeviceBuf<float> fitness;
+    DeviceBuf<double> sse;
+    int busy = 0;  // requests in flight that use it
+    int64_t bytes() const {
+        return (int64_t)(words.bytes() + offsets.bytes() + fitness.bytes() + sse.bytes());
+    }
+};
 
 struct galop_server {
     galop_ctx* ctx = nullptr;
@@ -97,85 +150,164 @@ struct galop_server {
     std::condition_variable cv_done;  // client wake-up
     std::deque<Request*> queue;
     std::unordered_map<int64_t, Request*> requests;
-    std::unordered_set<const galop_dataset*> resident;
-    int64_t resident_bytes = 0;
+    std::unordered_map<const galop_dataset*, int> datasets_in_flight;  // -> requests using it
+    std::vector<galop_population*> populations;  // live ones; freed with the server
+    int64_t dataset_bytes = 0;      // device bytes of datasets_in_flight
+    int64_t population_bytes = 0;   // device bytes of every live population
     int64_t next_id = 1;
     bool stop = false;
-    int64_t admitted = 0, completed = 0, peak_bytes = 0;
+    int64_t admitted = 0, completed = 0, peak_bytes = 0, rounds = 0;
 
     int64_t slot_bytes() const {
         int64_t b = 0;
-        for (const auto& s : slots) b += s.capacity_bytes();
+        for (const auto& s : slots) b += s.device_bytes();
         return b;
     }
+    int64_t resident_bytes() const { return population_bytes + dataset_bytes + slot_bytes(); }
     void run();
-    bool start(Slot& s, Request* r);
-    void finish(Slot& s);
+    bool start_round(Slot& s);
+    void finish_round(Slot& s, hipError_t status);
+    void complete(Request* r, int status);
 };
 
-bool galop_server::start(Slot& s, Request* r) {
-    // grow the slot's buffers; device growth happens only at a new high-water mark
-    hipError_t e = s.d_words.reserve((size_t)std::max<int64_t>(r->total_words, 1));
-    if (e == hipSuccess) e = s.d_offsets.reserve((size_t)r->n + 1);
-    if (e == hipSuccess) e = s.d_fitness.reserve((size_t)std::max<int64_t>(r->n, 1));
-    if (e == hipSuccess) e = s.d_sse.reserve((size_t)std::max<int64_t>(r->n, 1));
-    if (e == hipSuccess) e = s.h_words.reserve((size_t)std::max<int64_t>(r->total_words, 1));
-    if (e == hipSuccess) e = s.h_offsets.reserve((size_t)r->n + 1);
-    if (e == hipSuccess) e = s.h_fitness.reserve((size_t)std::max<int64_t>(r->n, 1));
-    if (e == hipSuccess) e = s.h_sse.reserve((size_t)std::max<int64_t>(r->n, 1));
+static int request_k(const galop_dataset* ds) {
+    return resolve_plain_tuning(ds->dev, ds->isa.num_registers, PlainTuning{}).cases_per_lane;
+}
+
+static bool same_config(const Request* a, const Request* b) {
+    return a->ds->isa.num_registers == b->ds->isa.num_registers && a->k == b->k &&
+           a->ds->isa.op_set == b->ds->isa.op_set;
+}
+
+// Caller holds mu.
+void galop_server::complete(Request* r, int status) {
+    r->status = status;
+    r->state = ReqState::Done;
+    r->t_done = Clock::now();
+    ++completed;
+    if (r->pop) --r->pop->busy;
+    cv_done.notify_all();
+}
+
+// Launch slot s's round (s.reqs, placed by the scheduler); called without mu.
+bool galop_server::start_round(Slot& s) {
+    RoundSize sz;
+    for (Request* r : s.reqs) sz.add(r->n, r->total_words, !r->pop);
+    hipError_t e = s.order.reserve((size_t)std::max<int64_t>(sz.programs, 1));
+    if (e == hipSuccess) e = s.counters.reserve(3 * kServeBuckets);
+    if (e == hipSuccess) e = s.desc.reserve((size_t)sz.requests);
+    if (e == hipSuccess) e = s.h_desc.reserve((size_t)sz.requests);
+    if (e == hipSuccess) e = s.d_words.reserve((size_t)std::max<int64_t>(sz.host_words, 1));
+    if (e == hipSuccess) e = s.d_offsets.reserve((size_t)std::max<int64_t>(sz.host_offsets, 1));
+    if (e == hipSuccess) e = s.d_fitness.reserve((size_t)std::max<int64_t>(sz.host_programs, 1));
+    if (e == hipSuccess) e = s.d_sse.reserve((size_t)std::max<int64_t>(sz.host_programs, 1));
+    if (e == hipSuccess) e = s.h_words.reserve((size_t)std::max<int64_t>(sz.host_words, 1));
+    if (e == hipSuccess) e = s.h_offsets.reserve((size_t)std::max<int64_t>(sz.host_offsets, 1));
+    if (e == hipSuccess) e = s.h_fitness.reserve((size_t)std::max<int64_t>(sz.programs, 1));
+    if (e == hipSuccess) e = s.h_sse.reserve((size_t)std::max<int64_t>(sz.programs, 1));
     if (e != hipSuccess) {
-        r->status = hip_fail(ctx, e, "server: slot allocation");
+        hip_fail(ctx, e, "server: round buffers");
         return false;
     }
-    if (ensure_batch_scratch(ctx, s.batch, r->n) != GALOP_OK) {
-        r->status = GALOP_ERR_NOMEM;
-        return false;
+    // descriptors; host requests' programs staged in pinned memory
+    int64_t wb = 0, pb = 0, ob = 0, hb = 0;
+    for (size_t i = 0; i < s.reqs.size(); ++i) {
+        Request* r = s.reqs[i];
+        ServeDesc& d = s.h_desc.ptr[i];
+        d.ds = r->ds->dev;
+        d.n = r->n;
+        const int64_t tile = (int64_t)kWave * r->k;
+        d.tiles = std::max<int64_t>(1, (r->ds->dev.n_cases + tile - 1) / tile);
+        r->out_base = ob;
+        ob += r->n;
+        if (r->pop) {
+            d.words = r->pop->words.ptr;
+            d.offsets = r->pop->offsets.ptr;
+            d.fitness = r->pop->fitness.ptr;
+            d.sse = r->pop->sse.ptr;
+        } else {
+            r->word_base = wb;
+            r->prog_base = pb;
+            const int64_t first = r->offsets[0];
+            for (int64_t j = 0; j <= r->n; ++j) s.h_offsets.ptr[pb + j] = r->offsets[j] - first;
+            uint32_t* hw = s.h_words.ptr + wb;
+            const uint32_t* src = r->words + first;
+            parallel_chunks(r->total_words, int64_t(1) << 22, [&](int64_t lo, int64_t hi) {
+                std::memcpy(hw + lo, src + lo, (size_t)(hi - lo) * sizeof(uint32_t));
+            });
+            d.words = s.d_words.ptr + wb;
+            d.offsets = s.d_offsets.ptr + pb;
+            d.fitness = s.d_fitness.ptr + hb;
+            d.sse = r->sse_out ? s.d_sse.ptr + hb : nullptr;
+            wb += r->total_words;
+            pb += r->n + 1;
+            hb += r->n;
+        }
     }
-    // stage the population in pinned memory (offsets rebased to 0)
-    const int64_t first = r->offsets[0];
-    for (int64_t i = 0; i <= r->n; ++i) s.h_offsets.ptr[i] = r->offsets[i] - first;
-    uint32_t* hw = s.h_words.ptr;
-    const uint32_t* src = r->words + first;
-    parallel_chunks(r->total_words, int64_t(1) << 22, [&](int64_t lo, int64_t hi) {
-        std::memcpy(hw + lo, src + lo, (size_t)(hi - lo) * sizeof(uint32_t));
-    });
-    const int n_slots = (int)slots.size();
-    BatchTuning t;
-    t.workers_per_cu = std::max(4, 32 / n_slots);
-    hipStream_t st = s.stream;
+    const hipStream_t st = s.stream;
+    const Request* r0 = s.reqs[0];
     e = hipEventRecord(s.ev_start, st);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(s.d_words.ptr, hw, (size_t)r->total_words * sizeof(uint32_t),
+    if (e == hipSuccess && wb > 0)
+        e = hipMemcpyAsync(s.d_words.ptr, s.h_words.ptr, (size_t)wb * sizeof(uint32_t),
+                           hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && pb > 0)
+        e = hipMemcpyAsync(s.d_offsets.ptr, s.h_offsets.ptr, (size_t)pb * sizeof(int64_t),
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(s.d_offsets.ptr, s.h_offsets.ptr, (size_t)(r->n + 1) * sizeof(int64_t),
+        e = hipMemcpyAsync(s.desc.ptr, s.h_desc.ptr, s.reqs.size() * sizeof(ServeDesc),
                            hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-        e = launch_batched(r->ds->dev, r->ds->isa.num_registers, t, ctx->n_cu, s.batch.view(),
-                           s.d_words.ptr, s.d_offsets.ptr, r->n, s.d_fitness.ptr,
-                           r->sse_out ? s.d_sse.ptr : nullptr, st);
+        e = launch_serve_order(s.desc.ptr, s.h_desc.ptr, (int)s.reqs.size(), s.counters.ptr,
+                               s.order.ptr, st);
     if (e == hipSuccess)
-        e = hipMemcpyAsync(s.h_fitness.ptr, s.d_fitness.ptr, (size_t)r->n * sizeof(float),
-                           hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess && r->sse_out)
-        e = hipMemcpyAsync(s.h_sse.ptr, s.d_sse.ptr, (size_t)r->n * sizeof(double),
-                           hipMemcpyDeviceToHost, st);
+        e = launch_serve_eval(s.desc.ptr, r0->ds->isa.num_registers, r0->k, r0->ds->isa.op_set,
+                              s.order.ptr, sz.programs, st);
+    // outputs the clients want on the host
+    for (size_t i = 0; e == hipSuccess && i < s.reqs.size(); ++i) {
+        const Request* r = s.reqs[i];
+        const ServeDesc& d = s.h_desc.ptr[i];
+        if (r->fitness_out)
+            e = hipMemcpyAsync(s.h_fitness.ptr + r->out_base, d.fitness, (size_t)r->n * sizeof(float),
+                               hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && r->sse_out && d.sse)
+            e = hipMemcpyAsync(s.h_sse.ptr + r->out_base, d.sse, (size_t)r->n * sizeof(double),
+                               hipMemcpyDeviceToHost, st);
+    }
     if (e == hipSuccess) e = hipEventRecord(s.ev_end, st);
     if (e != hipSuccess) {
-        r->status = hip_fail(ctx, e, "server: launch");
+        hip_fail(ctx, e, "server: round launch");
+        hipStreamSynchronize(st);  // nothing of the round may still run when it is torn down
         return false;
     }
     return true;
 }
 
-void galop_server::finish(Slot& s) {
-    Request* r = s.req;
-    std::memcpy(r->fitness_out, s.h_fitness.ptr, (size_t)r->n * sizeof(float));
-    if (r->sse_out) std::memcpy(r->sse_out, s.h_sse.ptr, (size_t)r->n * sizeof(double));
+// Outputs to the clients and completion; called without mu.  Returns the
+// requests' statuses through complete() under the lock.
+void galop_server::finish_round(Slot& s, hipError_t status) {
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, s.ev_start, s.ev_end) == hipSuccess) r->device_s = ms / 1e3;
-    r->status = GALOP_OK;
-    s.req = nullptr;
+    const double dev_s =
+        hipEventElapsedTime(&ms, s.ev_start, s.ev_end) == hipSuccess ? ms / 1e3 : 0.0;
+    if (status == hipSuccess) {
+        for (Request* r : s.reqs) {
+            if (r->fitness_out)
+                std::memcpy(r->fitness_out, s.h_fitness.ptr + r->out_base, (size_t)r->n * sizeof(float));
+            if (r->sse_out)
+                std::memcpy(r->sse_out, s.h_sse.ptr + r->out_base, (size_t)r->n * sizeof(double));
+            r->device_s = dev_s;
+        }
+    }
+    std::lock_guard<std::mutex> lk(mu);
+    const int rc = status == hipSuccess ? GALOP_OK : hip_fail(ctx, status, "server: round execution");
+    for (Request* r : s.reqs) {
+        auto it = datasets_in_flight.find(r->ds);
+        if (it != datasets_in_flight.end() && --it->second == 0) {
+            dataset_bytes -= galop_dataset_device_bytes(r->ds);
+            datasets_in_flight.erase(it);
+        }
+        complete(r, rc);
+    }
+    s.reqs.clear();
 }
 
 void galop_server::run() {
@@ -183,99 +315,104 @@ void galop_server::run() {
     std::unique_lock<std::mutex> lk(mu);
     while (true) {
         bool progressed = false;
-        // completions
+        // completed rounds
         for (auto& s : slots) {
-            if (!s.req) continue;
+            if (s.reqs.empty()) continue;
             const hipError_t q = hipEventQuery(s.ev_end);
             if (q == hipErrorNotReady) continue;
-            Request* r = s.req;
-            if (q == hipSuccess) {
-                finish(s);
-            } else {
-                r->status = hip_fail(ctx, q, "server: request execution");
-                s.req = nullptr;
-            }
-            r->t_done = Clock::now();
-            r->state = ReqState::Done;
-            ++completed;
+            lk.unlock();
+            finish_round(s, q);
+            lk.lock();
             progressed = true;
-            cv_done.notify_all();
         }
-        if (stop) {
-            bool busy = false;
-            for (auto& s : slots) busy |= s.req != nullptr;
-            if (!busy) break;
-        }
-        // admissions, in arrival order
-        while (!stop && !queue.empty()) {
-            Slot* free_slot = nullptr;
-            for (auto& s : slots)
-                if (!s.req) {
-                    free_slot = &s;
-                    break;
-                }
-            if (!free_slot) break;
-            Request* r = queue.front();
-            int64_t ds_bytes = resident.count(r->ds) ? 0 : galop_dataset_device_bytes(r->ds);
-            const int64_t need = request_bytes(r->n, r->total_words);
-            auto projected = [&]() {
-                int64_t others = slot_bytes() - free_slot->capacity_bytes();
-                return resident_bytes + ds_bytes + others +
-                       std::max<int64_t>(free_slot->capacity_bytes(), need);
+        bool busy = false;
+        for (auto& s : slots) busy |= !s.reqs.empty();
+        if (stop && !busy) break;
+        // a new round on a free stream: waiting requests in arrival order, of
+        // the first one's kernel configuration, while the budget holds
+        Slot* free_slot = nullptr;
+        for (auto& s : slots)
+            if (s.reqs.empty()) {
+                free_slot = &s;
+                break;
+            }
+        if (!stop && free_slot && !queue.empty()) {
+            const Request* head = queue.front();
+            RoundSize sz;
+            int64_t extra_ds = 0;  // datasets this round adds to those in flight
+            std::vector<const galop_dataset*> new_ds;
+            std::vector<Request*> take;
+            auto projected = [&](const RoundSize& z, int64_t ds_add) {
+                return population_bytes + dataset_bytes + ds_add + slot_bytes() -
+                       free_slot->device_bytes() + free_slot->bytes_after(z);
             };
-            if (projected() > budget) {
-                // give memory of idle slots back before waiting on busy ones
-                for (auto& s : slots)
-                    if (!s.req && &s != free_slot) s.release_device();
-                if (projected() > budget) {
-                    bool busy = false;
-                    for (auto& s : slots) busy |= s.req != nullptr;
-                    if (busy) break;  // wait for a running request to finish
-                    free_slot->release_device();
-                    if (projected() > budget) {
-                        queue.pop_front();
-                        r->status = set_error(ctx, GALOP_ERR_BUDGET,
-                                              "server: request does not fit the HBM budget");
-                        r->state = ReqState::Done;
-                        r->t_done = Clock::now();
-                        ++completed;
-                        cv_done.notify_all();
-                        progressed = true;
-                        continue;
+            for (auto it = queue.begin(); it != queue.end() && (int)take.size() < kMaxRoundRequests;) {
+                Request* r = *it;
+                if (!same_config(r, head) || sz.programs + r->n > (int64_t)INT32_MAX) {
+                    ++it;
+                    continue;
+                }
+                RoundSize z = sz;
+                z.add(r->n, r->total_words, !r->pop);
+                const bool ds_new = !datasets_in_flight.count(r->ds) &&
+                                    std::find(new_ds.begin(), new_ds.end(), r->ds) == new_ds.end();
+                const int64_t ds_add = extra_ds + (ds_new ? galop_dataset_device_bytes(r->ds) : 0);
+                if (projected(z, ds_add) > budget) {
+                    // idle streams give their memory back before the round waits
+                    for (auto& s : slots)
+                        if (s.reqs.empty() && &s != free_slot) s.release_device();
+                    if (projected(z, ds_add) > budget && !take.empty()) break;
+                    if (projected(z, ds_add) > budget) {
+                        if (busy) break;  // wait for a round to finish
+                        free_slot->release_device();
+                        if (projected(z, ds_add) > budget) {
+                            it = queue.erase(it);
+                            r->status = set_error(ctx, GALOP_ERR_BUDGET,
+                                                  "server: request does not fit the HBM budget");
+                            complete(r, GALOP_ERR_BUDGET);
+                            progressed = true;
+                            continue;
+                        }
                     }
                 }
+                sz = z;
+                if (ds_new) {
+                    new_ds.push_back(r->ds);
+                    extra_ds = ds_add;
+                }
+                take.push_back(r);
+                it = queue.erase(it);
             }
-            queue.pop_front();
-            if (ds_bytes) {
-                resident.insert(r->ds);
-                resident_bytes += ds_bytes;
+            if (!take.empty()) {
+                for (Request* r : take) {
+                    if (datasets_in_flight[r->ds]++ == 0)
+                        dataset_bytes += galop_dataset_device_bytes(r->ds);
+                    r->state = ReqState::Running;
+                    ++admitted;
+                }
+                free_slot->reqs = take;
+                ++rounds;
+                lk.unlock();
+                const bool ok = start_round(*free_slot);
+                lk.lock();
+                if (!ok) {
+                    for (Request* r : free_slot->reqs) {
+                        auto it = datasets_in_flight.find(r->ds);
+                        if (it != datasets_in_flight.end() && --it->second == 0) {
+                            dataset_bytes -= galop_dataset_device_bytes(r->ds);
+                            datasets_in_flight.erase(it);
+                        }
+                        complete(r, GALOP_ERR_HIP);
+                    }
+                    free_slot->reqs.clear();
+                }
+                peak_bytes = std::max(peak_bytes, resident_bytes());
+                progressed = true;
             }
-            ++admitted;
-            // staging copies can be large: do them without holding the lock
-            // (the slot is reserved, and only this thread touches slots)
-            free_slot->req = r;
-            r->state = ReqState::Running;
-            lk.unlock();
-            const bool ok = start(*free_slot, r);
-            lk.lock();
-            if (!ok) {
-                free_slot->req = nullptr;
-                r->state = ReqState::Done;
-                r->t_done = Clock::now();
-                ++completed;
-                cv_done.notify_all();
-            }
-            peak_bytes = std::max(peak_bytes, resident_bytes + slot_bytes());
-            progressed = true;
-        }
-        if (stop && queue.empty()) {
-            bool busy = false;
-            for (auto& s : slots) busy |= s.req != nullptr;
-            if (!busy) break;
         }
         if (!progressed) {
-            bool busy = false;
-            for (auto& s : slots) busy |= s.req != nullptr;
+            busy = false;
+            for (auto& s : slots) busy |= !s.reqs.empty();
             if (busy)
                 cv_work.wait_for(lk, std::chrono::microseconds(50));
             else
@@ -318,11 +455,7 @@ void galop_server_destroy(galop_server* s) {
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->stop = true;
-        for (Request* r : s->queue) {  // never started: cancelled
-            r->status = GALOP_ERR_CANCELLED;
-            r->state = ReqState::Done;
-            r->t_done = Clock::now();
-        }
+        for (Request* r : s->queue) s->complete(r, GALOP_ERR_CANCELLED);  // never started
         s->queue.clear();
     }
     s->cv_work.notify_all();
@@ -332,6 +465,7 @@ void galop_server_destroy(galop_server* s) {
     for (auto& sl : s->slots) {
         if (sl.stream) hipStreamSynchronize(sl.stream);
         sl.release_device();
+        sl.h_desc.release();
         sl.h_words.release();
         sl.h_offsets.release();
         sl.h_fitness.release();
@@ -341,7 +475,153 @@ void galop_server_destroy(galop_server* s) {
         if (sl.stream) hipStreamDestroy(sl.stream);
     }
     for (auto& kv : s->requests) delete kv.second;
+    for (galop_population* p : s->populations) {
+        p->words.release();
+        p->offsets.release();
+        p->fitness.release();
+        p->sse.release();
+        delete p;
+    }
     delete s;
+}
+
+// ---- resident populations ----
+int galop_population_create(galop_server* s, int64_t max_programs, int64_t max_words,
+                            galop_population** out) {
+    if (!s || !out) return set_error(nullptr, GALOP_ERR_INVALID, "galop_population_create: NULL argument");
+    *out = nullptr;
+    galop_ctx* ctx = s->ctx;
+    if (max_programs < 1 || max_words < 0 || max_programs > (int64_t)INT32_MAX)
+        return set_error(ctx, GALOP_ERR_INVALID,
+                         "galop_population_create: need 1 <= max_programs <= 2^31-1, max_words >= 0");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    auto* p = new galop_population();
+    p->srv = s;
+    p->max_programs = max_programs;
+    p->max_words = max_words;
+    const int64_t bytes = std::max<int64_t>(max_words, 1) * 4 + (max_programs + 1) * 8 +
+                          max_programs * 4 + max_programs * 8;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->resident_bytes() + bytes > s->budget) {
+            delete p;
+            return set_error(ctx, GALOP_ERR_BUDGET,
+                             "galop_population_create: the population does not fit the HBM budget");
+        }
+        s->population_bytes += bytes;  // reserved before the allocation
+    }
+    e = p->words.alloc((size_t)std::max<int64_t>(max_words, 1));
+    if (e == hipSuccess) e = p->offsets.alloc((size_t)max_programs + 1);
+    if (e == hipSuccess) e = p->fitness.alloc((size_t)max_programs);
+    if (e == hipSuccess) e = p->sse.alloc((size_t)max_programs);
+    if (e == hipSuccess) e = hipMemset(p->offsets.ptr, 0, sizeof(int64_t));
+    if (e != hipSuccess) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->population_bytes -= bytes;
+        }
+        p->words.release();
+        p->offsets.release();
+        p->fitness.release();
+        p->sse.release();
+        delete p;
+        return hip_fail(ctx, e, "galop_population_create: hipMalloc");
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->population_bytes += p->bytes() - bytes;
+        s->peak_bytes = std::max(s->peak_bytes, s->resident_bytes());
+        s->populations.push_back(p);
+    }
+    *out = p;
+    return GALOP_OK;
+}
+
+int galop_population_destroy(galop_population* p) {
+    if (!p) return GALOP_OK;
+    galop_server* s = p->srv;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return p->busy == 0; });
+    s->population_bytes -= p->bytes();
+    s->populations.erase(std::find(s->populations.begin(), s->populations.end(), p));
+    lk.unlock();
+    hipSetDevice(s->ctx->device);
+    p->words.release();
+    p->offsets.release();
+    p->fitness.release();
+    p->sse.release();
+    delete p;
+    return GALOP_OK;
+}
+
+int galop_population_upload(galop_population* p, const uint32_t* words, const int64_t* offsets,
+                            int64_t n_programs) {
+    if (!p) return set_error(nullptr, GALOP_ERR_INVALID, "galop_population_upload: NULL population");
+    galop_ctx* ctx = p->srv->ctx;
+    if (n_programs < 0 || n_programs > p->max_programs || !offsets)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_population_upload: n_programs outside [0, max]");
+    for (int64_t i = 0; i < n_programs; ++i)
+        if (offsets[i + 1] < offsets[i])
+            return set_error(ctx, GALOP_ERR_INVALID, "galop_population_upload: offsets must be non-decreasing");
+    const int64_t total = offsets[n_programs] - offsets[0];
+    if (total > p->max_words) return set_error(ctx, GALOP_ERR_INVALID, "galop_population_upload: more than max_words words");
+
+    {
+        std::lock_guard<std::mutex> lk(p->srv->mu);
+        if (p->busy) return set_error(ctx, GALOP_ERR_INVALID, "galop_population_upload: a request on it is in flight");
+    }
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    std::vector<int64_t> rebased((size_t)n_programs + 1);
+    for (int64_t i = 0; i <= n_programs; ++i) rebased[(size_t)i] = offsets[i] - offsets[0];
+    if (total > 0 && words)  // NULL words: the programs are already in place on the device
+        e = hipMemcpy(p->words.ptr, words + offsets[0], (size_t)total * sizeof(uint32_t),
+                      hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(p->offsets.ptr, rebased.data(), rebased.size() * sizeof(int64_t),
+                      hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(ctx, e, "galop_population_upload");
+    p->n_uploaded = n_programs;
+    return GALOP_OK;
+}
+
+int galop_population_device(galop_population* p, uint32_t** d_words, int64_t** d_offsets,
+                            float** d_fitness, double** d_sse) {
+    if (!p) return set_error(nullptr, GALOP_ERR_INVALID, "galop_population_device: NULL population");
+    if (d_words) *d_words = p->words.ptr;
+    if (d_offsets) *d_offsets = p->offsets.ptr;
+    if (d_fitness) *d_fitness = p->fitness.ptr;
+    if (d_sse) *d_sse = p->sse.ptr;
+    p->n_uploaded = -1;
+    return GALOP_OK;
+}
+
+// ---- requests ----
+static int enqueue(galop_server* s, Request* r, int64_t* request_id) {
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) {
+            delete r;
+            return set_error(s->ctx, GALOP_ERR_CANCELLED, "galop_submit: server is shutting down");
+        }
+        r->id = s->next_id++;
+        r->t_submit = Clock::now();
+        if (r->pop) ++r->pop->busy;
+        s->requests[r->id] = r;
+        s->queue.push_back(r);
+        *request_id = r->id;
+    }
+    s->cv_work.notify_all();
+    return GALOP_OK;
+}
+
+// Device bytes of a round holding only this request (it can never run when
+// they exceed the budget).
+static int64_t lone_request_bytes(const Request* r) {
+    RoundSize z;
+    z.add(r->n, r->total_words, !r->pop);
+    return Slot().bytes_after(z) + galop_dataset_device_bytes(r->ds);
 }
 
 int galop_submit(galop_server* s, const galop_dataset* ds, const uint32_t* words,
@@ -358,10 +638,8 @@ int galop_submit(galop_server* s, const galop_dataset* ds, const uint32_t* words
             return set_error(ctx, GALOP_ERR_INVALID, "galop_submit: offsets must be non-decreasing");
     const int64_t total = offsets[n_programs] - offsets[0];
     if (total > 0 && !words) return set_error(ctx, GALOP_ERR_INVALID, "galop_submit: NULL words");
-    if (n_programs > (int64_t)INT32_MAX - (int64_t(1) << 22))
+    if (n_programs > (int64_t)INT32_MAX)
         return set_error(ctx, GALOP_ERR_INVALID, "galop_submit: too many programs");
-    if (request_bytes(n_programs, total) + galop_dataset_device_bytes(ds) > s->budget)
-        return set_error(ctx, GALOP_ERR_BUDGET, "galop_submit: request larger than the HBM budget");
     auto* r = new Request();
     r->ds = ds;
     r->words = words;
@@ -370,20 +648,38 @@ int galop_submit(galop_server* s, const galop_dataset* ds, const uint32_t* words
     r->total_words = total;
     r->fitness_out = fitness_out;
     r->sse_out = sse_out;
-    r->t_submit = Clock::now();
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (s->stop) {
-            delete r;
-            return set_error(ctx, GALOP_ERR_CANCELLED, "galop_submit: server is shutting down");
-        }
-        r->id = s->next_id++;
-        s->requests[r->id] = r;
-        s->queue.push_back(r);
-        *request_id = r->id;
+    r->k = request_k(ds);
+    if (lone_request_bytes(r) > s->budget) {
+        delete r;
+        return set_error(ctx, GALOP_ERR_BUDGET, "galop_submit: request larger than the HBM budget");
     }
-    s->cv_work.notify_all();
-    return GALOP_OK;
+    return enqueue(s, r, request_id);
+}
+
+int galop_submit_resident(galop_server* s, const galop_dataset* ds, galop_population* p,
+                          int64_t n_programs, float* fitness_out, double* sse_out,
+                          int64_t* request_id) {
+    if (!s || !ds || !p || !request_id)
+        return set_error(nullptr, GALOP_ERR_INVALID, "galop_submit_resident: NULL argument");
+    galop_ctx* ctx = s->ctx;
+    if (ds->ctx != ctx || p->srv != s)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_submit_resident: dataset or population of another server");
+    if (n_programs < 1 || n_programs > p->max_programs)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_submit_resident: n_programs outside [1, max_programs]");
+    if (p->n_uploaded >= 0 && n_programs > p->n_uploaded)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_submit_resident: more programs than were uploaded");
+    auto* r = new Request();
+    r->ds = ds;
+    r->pop = p;
+    r->n = n_programs;
+    r->fitness_out = fitness_out;
+    r->sse_out = sse_out;
+    r->k = request_k(ds);
+    if (lone_request_bytes(r) > s->budget) {
+        delete r;
+        return set_error(ctx, GALOP_ERR_BUDGET, "galop_submit_resident: request larger than the HBM budget");
+    }
+    return enqueue(s, r, request_id);
 }
 
 int galop_poll(galop_server* s, int64_t request_id, int32_t* status) {
@@ -425,6 +721,17 @@ int galop_request_times(galop_server* s, int64_t request_id, double* latency_s, 
     return GALOP_OK;
 }
 
+int galop_request_release(galop_server* s, int64_t request_id) {
+    if (!s) return set_error(nullptr, GALOP_ERR_INVALID, "galop_request_release: NULL server");
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->requests.find(request_id);
+    if (it == s->requests.end() || it->second->state != ReqState::Done)
+        return set_error(s->ctx, GALOP_ERR_INVALID, "galop_request_release: unknown or unfinished request");
+    delete it->second;
+    s->requests.erase(it);
+    return GALOP_OK;
+}
+
 int galop_server_stats(galop_server* s, int64_t* admitted, int64_t* completed, int64_t* peak_bytes,
                        int64_t* queued) {
     if (!s) return set_error(nullptr, GALOP_ERR_INVALID, "galop_server_stats: NULL server");
@@ -433,6 +740,13 @@ int galop_server_stats(galop_server* s, int64_t* admitted, int64_t* completed, i
     if (completed) *completed = s->completed;
     if (peak_bytes) *peak_bytes = s->peak_bytes;
     if (queued) *queued = (int64_t)s->queue.size();
+    return GALOP_OK;
+}
+
+int galop_server_rounds(galop_server* s, int64_t* rounds) {
+    if (!s || !rounds) return set_error(nullptr, GALOP_ERR_INVALID, "galop_server_rounds: NULL argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *rounds = s->rounds;
     return GALOP_OK;
 }
 

@@ -177,7 +177,10 @@ def _ptr(a):
 
 
 def _dptr(t):
-    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+    """A device address from a torch tensor (or a plain integer address)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t if isinstance(t, int) else t.data_ptr())
 
 
 def eval_csr(ds, words, offsets, want_sse=False):
@@ -337,11 +340,47 @@ class HipEvaluator:
         self.footprint_log.append((words.value, eff.value))
 
 
+class Population:
+    """A population kept resident in HBM by a Server (galop_population_*):
+    upload() once, then submit it by handle every generation, or write it in
+    place on the device through device()."""
+
+    def __init__(self, server, max_programs, max_words):
+        self.server = server
+        self.lib = server.lib
+        h = ctypes.c_void_p()
+        check(self.lib.galop_population_create(server.handle, int(max_programs), int(max_words),
+                                               ctypes.byref(h)), server.ctx.handle)
+        self.handle = h
+        self.n = 0
+
+    def upload(self, words, offsets):
+        """words=None: only the offsets (the programs are written on the device)."""
+        words = None if words is None else np.ascontiguousarray(words, np.uint32)
+        offsets = np.ascontiguousarray(offsets, np.int64)
+        check(self.lib.galop_population_upload(self.handle, _ptr(words), _ptr(offsets),
+                                               offsets.size - 1), self.server.ctx.handle)
+        self.n = offsets.size - 1
+
+    def device(self):
+        """Device addresses of (words, offsets, fitness, sse)."""
+        ptrs = [ctypes.c_void_p() for _ in range(4)]
+        check(self.lib.galop_population_device(self.handle, *[ctypes.byref(p) for p in ptrs]),
+              self.server.ctx.handle)
+        return tuple(p.value for p in ptrs)
+
+    def close(self):
+        if self.handle and self.server.handle:
+            self.lib.galop_population_destroy(self.handle)
+        self.handle = None
+
+
 class Server:
     """Continuous-batching evaluation server (galop_server_* in the C ABI).
 
     submit() returns at once; the population and output arrays passed in must
-    stay alive until wait() reports the request done."""
+    stay alive until wait() reports the request done.  submit_resident()
+    evaluates a Population resident in HBM."""
 
     def __init__(self, ctx, hbm_budget_bytes, n_streams=4):
         self.ctx = ctx
@@ -362,11 +401,23 @@ class Server:
         self._keep[rid.value] = (words, offsets, fitness_out, sse_out)
         return rid.value
 
+    def submit_resident(self, dds, pop, n=None, fitness_out=None, sse_out=None):
+        rid = ctypes.c_int64()
+        check(self.lib.galop_submit_resident(self.handle, dds.handle, pop.handle,
+                                             int(pop.n if n is None else n), _ptr(fitness_out),
+                                             _ptr(sse_out), ctypes.byref(rid)), self.ctx.handle)
+        self._keep[rid.value] = (fitness_out, sse_out)
+        return rid.value
+
     def wait(self, rid, timeout_s=-1.0):
         rc = self.lib.galop_wait(self.handle, int(rid), float(timeout_s))
         if rc == _lib.GALOP_OK:
             self._keep.pop(rid, None)
         return rc
+
+    def release(self, rid):
+        check(self.lib.galop_request_release(self.handle, int(rid)), self.ctx.handle)
+        self._keep.pop(rid, None)
 
     def poll(self, rid):
         st = ctypes.c_int32(0)
@@ -384,8 +435,10 @@ class Server:
         a, c, p, q = (ctypes.c_int64() for _ in range(4))
         check(self.lib.galop_server_stats(self.handle, ctypes.byref(a), ctypes.byref(c),
                                           ctypes.byref(p), ctypes.byref(q)), self.ctx.handle)
+        rounds = ctypes.c_int64()
+        check(self.lib.galop_server_rounds(self.handle, ctypes.byref(rounds)), self.ctx.handle)
         return {"admitted": a.value, "completed": c.value, "peak_bytes": p.value,
-                "queued": q.value}
+                "queued": q.value, "rounds": rounds.value}
 
     def close(self):
         if self.handle:

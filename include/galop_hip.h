@@ -238,15 +238,56 @@ int  galop_batch_autotune(galop_batch* b, const uint32_t* d_words,
                           int32_t* chunk);
 
 /* ---- serving: many evolution runs, continuous batching -----------------
- * A server keeps datasets and populations resident, admits requests while
- * their device bytes fit under hbm_budget_bytes, and overlaps admitted
- * requests on n_streams HIP streams.  galop_submit returns at once; the
- * population is read later, when the request is admitted, so words, offsets,
- * fitness_out and sse_out must all stay valid and unchanged until galop_wait
- * (or galop_poll) reports the request done. */
+ * A server runs up to n_streams rounds at once, one per HIP stream.  When a
+ * stream is free it forms a round from the waiting requests, in arrival
+ * order, while the HBM budget (hbm_budget_bytes) holds the resident
+ * populations, the datasets of the requests in flight and every stream's work
+ * buffers.  A round merges the programs of all its requests into one work
+ * queue, heaviest work first across requests, so requests of any size share
+ * the chip; requests that arrive while rounds run join the next round as
+ * soon as a stream and memory free up.  Fitness of a program is bitwise the
+ * same whatever ran beside it.  No counterpart in the reference, whose
+ * evaluation is one synchronous call per generation
+ * (src/EvaluatorCuda.cu:298-469). */
+typedef struct galop_population galop_population;
 int  galop_server_create(galop_ctx* ctx, int64_t hbm_budget_bytes,
                          int32_t n_streams, galop_server** out);
+/* Cancels queued requests, lets running rounds finish, frees every
+ * population still alive. */
 void galop_server_destroy(galop_server* s);
+
+/* A population resident in HBM, owned by the server and counted against its
+ * budget (GALOP_ERR_BUDGET when it does not fit): up to max_programs CSR
+ * programs of max_words words in total, with device fitness and FP64 SSE
+ * arrays that requests on it write. */
+int  galop_population_create(galop_server* s, int64_t max_programs, int64_t max_words,
+                             galop_population** out);
+/* Waits until no request uses it, then frees it. */
+int  galop_population_destroy(galop_population* p);
+/* Copies a host CSR population in (blocking; offsets rebased to 0).  words
+ * may be NULL to copy only the offsets of programs already written on the
+ * device (galop_population_device).  Refused while a request on the
+ * population is in flight. */
+int  galop_population_upload(galop_population* p, const uint32_t* words,
+                             const int64_t* offsets, int64_t n_programs);
+/* The population's device buffers, for a caller that writes programs in
+ * place on the device (CSR: d_offsets[0..n], d_words) and reads fitness
+ * there.  Nothing may write them while a request on the population is in
+ * flight.  Any output may be NULL. */
+int  galop_population_device(galop_population* p, uint32_t** d_words,
+                             int64_t** d_offsets, float** d_fitness, double** d_sse);
+
+/* Evaluate programs [0, n_programs) of a resident population: results go to
+ * its device fitness / SSE arrays, and also to fitness_out / sse_out on the
+ * host when they are not NULL (valid until the request is done).  Returns at
+ * once. */
+int  galop_submit_resident(galop_server* s, const galop_dataset* ds,
+                           galop_population* p, int64_t n_programs,
+                           float* fitness_out, double* sse_out, int64_t* request_id);
+/* Evaluate a host CSR population.  Returns at once; the population is read
+ * later, when the request is admitted, so words, offsets, fitness_out and
+ * sse_out must all stay valid and unchanged until galop_wait (or galop_poll)
+ * reports the request done. */
 int  galop_submit(galop_server* s, const galop_dataset* ds,
                   const uint32_t* words, const int64_t* offsets,
                   int64_t n_programs, float* fitness_out, double* sse_out,
@@ -254,12 +295,15 @@ int  galop_submit(galop_server* s, const galop_dataset* ds,
 /* 0 = done (status in *status), 1 = pending */
 int  galop_poll(galop_server* s, int64_t request_id, int32_t* status);
 int  galop_wait(galop_server* s, int64_t request_id, double timeout_s);
-/* submit->result-visible latency and device execution time of a completed
- * request, in seconds. */
+/* submit->result-visible latency and device time of its round, in seconds,
+ * of a completed request. */
 int  galop_request_times(galop_server* s, int64_t request_id,
                          double* latency_s, double* device_s);
+/* Forget a completed request (its id becomes unknown). */
+int  galop_request_release(galop_server* s, int64_t request_id);
 int  galop_server_stats(galop_server* s, int64_t* admitted, int64_t* completed,
                         int64_t* peak_bytes, int64_t* queued);
+int  galop_server_rounds(galop_server* s, int64_t* rounds);
 
 #ifdef __cplusplus
 }
