@@ -258,7 +258,12 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
     } while (0)
 
 // sin / cos: the branch-free Cody-Waite path for the whole word unless some
-// lane holds, in some case, a finite argument of magnitude >= 2^25.
+// lane holds, in some case, a finite argument of magnitude >= 2^25.  The
+// word's arguments are classified by the largest of their bits shifted left
+// by one (the magnitude's bits, in order): below 0x98 << 24 (|x| < 2^25, no
+// infinity or NaN) in every lane, the word takes Cody-Waite, or the
+// select-only path when every argument is a zero; otherwise each case is
+// classified on its own.
 #define GALOP_SINCOS(U, WANT_COS)                                          \
     do {                                                                  \
         if (kc) {                                                         \
@@ -267,16 +272,27 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
         } else {                                                          \
             const galop_math::CoefPtr C = galop_math::coef_table();       \
             float bv[K], cv[K];                                           \
-            bool ph = false, sp = true;                                   \
+            uint32_t mag = 0;                                             \
             _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
                 bv[k] = r.get(k, s2);                                     \
-                ph |= galop_math::sincos_needs_ph(bv[k]);                 \
-                sp = sp && galop_math::sincos_special(bv[k]);             \
+                mag = max(mag, galop_math::f2u(bv[k]) << 1);              \
             }                                                             \
-            if (__all(sp)) {                                              \
+            bool cw = __all(mag < 0x98000000u);                           \
+            bool special = cw && __all(mag == 0u);                        \
+            bool mixed = false;                                           \
+            if (!cw) {                                                    \
+                bool ph = false, sp = true;                               \
+                _Pragma("unroll") for (int k = 0; k < K; ++k) {           \
+                    ph |= galop_math::sincos_needs_ph(bv[k]);             \
+                    sp = sp && galop_math::sincos_special(bv[k]);         \
+                }                                                         \
+                special = __all(sp);                                      \
+                mixed = !special && __any(ph);                            \
+            }                                                             \
+            if (special) {                                                \
                 _Pragma("unroll") for (int k = 0; k < K; ++k)             \
                     cv[k] = galop_math::sincos_special_value(bv[k], WANT_COS); \
-            } else if (__any(ph)) {                                       \
+            } else if (mixed) {                                           \
                 _Pragma("unroll") for (int k = 0; k < K; ++k)             \
                     cv[k] = galop_math::sincos_needs_ph(bv[k])            \
                                 ? galop_math::sincos_ph(bv[k], WANT_COS, C) \
@@ -288,9 +304,9 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                     double t[kRowGroup], ts[kRowGroup], tc[kRowGroup];    \
                     _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) { \
                         uint32_t j;                                       \
-                        t[i] = galop_math::sincos_reduce(bv[g + i], WANT_COS, j, C); \
-                        ts[i] = galop_math::SINCOS_T[2 * j];              \
-                        tc[i] = galop_math::SINCOS_T[2 * j + 1];          \
+                        t[i] = galop_math::sincos_reduce(bv[g + i], j, C); \
+                        ts[i] = galop_math::SINCOS_T[2 * j + 32 * (WANT_COS)]; \
+                        tc[i] = galop_math::SINCOS_T[2 * j + 1 + 32 * (WANT_COS)]; \
                     }                                                     \
                     _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) \
                         cv[g + i] = galop_math::sincos_finish(bv[g + i], WANT_COS, t[i], \

@@ -50,17 +50,23 @@
 
 namespace galop_math {
 
-// pi/32 = PIO32_0 + PIO32_1 + PIO32_2; the first two carry 24 bits each, so
-// k * PIO32_{0,1} is exact for |k| < 2^29 (|x| < 2^25 gives |k| < 2^28.4).
-constexpr double PIO32_0 = 0x1.921fb60000000p-4;
-constexpr double PIO32_1 = -0x1.777a5c0000000p-29;
-constexpr double PIO32_2 = -0x1.ee59d9cceba40p-54;
+// pi/32 = PIO32_HI + PIO32_LO to about 2^-110: PIO32_HI is pi/32 rounded to
+// a double, PIO32_LO the rounded rest.  For a float x with |x| < 2^25 and
+// k = rint(x 32/pi) (|k| < 2^28.4), x - k PIO32_HI is a multiple of 2^-56
+// smaller than 2^-4 (or x itself when k = 0), so fma(-k, PIO32_HI, x) is
+// exact and the reduction takes two FMAs.
+constexpr double PIO32_HI = 0x1.921fb54442d18p-4;
+constexpr double PIO32_LO = 0x1.1a62633145c07p-58;
 constexpr double INV_PIO32 = 0x1.45f306dc9c883p+3;       // 32/pi
 constexpr double PIO32_SCALED64 = 0x1.921fb54442d18p-68;  // pi/32 * 2^-64
 // ln 2 = LN2_0 + LN2_1 + LN2_2; LN2_{0,1} carry 32 bits each.
 constexpr double LN2_0 = 0x1.62e42ff000000p-1;
 constexpr double LN2_1 = -0x1.718432a200000p-35;
 constexpr double LN2_2 = 0x1.3c7673007e5edp-69;
+// ln 2 = LN2_HI + LN2_LO for exp's reduction, the same way: |k| <= 151 and
+// x - k LN2_HI is a multiple of 2^-53 below 1/2, so fma(-k, LN2_HI, x) is exact.
+constexpr double LN2_HI = 0x1.62e42fefa39efp-1;
+constexpr double LN2_LO = 0x1.abc9e3b39803fp-56;
 constexpr double INV_LN2 = 0x1.71547652b82fep+0;
 // Rounding by addition: for |v| < 2^51, fl(v + 1.5 2^52) is 1.5 2^52 plus v
 // rounded to the nearest integer (ties to even), and the low word of that
@@ -84,8 +90,9 @@ constexpr double LOG1P_C[6] = {-0x1.0000000000000p-1, 0x1.5555555555555p-2, -0x1
 // bits 2^-1 .. 2^-256 of 2/pi (word 0 first)
 GM_CONST uint32_t TWO_OVER_PI_W[8] = {0xA2F9836Eu, 0x4E441529u, 0xFC2757D1u, 0xF534DDC0u,
                                       0xDB629599u, 0x3C439041u, 0xFE5163ABu, 0xDEBBC561u};
-// (sin, cos)(j pi/32), j = 0..63, interleaved
-GM_CONST double SINCOS_T[128] = {
+// (sin, cos)(j pi/32), j = 0..79, interleaved; rows 64..79 repeat rows 0..15
+// so that cos (row j + 16) needs no wrap-around
+GM_CONST double SINCOS_T[160] = {
     0x0.0p+0, 0x1.0000000000000p+0,
     0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1,
     0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1,
@@ -150,6 +157,22 @@ GM_CONST double SINCOS_T[128] = {
     -0x1.294062ed59f06p-2, 0x1.e9f4156c62ddap-1,
     -0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1,
     -0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1,
+    0x0.0p+0, 0x1.0000000000000p+0,
+    0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1,
+    0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1,
+    0x1.294062ed59f06p-2, 0x1.e9f4156c62ddap-1,
+    0x1.87de2a6aea963p-2, 0x1.d906bcf328d46p-1,
+    0x1.e2b5d3806f63bp-2, 0x1.c38b2f180bdb1p-1,
+    0x1.1c73b39ae68c8p-1, 0x1.a9b66290ea1a3p-1,
+    0x1.44cf325091dd6p-1, 0x1.8bc806b151741p-1,
+    0x1.6a09e667f3bcdp-1, 0x1.6a09e667f3bcdp-1,
+    0x1.8bc806b151741p-1, 0x1.44cf325091dd6p-1,
+    0x1.a9b66290ea1a3p-1, 0x1.1c73b39ae68c8p-1,
+    0x1.c38b2f180bdb1p-1, 0x1.e2b5d3806f63bp-2,
+    0x1.d906bcf328d46p-1, 0x1.87de2a6aea963p-2,
+    0x1.e9f4156c62ddap-1, 0x1.294062ed59f06p-2,
+    0x1.f6297cff75cb0p-1, 0x1.8f8b83c69a60bp-3,
+    0x1.fd88da3d12526p-1, 0x1.917a6bc29b42cp-4,
 };
 
 // (c_j, ln(1/c_j)), j = -37..53 at index j + 37, interleaved
@@ -259,21 +282,22 @@ enum CoefSlot : int {
     kLog1pC = 13,     // LOG1P_C[0..5]
     kSinT = 19,       // SINT_C[0..2]
     kCosT = 22,       // COST_C[0..3]
-    kPio32 = 26,      // PIO32_0, PIO32_1, PIO32_2
-    kInvPio32 = 29,
-    kPio32Scaled64 = 30,
-    kLn2 = 31,        // LN2_0, LN2_1, LN2_2
-    kInvLn2 = 34,
-    kRoundMagic = 35,
-    kCoefCount = 36
+    kPio32 = 26,      // PIO32_HI, PIO32_LO
+    kInvPio32 = 28,
+    kPio32Scaled64 = 29,
+    kLn2 = 30,        // LN2_0, LN2_1, LN2_2
+    kLn2Fma = 33,     // LN2_HI, LN2_LO
+    kInvLn2 = 35,
+    kRoundMagic = 36,
+    kCoefCount = 37
 };
 #define GM_COEF_INIT                                                              \
     {EXP_C[0], EXP_C[1], EXP_C[2], EXP_C[3], EXP_C[4], EXP_C[5], EXP_C[6],        \
      EXP_C[7], EXP_C[8], EXP_C[9], EXP_C[10], EXP_C[11], EXP_C[12],               \
      LOG1P_C[0], LOG1P_C[1], LOG1P_C[2], LOG1P_C[3], LOG1P_C[4], LOG1P_C[5],      \
      SINT_C[0], SINT_C[1], SINT_C[2], COST_C[0], COST_C[1], COST_C[2], COST_C[3],  \
-     PIO32_0, PIO32_1, PIO32_2, INV_PIO32, PIO32_SCALED64,                        \
-     LN2_0, LN2_1, LN2_2, INV_LN2, ROUND_MAGIC}
+     PIO32_HI, PIO32_LO, INV_PIO32, PIO32_SCALED64,                               \
+     LN2_0, LN2_1, LN2_2, LN2_HI, LN2_LO, INV_LN2, ROUND_MAGIC}
 
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const __attribute__((address_space(4))) double* CoefPtr;
@@ -291,6 +315,28 @@ static inline CoefPtr coef_table() { return COEF_TABLE; }
 
 GM_HD double fma_d(double a, double b, double c) { return __builtin_fma(a, b, c); }
 GM_HD double rint_d(double a) { return __builtin_rint(a); }
+
+// A coefficient copied into a VGPR pair where it is used.  One VOP3
+// instruction reads at most one SGPR pair, so the first Horner step, whose
+// other operand is a coefficient too, needs one of them in VGPRs; the empty
+// asm makes that copy from the per-word coefficient load (so once per
+// instruction word, shared by its cases) instead of a multiply and an add per
+// case, or a VGPR pair held for the whole kernel.
+GM_HD double vreg(double c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    __asm__("" : "+v"(c));
+#endif
+    return c;
+}
+
+// x clamped to [lo, hi] (x itself when NaN is not a concern of the caller)
+GM_HD float clampf(float x, float lo, float hi) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fmed3f(x, lo, hi);
+#else
+    return fminf(fmaxf(x, lo), hi);
+#endif
+}
 
 GM_HD uint32_t f2u(float x) {
     union { float f; uint32_t u; } v;
@@ -326,14 +372,12 @@ GM_HD float expf_special_value(float x) {
 // exp(-104.5) to +0 in FP32, as every argument beyond does; NaN is patched
 // at the end (the clamp maps it to -104.5).
 GM_HD float expf_r(float x, CoefPtr C) {
-    const float xc = __builtin_fminf(__builtin_fmaxf(x, -104.5f), 89.5f);
+    const float xc = clampf(x, -104.5f, 89.5f);  // NaN: any value, replaced below
     const double xd = (double)xc;
     const double k = rint_d(xd * C[kInvLn2]);
-    double r = fma_d(-k, C[kLn2], xd);
-    r = fma_d(-k, C[kLn2 + 1], r);
-    r = fma_d(-k, C[kLn2 + 2], r);
-    double p = C[kExpC + 12];
-    for (int n = 11; n >= 0; --n) p = fma_d(p, r, C[kExpC + n]);
+    const double r = fma_d(-k, C[kLn2Fma + 1], fma_d(-k, C[kLn2Fma], xd));
+    double p = fma_d(vreg(C[kExpC + 12]), r, C[kExpC + 11]);
+    for (int n = 10; n >= 0; --n) p = fma_d(p, r, C[kExpC + n]);
     const float v = (float)ldexp(p, (int)k);
     return (x != x) ? x : v;
 }
@@ -363,8 +407,8 @@ GM_HD double log_reduce(float x, int32_t& e, uint32_t& row91, CoefPtr C) {
 GM_HD float log_finish(double y, int32_t e, double cj, double lncj, CoefPtr C) {
     const double r = fma_d(y, cj, -1.0);  // exact: c_j has 24 bits
     // log1p(r) to r^7: the rest is below |r|^8 / 8 < 1.2e-19 for |r| < 0.0056
-    double p = C[kLog1pC + 5];
-    for (int n = 4; n >= 0; --n) p = fma_d(p, r, C[kLog1pC + n]);
+    double p = fma_d(vreg(C[kLog1pC + 5]), r, C[kLog1pC + 4]);
+    for (int n = 3; n >= 0; --n) p = fma_d(p, r, C[kLog1pC + n]);
     const double q = fma_d(r * r, p, r);  // log1p(r)
     const double ed = (double)e;
     // e ln2 + ln(1/c_j) + log1p(r); e * LN2_0 is exact
@@ -452,14 +496,11 @@ GM_HD double reduce_ph(float x, int& j, CoefPtr C) {
 // (s + c t) + t^2 (s pc + c t ps): five operations after the two series.
 GM_HD double sin_from_row(double t, double s, double c, CoefPtr C) {
     const double t2 = t * t;
-    // sin t = t + t^3 (s3 + t^2 (s5 + t^2 s7)); the innermost step is a
-    // multiply and an add: one VOP3 instruction on gfx950 reads at most one
-    // SGPR pair, and an FMA of two coefficients would keep one of them in a
-    // VGPR pair for the whole kernel
-    double ps = t2 * C[kSinT + 2] + C[kSinT + 1];
+    // sin t = t + t^3 (s3 + t^2 (s5 + t^2 s7))
+    double ps = fma_d(t2, vreg(C[kSinT + 2]), C[kSinT + 1]);
     ps = fma_d(ps, t2, C[kSinT]);
     // cos t - 1 = t^2 (c2 + t^2 (c4 + t^2 (c6 + t^2 c8)))
-    double pc = t2 * C[kCosT + 3] + C[kCosT + 2];
+    double pc = fma_d(t2, vreg(C[kCosT + 3]), C[kCosT + 2]);
     pc = fma_d(pc, t2, C[kCosT + 1]);
     pc = fma_d(pc, t2, C[kCosT]);
     const double ct = c * t;
@@ -469,19 +510,18 @@ GM_HD double sin_from_reduced(double t, int j, CoefPtr C) {
     return sin_from_row(t, SINCOS_T[2 * j], SINCOS_T[2 * j + 1], C);
 }
 
-// Cody-Waite for an x that does not need Payne-Hanek: x = k pi/32 + t, and the
-// table row j = (k + 16 want_cos) mod 64 (cos x = sin(x + pi/2)).  k =
-// rint(x 32/pi) is rounded by addition (|k| < 2^28.4), so the row is the low
-// word of that sum: no conversion, and for an infinite or NaN x, whose t is
-// NaN, the row is still in 0..63.
-GM_HD double sincos_reduce(float x, int want_cos, uint32_t& j, CoefPtr C) {
+// Cody-Waite for an x that does not need Payne-Hanek: x = k pi/32 + t, and
+// j = k mod 64; the table row is j for sin and j + 16 for cos (cos x =
+// sin(x + pi/2); the table repeats rows 0..15 as 64..79).  k = rint(x 32/pi)
+// is rounded by addition (|k| < 2^28.4), so j is the low word of that sum: no
+// conversion, and for an infinite or NaN x, whose t is NaN, j is still in
+// 0..63.
+GM_HD double sincos_reduce(float x, uint32_t& j, CoefPtr C) {
     const double xd = (double)x;
     const double kb = xd * C[kInvPio32] + C[kRoundMagic];
     const double k = kb - C[kRoundMagic];
-    j = ((uint32_t)d2u(kb) + 16u * (uint32_t)want_cos) & 63u;
-    double t = fma_d(-k, C[kPio32], xd);
-    t = fma_d(-k, C[kPio32 + 1], t);
-    return fma_d(-k, C[kPio32 + 2], t);
+    j = (uint32_t)d2u(kb) & 63u;
+    return fma_d(-k, C[kPio32 + 1], fma_d(-k, C[kPio32], xd));
 }
 
 // The rounded result from the reduced argument and its row: infinities and
@@ -497,8 +537,9 @@ GM_HD float sincos_finish(float x, int want_cos, double t, double s, double c, C
 // interleaved.
 GM_HD float sincos_cw(float x, int want_cos, CoefPtr C) {
     uint32_t j;
-    const double t = sincos_reduce(x, want_cos, j, C);
-    return sincos_finish(x, want_cos, t, SINCOS_T[2 * j], SINCOS_T[2 * j + 1], C);
+    const double t = sincos_reduce(x, j, C);
+    const uint32_t row = j + 16u * (uint32_t)want_cos;
+    return sincos_finish(x, want_cos, t, SINCOS_T[2 * row], SINCOS_T[2 * row + 1], C);
 }
 
 // Arguments whose sin / cos is fixed without arithmetic: zeros (sin keeps the

@@ -81,6 +81,16 @@ def split(value_fixed, prec, bits):
     return parts
 
 
+def split_fma(value_fixed, prec):
+    """value = value_fixed/2^prec as hi + lo: hi rounded to a double, lo the
+    rounded remainder.  With an FMA, fma(-k, hi, x) is exact whenever the
+    result is smaller than hi (its bits lie between those of x and k hi), so
+    two parts carry the reduction."""
+    v = Fraction(value_fixed, 1 << prec)
+    hi = float(v)
+    return [hi, float(v - Fraction(hi))]
+
+
 def taylor(kind, n_terms):
     """Coefficients (as doubles) of the series used by lgp_math.h."""
     from math import factorial
@@ -118,8 +128,9 @@ def sin_cos_fixed(r, prec=PREC):
 
 
 def sincos_table():
-    """64 pairs (sin(j pi/32), cos(j pi/32)), j = 0..63, rounded to double;
-    the multiples of pi/2 hold exact 0 and +-1."""
+    """80 pairs (sin(j pi/32), cos(j pi/32)), j = 0..79, rounded to double;
+    the multiples of pi/2 hold exact 0 and +-1.  Rows 64..79 repeat rows
+    0..15, so cos x = sin(x + pi/2), row j + 16, needs no wrap-around."""
     first = []
     for r in range(17):
         sf, cf = sin_cos_fixed(r)
@@ -132,7 +143,7 @@ def sincos_table():
         s, c = first[r]
         s, c = [(s, c), (c, -s), (-s, -c), (-c, s)][q]
         out.extend([s + 0.0, c + 0.0])  # no negative zeros
-    return out
+    return out + out[:32]
 
 
 def round_bits(v, bits):
@@ -188,9 +199,11 @@ def constants():
     ln2 = ln2_fixed()
     c = {
         "PIO32": split(pi, PREC + 5, 24),
+        "PIO32_FMA": split_fma(pi, PREC + 5),
         "INV_PIO32": float(Fraction(32 << PREC, pi)),
         "PIO32_SCALED64": float(Fraction(pi, 1 << (PREC + 5)) / (1 << 64)),
         "LN2": split(ln2, PREC, 32),
+        "LN2_FMA": split_fma(ln2, PREC),
         "INV_LN2": float(Fraction(1 << PREC, ln2)),
         # 1.5 2^52: adding it rounds |v| < 2^51 to an integer held in the low word
         "ROUND_MAGIC": float(3 << 51),

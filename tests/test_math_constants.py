@@ -35,7 +35,7 @@ def _header_values():
         vals[name] = nums if len(nums) > 1 else nums[0]
     words = re.search(r"TWO_OVER_PI_W\[8\] = \{([^}]*)\}", text).group(1)
     vals["TWO_OVER_PI_W"] = [int(w.strip().rstrip("u"), 16) for w in words.split(",")]
-    for name, n in (("SINCOS_T", 128), ("LOG_T", 182)):
+    for name, n in (("SINCOS_T", 160), ("LOG_T", 182)):
         table = re.search(name + r"\[%d\] = \{([^}]*)\}" % n, text).group(1)
         vals[name] = [float.fromhex(t.strip()) for t in table.split(",") if t.strip()]
     return vals
@@ -44,7 +44,8 @@ def _header_values():
 def test_constants_match_derivation():
     c = MC.constants()
     h = _header_values()
-    assert [h["PIO32_0"], h["PIO32_1"], h["PIO32_2"]] == c["PIO32"]
+    assert [h["PIO32_HI"], h["PIO32_LO"]] == c["PIO32_FMA"]
+    assert [h["LN2_HI"], h["LN2_LO"]] == c["LN2_FMA"]
     assert [h["LN2_0"], h["LN2_1"], h["LN2_2"]] == c["LN2"]
     assert h["INV_PIO32"] == c["INV_PIO32"]
     assert h["PIO32_SCALED64"] == c["PIO32_SCALED64"]
@@ -59,14 +60,47 @@ def test_constants_match_derivation():
     assert h["LOG_T"] == c["LOG_T"]
     # the splits are exact decompositions with short leading parts
     from fractions import Fraction
-    for parts, bits in ((c["PIO32"], 24), (c["LN2"], 32)):
+    for parts, bits in ((c["LN2"], 32),):
         for p in parts[:2]:
             m = Fraction(p)
             while m.denominator != 1:
                 m *= 2
             assert abs(m.numerator).bit_length() <= bits
     import math
-    assert sum(c["PIO32"]) == math.pi / 32 and sum(c["LN2"]) == math.log(2)
+    assert sum(c["LN2"]) == math.log(2)
+    # two-part splits: the leading part is the double nearest the value, the
+    # second the rest to within half its ulp
+    pi32 = Fraction(MC.pi_fixed(), 1 << (MC.PREC + 5))
+    ln2 = Fraction(MC.ln2_fixed(), 1 << MC.PREC)
+    for (hi, lo), v in ((c["PIO32_FMA"], pi32), (c["LN2_FMA"], ln2)):
+        assert hi == float(v) and lo == float(v - Fraction(hi))
+        assert abs(Fraction(hi) + Fraction(lo) - v) < v * Fraction(1, 2 ** 105)
+    assert c["PIO32_FMA"][0] == math.pi / 32 and c["LN2_FMA"][0] == math.log(2)
+
+
+def test_fma_reductions_are_exact():
+    """fma(-k, PIO32_HI, x) for the arguments that take Cody-Waite (|x| < 2^25,
+    k = rint(x 32/pi)) and fma(-k, LN2_HI, x) for exp's clamped arguments
+    (|x| <= 104.5, k = rint(x/ln 2)) are exact: the exact difference is a
+    double.  Checked in exact rationals on random arguments of every binade
+    and on the floats nearest multiples of pi/32 and ln 2."""
+    from fractions import Fraction
+    c = MC.constants()
+    rng = np.random.default_rng(9)
+    for (hi, _), inv, lim in ((c["PIO32_FMA"], c["INV_PIO32"], 2.0 ** 25),
+                              (c["LN2_FMA"], c["INV_LN2"], 104.5)):
+        xs = np.concatenate([
+            rng.uniform(-lim, lim, 20000),
+            np.exp2(rng.uniform(-30, np.log2(lim), 20000)) * rng.choice([-1, 1], 20000),
+            # floats next to k * hi, where the difference is smallest
+            np.array([k * hi for k in rng.integers(-int(lim / hi), int(lim / hi), 5000)]),
+        ]).astype(np.float32)
+        xs = xs[np.abs(xs) < lim]
+        for x in xs:
+            x = float(x)
+            k = float(np.rint(x * inv))
+            exact = Fraction(x) - Fraction(k) * Fraction(hi)
+            assert Fraction(float(exact)) == exact, (x, k)
 
 
 def test_sincos_table_identities():
