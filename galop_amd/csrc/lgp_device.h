@@ -111,15 +111,10 @@ __device__ __forceinline__ float op_cos(float b, galop_math::CoefPtr C) {
 __device__ __forceinline__ float op_exp(float b, galop_math::CoefPtr C) {
     return galop_math::expf_r(b, C);
 }
-// Arguments whose result needs no arithmetic, and that result (a word whose
-// every case is such skips the evaluation).
-__device__ __forceinline__ bool op_exp_special(float b) { return galop_math::expf_special(b); }
+// The result of an argument that needs no arithmetic (a word whose every
+// case is such skips the evaluation).
 __device__ __forceinline__ float op_exp_special_value(float b) {
     return galop_math::expf_special_value(b);
-}
-__device__ __forceinline__ bool op_plog_special(float b) { return galop_math::plogf_special(b); }
-__device__ __forceinline__ float op_plog_special_value(float b) {
-    return galop_math::plogf_special_value(b);
 }
 __device__ __forceinline__ float op_plog(float b, galop_math::CoefPtr C) {
     return galop_math::plogf_r(b, C);
@@ -169,6 +164,42 @@ __device__ __forceinline__ uint32_t wave_word(uint32_t chunk, int j) {
 }
 __device__ __forceinline__ float wave_bcast(float v, int j) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+
+// Largest |v[k]| over the K values of a lane, three at a time with
+// v_max3_f32 and absolute-value source modifiers (no canonicalisation: a NaN
+// may be ignored by the maximum or returned, and callers accept either).
+template <int K>
+__device__ __forceinline__ float max_abs(const float (&v)[K]) {
+    if constexpr (K == 1) {
+        return __builtin_fabsf(v[0]);
+    } else {
+        float m;
+        __asm__("v_max_f32_e64 %0, |%1|, |%2|" : "=v"(m) : "v"(v[0]), "v"(v[1]));
+#pragma unroll
+        for (int k = 2; k + 1 < K; k += 2)
+            __asm__("v_max3_f32 %0, %1, |%2|, |%3|" : "=v"(m) : "v"(m), "v"(v[k]), "v"(v[k + 1]));
+        if constexpr (K % 2 == 1)
+            __asm__("v_max_f32_e64 %0, %1, |%2|" : "=v"(m) : "v"(m), "v"(v[K - 1]));
+        return m;
+    }
+}
+
+// Smallest |v[k]| over the K values of a lane, like max_abs.
+template <int K>
+__device__ __forceinline__ float min_abs(const float (&v)[K]) {
+    if constexpr (K == 1) {
+        return __builtin_fabsf(v[0]);
+    } else {
+        float m;
+        __asm__("v_min_f32_e64 %0, |%1|, |%2|" : "=v"(m) : "v"(v[0]), "v"(v[1]));
+#pragma unroll
+        for (int k = 2; k + 1 < K; k += 2)
+            __asm__("v_min3_f32 %0, %1, |%2|, |%3|" : "=v"(m) : "v"(m), "v"(v[k]), "v"(v[k + 1]));
+        if constexpr (K % 2 == 1)
+            __asm__("v_min_f32_e64 %0, %1, |%2|" : "=v"(m) : "v"(m), "v"(v[K - 1]));
+        return m;
+    }
 }
 
 // Execute one instruction word on K register vectors.  Everything derived
@@ -234,24 +265,24 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
         _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]);    \
     } while (0)
 
-#define GALOP_UNARY(U, FN)                                                \
+// exp (EXT8 opcode 6): a word whose every argument has |x| > 104 (every
+// result +inf or +0, without arithmetic) takes the select-only path; the
+// smallest magnitude decides (min_abs: a NaN argument may or may not take
+// part, and both paths give it NaN).
+#define GALOP_EXP()                                                       \
     do {                                                                  \
         if (kc) {                                                         \
-            const float v = wave_bcast(cr.table, (U) * 16 + s2);          \
+            const float v = wave_bcast(cr.table, 2 * 16 + s2);            \
             _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, v);    \
         } else {                                                          \
             const galop_math::CoefPtr C = galop_math::coef_table();       \
             float bv[K], cv[K];                                           \
-            bool sp = true;                                               \
-            _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
-                bv[k] = r.get(k, s2);                                     \
-                sp = sp && FN##_special(bv[k]);                           \
-            }                                                             \
-            if (__all(sp)) {                                              \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) bv[k] = r.get(k, s2); \
+            if (__all(min_abs<K>(bv) > 104.0f)) {                         \
                 _Pragma("unroll") for (int k = 0; k < K; ++k)             \
-                    cv[k] = FN##_special_value(bv[k]);                    \
+                    cv[k] = op_exp_special_value(bv[k]);                  \
             } else {                                                      \
-                _Pragma("unroll") for (int k = 0; k < K; ++k) cv[k] = FN(bv[k], C); \
+                _Pragma("unroll") for (int k = 0; k < K; ++k) cv[k] = op_exp(bv[k], C); \
             }                                                             \
             _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]); \
         }                                                                 \
@@ -259,11 +290,11 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
 
 // sin / cos: the branch-free Cody-Waite path for the whole word unless some
 // lane holds, in some case, a finite argument of magnitude >= 2^25.  The
-// word's arguments are classified by the largest of their bits shifted left
-// by one (the magnitude's bits, in order): below 0x98 << 24 (|x| < 2^25, no
-// infinity or NaN) in every lane, the word takes Cody-Waite, or the
-// select-only path when every argument is a zero; otherwise each case is
-// classified on its own.
+// word's arguments are classified by their largest magnitude (max_abs):
+// below 2^25 in every lane, the word takes Cody-Waite, or the select-only
+// path when every argument is a zero; otherwise each case is classified on
+// its own.  A NaN argument may or may not take part in the maximum; both
+// paths give it NaN.
 #define GALOP_SINCOS(U, WANT_COS)                                          \
     do {                                                                  \
         if (kc) {                                                         \
@@ -272,13 +303,10 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
         } else {                                                          \
             const galop_math::CoefPtr C = galop_math::coef_table();       \
             float bv[K], cv[K];                                           \
-            uint32_t mag = 0;                                             \
-            _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
-                bv[k] = r.get(k, s2);                                     \
-                mag = max(mag, galop_math::f2u(bv[k]) << 1);              \
-            }                                                             \
-            bool cw = __all(mag < 0x98000000u);                           \
-            bool special = cw && __all(mag == 0u);                        \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) bv[k] = r.get(k, s2); \
+            const float mag = max_abs<K>(bv);                             \
+            bool cw = __all(mag < 0x1p25f);                               \
+            bool special = cw && __all(mag == 0.0f);                      \
             bool mixed = false;                                           \
             if (!cw) {                                                    \
                 bool ph = false, sp = true;                               \
@@ -305,8 +333,8 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                     _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) { \
                         uint32_t j;                                       \
                         t[i] = galop_math::sincos_reduce(bv[g + i], j, C); \
-                        ts[i] = galop_math::SINCOS_T[2 * j + 32 * (WANT_COS)]; \
-                        tc[i] = galop_math::SINCOS_T[2 * j + 1 + 32 * (WANT_COS)]; \
+                        ts[i] = galop_math::SINCOS_T[2 * j + 128 * (WANT_COS)]; \
+                        tc[i] = galop_math::SINCOS_T[2 * j + 1 + 128 * (WANT_COS)]; \
                     }                                                     \
                     _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) \
                         cv[g + i] = galop_math::sincos_finish(bv[g + i], WANT_COS, t[i], \
@@ -318,10 +346,13 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
         }                                                                 \
     } while (0)
 
-// protected log (EXT8 opcode 7): like GALOP_UNARY, with the table rows of
-// kRowGroup cases loaded before any is used.  The arithmetic runs on |b| for
-// every case (log_reduce keeps the row in range for any bits), and 0,
-// infinities and NaN take their fixed results at the end, as in plogf_r.
+// protected log (EXT8 opcode 7): a word whose every argument is a zero (its
+// result +0, by the largest magnitude: a NaN may or may not take part, and
+// both paths give it NaN) takes the select-only path; otherwise the table
+// rows of kRowGroup cases are loaded before any is used.  The arithmetic runs
+// on |b| for every case (log_reduce keeps the row in range for any bits),
+// and 0, infinities and NaN take their fixed results at the end, as in
+// plogf_r.
 #define GALOP_PLOG()                                                      \
     do {                                                                  \
         if (kc) {                                                         \
@@ -330,12 +361,8 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
         } else {                                                          \
             const galop_math::CoefPtr C = galop_math::coef_table();       \
             float bv[K], cv[K];                                           \
-            bool sp = true;                                               \
-            _Pragma("unroll") for (int k = 0; k < K; ++k) {               \
-                bv[k] = r.get(k, s2);                                     \
-                sp = sp && galop_math::plogf_special(bv[k]);              \
-            }                                                             \
-            if (__all(sp)) {                                              \
+            _Pragma("unroll") for (int k = 0; k < K; ++k) bv[k] = r.get(k, s2); \
+            if (__all(max_abs<K>(bv) == 0.0f)) {                          \
                 _Pragma("unroll") for (int k = 0; k < K; ++k)             \
                     cv[k] = galop_math::plogf_special_value(bv[k]);       \
             } else {                                                      \
@@ -399,7 +426,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
     case 4: GALOP_SINCOS(0, 0); break;
     case 5: GALOP_SINCOS(1, 1); break;
     case 6:
-        if constexpr (EXT) GALOP_UNARY(2, op_exp);
+        if constexpr (EXT) GALOP_EXP();
         else GALOP_BIN((a < b) ? 1.0f : 0.0f);
         break;
     default:
@@ -409,7 +436,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
     }
 #undef GALOP_BIN
 #undef GALOP_DIV
-#undef GALOP_UNARY
+#undef GALOP_EXP
 #undef GALOP_SINCOS
 #undef GALOP_PLOG
 #undef GALOP_GROUP_FENCE

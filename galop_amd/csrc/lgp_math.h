@@ -24,14 +24,15 @@
 //        table-free Taylor form: its per-lane table load stalls the wave
 //        behind the prefetch of the next program chunk.)
 //   sin, cos
-//        x = k pi/32 + t, |t| <= pi/64: |x| < 2^25 by Cody-Waite with pi/32
-//        split in three and k = rint(x 32/pi) rounded by adding 1.5 2^52
-//        (the low word of the sum is k mod 2^32); |x| >= 2^25 by Payne-Hanek -- the 24-bit mantissa
-//        times a 128-bit window of the binary expansion of 2/pi gives k mod 64
-//        and a 122-bit fraction.  Then sin x = S_j cos t + C_j sin t with
-//        (S_j, C_j) = (sin, cos)(j pi/32) from a 64-entry table, j = k mod 64
-//        (cos x is j + 16), and Taylor series of sin t to t^7 and cos t - 1
-//        to t^8.
+//        x = k pi/128 + t, |t| <= pi/256: |x| < 2^25 by Cody-Waite with
+//        pi/128 in two parts for two FMAs and k = rint(x 128/pi) rounded by
+//        adding 1.5 2^52 (the low word of the sum is k mod 2^32); |x| >= 2^25
+//        by Payne-Hanek -- the 24-bit mantissa times a 128-bit window of the
+//        binary expansion of 2/pi gives k mod 256 and a 120-bit fraction.
+//        Then sin x = S_j cos t + C_j sin t with (S_j, C_j) =
+//        (sin, cos)(j pi/128) from a 256-entry table, j = k mod 256 (cos x is
+//        row j + 64), and Taylor series of sin t to t^5 and cos t - 1 to
+//        t^6.
 // Everything is IEEE double add/mul/fma, integer operations on the bits,
 // ldexp and conversions, so the same source compiled for the host gives the
 // same bits (NaN payloads aside).
@@ -50,15 +51,15 @@
 
 namespace galop_math {
 
-// pi/32 = PIO32_HI + PIO32_LO to about 2^-110: PIO32_HI is pi/32 rounded to
-// a double, PIO32_LO the rounded rest.  For a float x with |x| < 2^25 and
-// k = rint(x 32/pi) (|k| < 2^28.4), x - k PIO32_HI is a multiple of 2^-56
-// smaller than 2^-4 (or x itself when k = 0), so fma(-k, PIO32_HI, x) is
-// exact and the reduction takes two FMAs.
-constexpr double PIO32_HI = 0x1.921fb54442d18p-4;
-constexpr double PIO32_LO = 0x1.1a62633145c07p-58;
-constexpr double INV_PIO32 = 0x1.45f306dc9c883p+3;       // 32/pi
-constexpr double PIO32_SCALED64 = 0x1.921fb54442d18p-68;  // pi/32 * 2^-64
+// pi/128 = PIO128_HI + PIO128_LO to about 2^-112: PIO128_HI is pi/128
+// rounded to a double, PIO128_LO the rounded rest.  For a float x with
+// |x| < 2^25 and k = rint(x 128/pi) (|k| < 2^30.4), x - k PIO128_HI is a
+// multiple of 2^-58 smaller than 2^-6 (or x itself when k = 0), so
+// fma(-k, PIO128_HI, x) is exact and the reduction takes two FMAs.
+constexpr double PIO128_HI = 0x1.921fb54442d18p-6;
+constexpr double PIO128_LO = 0x1.1a62633145c07p-60;
+constexpr double INV_PIO128 = 0x1.45f306dc9c883p+5;      // 128/pi
+constexpr double PIO128_SCALED64 = 0x1.921fb54442d18p-70;  // pi/128 * 2^-64
 // ln 2 = LN2_0 + LN2_1 + LN2_2; LN2_{0,1} carry 32 bits each.
 constexpr double LN2_0 = 0x1.62e42ff000000p-1;
 constexpr double LN2_1 = -0x1.718432a200000p-35;
@@ -81,98 +82,338 @@ constexpr double EXP_C[13] = {
     0x1.5555555555555p-5, 0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13,
     0x1.a01a01a01a01ap-16, 0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22, 0x1.ae64567f544e4p-26,
     0x1.1eed8eff8d898p-29};
-// (-1)^k / (2k+1)!, k = 1..3
-constexpr double SINT_C[3] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13};
-// (-1)^k / (2k)!, k = 1..4
-constexpr double COST_C[4] = {-0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-16};
+// (-1)^k / (2k+1)!, k = 1..2
+constexpr double SINT_C[2] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7};
+// (-1)^k / (2k)!, k = 1..3
+constexpr double COST_C[3] = {-0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10};
 // (-1)^(n+1)/n, n = 2..7  (log1p r = r + r^2 sum_n (-1)^(n+1)/n r^(n-2))
 constexpr double LOG1P_C[6] = {-0x1.0000000000000p-1, 0x1.5555555555555p-2, -0x1.0000000000000p-2, 0x1.999999999999ap-3, -0x1.5555555555555p-3, 0x1.2492492492492p-3};
 // bits 2^-1 .. 2^-256 of 2/pi (word 0 first)
 GM_CONST uint32_t TWO_OVER_PI_W[8] = {0xA2F9836Eu, 0x4E441529u, 0xFC2757D1u, 0xF534DDC0u,
                                       0xDB629599u, 0x3C439041u, 0xFE5163ABu, 0xDEBBC561u};
-// (sin, cos)(j pi/32), j = 0..79, interleaved; rows 64..79 repeat rows 0..15
-// so that cos (row j + 16) needs no wrap-around
-GM_CONST double SINCOS_T[160] = {
+// (sin, cos)(j pi/128), j = 0..319, interleaved; rows 256..319 repeat rows
+// 0..63 so that cos (row j + 64) needs no wrap-around
+GM_CONST double SINCOS_T[640] = {
     0x0.0p+0, 0x1.0000000000000p+0,
+    0x1.92155f7a3667ep-6, 0x1.ffd886084cd0dp-1,
+    0x1.91f65f10dd814p-5, 0x1.ff621e3796d7ep-1,
+    0x1.2d52092ce19f6p-4, 0x1.fe9cdad01883ap-1,
     0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1,
+    0x1.f564e56a9730ep-4, 0x1.fc26470e19fd3p-1,
+    0x1.2c8106e8e613ap-3, 0x1.fa7557f08a517p-1,
+    0x1.5e214448b3fc6p-3, 0x1.f8764fa714ba9p-1,
     0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1,
+    0x1.c0b826a7e4f63p-3, 0x1.f38f3ac64e589p-1,
+    0x1.f19f97b215f1bp-3, 0x1.f0a7efb9230d7p-1,
+    0x1.111d262b1f677p-2, 0x1.ed740e7684963p-1,
     0x1.294062ed59f06p-2, 0x1.e9f4156c62ddap-1,
+    0x1.4135c94176601p-2, 0x1.e6288ec48e112p-1,
+    0x1.58f9a75ab1fddp-2, 0x1.e212104f686e5p-1,
+    0x1.7088530fa459fp-2, 0x1.ddb13b6ccc23cp-1,
     0x1.87de2a6aea963p-2, 0x1.d906bcf328d46p-1,
+    0x1.9ef7943a8ed8ap-2, 0x1.d4134d14dc93ap-1,
+    0x1.b5d1009e15cc0p-2, 0x1.ced7af43cc773p-1,
+    0x1.cc66e9931c45ep-2, 0x1.c954b213411f5p-1,
     0x1.e2b5d3806f63bp-2, 0x1.c38b2f180bdb1p-1,
+    0x1.f8ba4dbf89abap-2, 0x1.bd7c0ac6f952ap-1,
+    0x1.073879922ffeep-1, 0x1.b728345196e3ep-1,
+    0x1.11eb3541b4b23p-1, 0x1.b090a58150200p-1,
     0x1.1c73b39ae68c8p-1, 0x1.a9b66290ea1a3p-1,
+    0x1.26d054cdd12dfp-1, 0x1.a29a7a0462782p-1,
+    0x1.30ff7fce17035p-1, 0x1.9b3e047f38741p-1,
+    0x1.3affa292050b9p-1, 0x1.93a22499263fbp-1,
     0x1.44cf325091dd6p-1, 0x1.8bc806b151741p-1,
+    0x1.4e6cabbe3e5e9p-1, 0x1.83b0e0bff976ep-1,
+    0x1.57d69348ceca0p-1, 0x1.7b5df226aafafp-1,
+    0x1.610b7551d2cdfp-1, 0x1.72d0837efff96p-1,
     0x1.6a09e667f3bcdp-1, 0x1.6a09e667f3bcdp-1,
+    0x1.72d0837efff96p-1, 0x1.610b7551d2cdfp-1,
+    0x1.7b5df226aafafp-1, 0x1.57d69348ceca0p-1,
+    0x1.83b0e0bff976ep-1, 0x1.4e6cabbe3e5e9p-1,
     0x1.8bc806b151741p-1, 0x1.44cf325091dd6p-1,
+    0x1.93a22499263fbp-1, 0x1.3affa292050b9p-1,
+    0x1.9b3e047f38741p-1, 0x1.30ff7fce17035p-1,
+    0x1.a29a7a0462782p-1, 0x1.26d054cdd12dfp-1,
     0x1.a9b66290ea1a3p-1, 0x1.1c73b39ae68c8p-1,
+    0x1.b090a58150200p-1, 0x1.11eb3541b4b23p-1,
+    0x1.b728345196e3ep-1, 0x1.073879922ffeep-1,
+    0x1.bd7c0ac6f952ap-1, 0x1.f8ba4dbf89abap-2,
     0x1.c38b2f180bdb1p-1, 0x1.e2b5d3806f63bp-2,
+    0x1.c954b213411f5p-1, 0x1.cc66e9931c45ep-2,
+    0x1.ced7af43cc773p-1, 0x1.b5d1009e15cc0p-2,
+    0x1.d4134d14dc93ap-1, 0x1.9ef7943a8ed8ap-2,
     0x1.d906bcf328d46p-1, 0x1.87de2a6aea963p-2,
+    0x1.ddb13b6ccc23cp-1, 0x1.7088530fa459fp-2,
+    0x1.e212104f686e5p-1, 0x1.58f9a75ab1fddp-2,
+    0x1.e6288ec48e112p-1, 0x1.4135c94176601p-2,
     0x1.e9f4156c62ddap-1, 0x1.294062ed59f06p-2,
+    0x1.ed740e7684963p-1, 0x1.111d262b1f677p-2,
+    0x1.f0a7efb9230d7p-1, 0x1.f19f97b215f1bp-3,
+    0x1.f38f3ac64e589p-1, 0x1.c0b826a7e4f63p-3,
     0x1.f6297cff75cb0p-1, 0x1.8f8b83c69a60bp-3,
+    0x1.f8764fa714ba9p-1, 0x1.5e214448b3fc6p-3,
+    0x1.fa7557f08a517p-1, 0x1.2c8106e8e613ap-3,
+    0x1.fc26470e19fd3p-1, 0x1.f564e56a9730ep-4,
     0x1.fd88da3d12526p-1, 0x1.917a6bc29b42cp-4,
+    0x1.fe9cdad01883ap-1, 0x1.2d52092ce19f6p-4,
+    0x1.ff621e3796d7ep-1, 0x1.91f65f10dd814p-5,
+    0x1.ffd886084cd0dp-1, 0x1.92155f7a3667ep-6,
     0x1.0000000000000p+0, 0x0.0p+0,
+    0x1.ffd886084cd0dp-1, -0x1.92155f7a3667ep-6,
+    0x1.ff621e3796d7ep-1, -0x1.91f65f10dd814p-5,
+    0x1.fe9cdad01883ap-1, -0x1.2d52092ce19f6p-4,
     0x1.fd88da3d12526p-1, -0x1.917a6bc29b42cp-4,
+    0x1.fc26470e19fd3p-1, -0x1.f564e56a9730ep-4,
+    0x1.fa7557f08a517p-1, -0x1.2c8106e8e613ap-3,
+    0x1.f8764fa714ba9p-1, -0x1.5e214448b3fc6p-3,
     0x1.f6297cff75cb0p-1, -0x1.8f8b83c69a60bp-3,
+    0x1.f38f3ac64e589p-1, -0x1.c0b826a7e4f63p-3,
+    0x1.f0a7efb9230d7p-1, -0x1.f19f97b215f1bp-3,
+    0x1.ed740e7684963p-1, -0x1.111d262b1f677p-2,
     0x1.e9f4156c62ddap-1, -0x1.294062ed59f06p-2,
+    0x1.e6288ec48e112p-1, -0x1.4135c94176601p-2,
+    0x1.e212104f686e5p-1, -0x1.58f9a75ab1fddp-2,
+    0x1.ddb13b6ccc23cp-1, -0x1.7088530fa459fp-2,
     0x1.d906bcf328d46p-1, -0x1.87de2a6aea963p-2,
+    0x1.d4134d14dc93ap-1, -0x1.9ef7943a8ed8ap-2,
+    0x1.ced7af43cc773p-1, -0x1.b5d1009e15cc0p-2,
+    0x1.c954b213411f5p-1, -0x1.cc66e9931c45ep-2,
     0x1.c38b2f180bdb1p-1, -0x1.e2b5d3806f63bp-2,
+    0x1.bd7c0ac6f952ap-1, -0x1.f8ba4dbf89abap-2,
+    0x1.b728345196e3ep-1, -0x1.073879922ffeep-1,
+    0x1.b090a58150200p-1, -0x1.11eb3541b4b23p-1,
     0x1.a9b66290ea1a3p-1, -0x1.1c73b39ae68c8p-1,
+    0x1.a29a7a0462782p-1, -0x1.26d054cdd12dfp-1,
+    0x1.9b3e047f38741p-1, -0x1.30ff7fce17035p-1,
+    0x1.93a22499263fbp-1, -0x1.3affa292050b9p-1,
     0x1.8bc806b151741p-1, -0x1.44cf325091dd6p-1,
+    0x1.83b0e0bff976ep-1, -0x1.4e6cabbe3e5e9p-1,
+    0x1.7b5df226aafafp-1, -0x1.57d69348ceca0p-1,
+    0x1.72d0837efff96p-1, -0x1.610b7551d2cdfp-1,
     0x1.6a09e667f3bcdp-1, -0x1.6a09e667f3bcdp-1,
+    0x1.610b7551d2cdfp-1, -0x1.72d0837efff96p-1,
+    0x1.57d69348ceca0p-1, -0x1.7b5df226aafafp-1,
+    0x1.4e6cabbe3e5e9p-1, -0x1.83b0e0bff976ep-1,
     0x1.44cf325091dd6p-1, -0x1.8bc806b151741p-1,
+    0x1.3affa292050b9p-1, -0x1.93a22499263fbp-1,
+    0x1.30ff7fce17035p-1, -0x1.9b3e047f38741p-1,
+    0x1.26d054cdd12dfp-1, -0x1.a29a7a0462782p-1,
     0x1.1c73b39ae68c8p-1, -0x1.a9b66290ea1a3p-1,
+    0x1.11eb3541b4b23p-1, -0x1.b090a58150200p-1,
+    0x1.073879922ffeep-1, -0x1.b728345196e3ep-1,
+    0x1.f8ba4dbf89abap-2, -0x1.bd7c0ac6f952ap-1,
     0x1.e2b5d3806f63bp-2, -0x1.c38b2f180bdb1p-1,
+    0x1.cc66e9931c45ep-2, -0x1.c954b213411f5p-1,
+    0x1.b5d1009e15cc0p-2, -0x1.ced7af43cc773p-1,
+    0x1.9ef7943a8ed8ap-2, -0x1.d4134d14dc93ap-1,
     0x1.87de2a6aea963p-2, -0x1.d906bcf328d46p-1,
+    0x1.7088530fa459fp-2, -0x1.ddb13b6ccc23cp-1,
+    0x1.58f9a75ab1fddp-2, -0x1.e212104f686e5p-1,
+    0x1.4135c94176601p-2, -0x1.e6288ec48e112p-1,
     0x1.294062ed59f06p-2, -0x1.e9f4156c62ddap-1,
+    0x1.111d262b1f677p-2, -0x1.ed740e7684963p-1,
+    0x1.f19f97b215f1bp-3, -0x1.f0a7efb9230d7p-1,
+    0x1.c0b826a7e4f63p-3, -0x1.f38f3ac64e589p-1,
     0x1.8f8b83c69a60bp-3, -0x1.f6297cff75cb0p-1,
+    0x1.5e214448b3fc6p-3, -0x1.f8764fa714ba9p-1,
+    0x1.2c8106e8e613ap-3, -0x1.fa7557f08a517p-1,
+    0x1.f564e56a9730ep-4, -0x1.fc26470e19fd3p-1,
     0x1.917a6bc29b42cp-4, -0x1.fd88da3d12526p-1,
+    0x1.2d52092ce19f6p-4, -0x1.fe9cdad01883ap-1,
+    0x1.91f65f10dd814p-5, -0x1.ff621e3796d7ep-1,
+    0x1.92155f7a3667ep-6, -0x1.ffd886084cd0dp-1,
     0x0.0p+0, -0x1.0000000000000p+0,
+    -0x1.92155f7a3667ep-6, -0x1.ffd886084cd0dp-1,
+    -0x1.91f65f10dd814p-5, -0x1.ff621e3796d7ep-1,
+    -0x1.2d52092ce19f6p-4, -0x1.fe9cdad01883ap-1,
     -0x1.917a6bc29b42cp-4, -0x1.fd88da3d12526p-1,
+    -0x1.f564e56a9730ep-4, -0x1.fc26470e19fd3p-1,
+    -0x1.2c8106e8e613ap-3, -0x1.fa7557f08a517p-1,
+    -0x1.5e214448b3fc6p-3, -0x1.f8764fa714ba9p-1,
     -0x1.8f8b83c69a60bp-3, -0x1.f6297cff75cb0p-1,
+    -0x1.c0b826a7e4f63p-3, -0x1.f38f3ac64e589p-1,
+    -0x1.f19f97b215f1bp-3, -0x1.f0a7efb9230d7p-1,
+    -0x1.111d262b1f677p-2, -0x1.ed740e7684963p-1,
     -0x1.294062ed59f06p-2, -0x1.e9f4156c62ddap-1,
+    -0x1.4135c94176601p-2, -0x1.e6288ec48e112p-1,
+    -0x1.58f9a75ab1fddp-2, -0x1.e212104f686e5p-1,
+    -0x1.7088530fa459fp-2, -0x1.ddb13b6ccc23cp-1,
     -0x1.87de2a6aea963p-2, -0x1.d906bcf328d46p-1,
+    -0x1.9ef7943a8ed8ap-2, -0x1.d4134d14dc93ap-1,
+    -0x1.b5d1009e15cc0p-2, -0x1.ced7af43cc773p-1,
+    -0x1.cc66e9931c45ep-2, -0x1.c954b213411f5p-1,
     -0x1.e2b5d3806f63bp-2, -0x1.c38b2f180bdb1p-1,
+    -0x1.f8ba4dbf89abap-2, -0x1.bd7c0ac6f952ap-1,
+    -0x1.073879922ffeep-1, -0x1.b728345196e3ep-1,
+    -0x1.11eb3541b4b23p-1, -0x1.b090a58150200p-1,
     -0x1.1c73b39ae68c8p-1, -0x1.a9b66290ea1a3p-1,
+    -0x1.26d054cdd12dfp-1, -0x1.a29a7a0462782p-1,
+    -0x1.30ff7fce17035p-1, -0x1.9b3e047f38741p-1,
+    -0x1.3affa292050b9p-1, -0x1.93a22499263fbp-1,
     -0x1.44cf325091dd6p-1, -0x1.8bc806b151741p-1,
+    -0x1.4e6cabbe3e5e9p-1, -0x1.83b0e0bff976ep-1,
+    -0x1.57d69348ceca0p-1, -0x1.7b5df226aafafp-1,
+    -0x1.610b7551d2cdfp-1, -0x1.72d0837efff96p-1,
     -0x1.6a09e667f3bcdp-1, -0x1.6a09e667f3bcdp-1,
+    -0x1.72d0837efff96p-1, -0x1.610b7551d2cdfp-1,
+    -0x1.7b5df226aafafp-1, -0x1.57d69348ceca0p-1,
+    -0x1.83b0e0bff976ep-1, -0x1.4e6cabbe3e5e9p-1,
     -0x1.8bc806b151741p-1, -0x1.44cf325091dd6p-1,
+    -0x1.93a22499263fbp-1, -0x1.3affa292050b9p-1,
+    -0x1.9b3e047f38741p-1, -0x1.30ff7fce17035p-1,
+    -0x1.a29a7a0462782p-1, -0x1.26d054cdd12dfp-1,
     -0x1.a9b66290ea1a3p-1, -0x1.1c73b39ae68c8p-1,
+    -0x1.b090a58150200p-1, -0x1.11eb3541b4b23p-1,
+    -0x1.b728345196e3ep-1, -0x1.073879922ffeep-1,
+    -0x1.bd7c0ac6f952ap-1, -0x1.f8ba4dbf89abap-2,
     -0x1.c38b2f180bdb1p-1, -0x1.e2b5d3806f63bp-2,
+    -0x1.c954b213411f5p-1, -0x1.cc66e9931c45ep-2,
+    -0x1.ced7af43cc773p-1, -0x1.b5d1009e15cc0p-2,
+    -0x1.d4134d14dc93ap-1, -0x1.9ef7943a8ed8ap-2,
     -0x1.d906bcf328d46p-1, -0x1.87de2a6aea963p-2,
+    -0x1.ddb13b6ccc23cp-1, -0x1.7088530fa459fp-2,
+    -0x1.e212104f686e5p-1, -0x1.58f9a75ab1fddp-2,
+    -0x1.e6288ec48e112p-1, -0x1.4135c94176601p-2,
     -0x1.e9f4156c62ddap-1, -0x1.294062ed59f06p-2,
+    -0x1.ed740e7684963p-1, -0x1.111d262b1f677p-2,
+    -0x1.f0a7efb9230d7p-1, -0x1.f19f97b215f1bp-3,
+    -0x1.f38f3ac64e589p-1, -0x1.c0b826a7e4f63p-3,
     -0x1.f6297cff75cb0p-1, -0x1.8f8b83c69a60bp-3,
+    -0x1.f8764fa714ba9p-1, -0x1.5e214448b3fc6p-3,
+    -0x1.fa7557f08a517p-1, -0x1.2c8106e8e613ap-3,
+    -0x1.fc26470e19fd3p-1, -0x1.f564e56a9730ep-4,
     -0x1.fd88da3d12526p-1, -0x1.917a6bc29b42cp-4,
+    -0x1.fe9cdad01883ap-1, -0x1.2d52092ce19f6p-4,
+    -0x1.ff621e3796d7ep-1, -0x1.91f65f10dd814p-5,
+    -0x1.ffd886084cd0dp-1, -0x1.92155f7a3667ep-6,
     -0x1.0000000000000p+0, 0x0.0p+0,
+    -0x1.ffd886084cd0dp-1, 0x1.92155f7a3667ep-6,
+    -0x1.ff621e3796d7ep-1, 0x1.91f65f10dd814p-5,
+    -0x1.fe9cdad01883ap-1, 0x1.2d52092ce19f6p-4,
     -0x1.fd88da3d12526p-1, 0x1.917a6bc29b42cp-4,
+    -0x1.fc26470e19fd3p-1, 0x1.f564e56a9730ep-4,
+    -0x1.fa7557f08a517p-1, 0x1.2c8106e8e613ap-3,
+    -0x1.f8764fa714ba9p-1, 0x1.5e214448b3fc6p-3,
     -0x1.f6297cff75cb0p-1, 0x1.8f8b83c69a60bp-3,
+    -0x1.f38f3ac64e589p-1, 0x1.c0b826a7e4f63p-3,
+    -0x1.f0a7efb9230d7p-1, 0x1.f19f97b215f1bp-3,
+    -0x1.ed740e7684963p-1, 0x1.111d262b1f677p-2,
     -0x1.e9f4156c62ddap-1, 0x1.294062ed59f06p-2,
+    -0x1.e6288ec48e112p-1, 0x1.4135c94176601p-2,
+    -0x1.e212104f686e5p-1, 0x1.58f9a75ab1fddp-2,
+    -0x1.ddb13b6ccc23cp-1, 0x1.7088530fa459fp-2,
     -0x1.d906bcf328d46p-1, 0x1.87de2a6aea963p-2,
+    -0x1.d4134d14dc93ap-1, 0x1.9ef7943a8ed8ap-2,
+    -0x1.ced7af43cc773p-1, 0x1.b5d1009e15cc0p-2,
+    -0x1.c954b213411f5p-1, 0x1.cc66e9931c45ep-2,
     -0x1.c38b2f180bdb1p-1, 0x1.e2b5d3806f63bp-2,
+    -0x1.bd7c0ac6f952ap-1, 0x1.f8ba4dbf89abap-2,
+    -0x1.b728345196e3ep-1, 0x1.073879922ffeep-1,
+    -0x1.b090a58150200p-1, 0x1.11eb3541b4b23p-1,
     -0x1.a9b66290ea1a3p-1, 0x1.1c73b39ae68c8p-1,
+    -0x1.a29a7a0462782p-1, 0x1.26d054cdd12dfp-1,
+    -0x1.9b3e047f38741p-1, 0x1.30ff7fce17035p-1,
+    -0x1.93a22499263fbp-1, 0x1.3affa292050b9p-1,
     -0x1.8bc806b151741p-1, 0x1.44cf325091dd6p-1,
+    -0x1.83b0e0bff976ep-1, 0x1.4e6cabbe3e5e9p-1,
+    -0x1.7b5df226aafafp-1, 0x1.57d69348ceca0p-1,
+    -0x1.72d0837efff96p-1, 0x1.610b7551d2cdfp-1,
     -0x1.6a09e667f3bcdp-1, 0x1.6a09e667f3bcdp-1,
+    -0x1.610b7551d2cdfp-1, 0x1.72d0837efff96p-1,
+    -0x1.57d69348ceca0p-1, 0x1.7b5df226aafafp-1,
+    -0x1.4e6cabbe3e5e9p-1, 0x1.83b0e0bff976ep-1,
     -0x1.44cf325091dd6p-1, 0x1.8bc806b151741p-1,
+    -0x1.3affa292050b9p-1, 0x1.93a22499263fbp-1,
+    -0x1.30ff7fce17035p-1, 0x1.9b3e047f38741p-1,
+    -0x1.26d054cdd12dfp-1, 0x1.a29a7a0462782p-1,
     -0x1.1c73b39ae68c8p-1, 0x1.a9b66290ea1a3p-1,
+    -0x1.11eb3541b4b23p-1, 0x1.b090a58150200p-1,
+    -0x1.073879922ffeep-1, 0x1.b728345196e3ep-1,
+    -0x1.f8ba4dbf89abap-2, 0x1.bd7c0ac6f952ap-1,
     -0x1.e2b5d3806f63bp-2, 0x1.c38b2f180bdb1p-1,
+    -0x1.cc66e9931c45ep-2, 0x1.c954b213411f5p-1,
+    -0x1.b5d1009e15cc0p-2, 0x1.ced7af43cc773p-1,
+    -0x1.9ef7943a8ed8ap-2, 0x1.d4134d14dc93ap-1,
     -0x1.87de2a6aea963p-2, 0x1.d906bcf328d46p-1,
+    -0x1.7088530fa459fp-2, 0x1.ddb13b6ccc23cp-1,
+    -0x1.58f9a75ab1fddp-2, 0x1.e212104f686e5p-1,
+    -0x1.4135c94176601p-2, 0x1.e6288ec48e112p-1,
     -0x1.294062ed59f06p-2, 0x1.e9f4156c62ddap-1,
+    -0x1.111d262b1f677p-2, 0x1.ed740e7684963p-1,
+    -0x1.f19f97b215f1bp-3, 0x1.f0a7efb9230d7p-1,
+    -0x1.c0b826a7e4f63p-3, 0x1.f38f3ac64e589p-1,
     -0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1,
+    -0x1.5e214448b3fc6p-3, 0x1.f8764fa714ba9p-1,
+    -0x1.2c8106e8e613ap-3, 0x1.fa7557f08a517p-1,
+    -0x1.f564e56a9730ep-4, 0x1.fc26470e19fd3p-1,
     -0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1,
+    -0x1.2d52092ce19f6p-4, 0x1.fe9cdad01883ap-1,
+    -0x1.91f65f10dd814p-5, 0x1.ff621e3796d7ep-1,
+    -0x1.92155f7a3667ep-6, 0x1.ffd886084cd0dp-1,
     0x0.0p+0, 0x1.0000000000000p+0,
+    0x1.92155f7a3667ep-6, 0x1.ffd886084cd0dp-1,
+    0x1.91f65f10dd814p-5, 0x1.ff621e3796d7ep-1,
+    0x1.2d52092ce19f6p-4, 0x1.fe9cdad01883ap-1,
     0x1.917a6bc29b42cp-4, 0x1.fd88da3d12526p-1,
+    0x1.f564e56a9730ep-4, 0x1.fc26470e19fd3p-1,
+    0x1.2c8106e8e613ap-3, 0x1.fa7557f08a517p-1,
+    0x1.5e214448b3fc6p-3, 0x1.f8764fa714ba9p-1,
     0x1.8f8b83c69a60bp-3, 0x1.f6297cff75cb0p-1,
+    0x1.c0b826a7e4f63p-3, 0x1.f38f3ac64e589p-1,
+    0x1.f19f97b215f1bp-3, 0x1.f0a7efb9230d7p-1,
+    0x1.111d262b1f677p-2, 0x1.ed740e7684963p-1,
     0x1.294062ed59f06p-2, 0x1.e9f4156c62ddap-1,
+    0x1.4135c94176601p-2, 0x1.e6288ec48e112p-1,
+    0x1.58f9a75ab1fddp-2, 0x1.e212104f686e5p-1,
+    0x1.7088530fa459fp-2, 0x1.ddb13b6ccc23cp-1,
     0x1.87de2a6aea963p-2, 0x1.d906bcf328d46p-1,
+    0x1.9ef7943a8ed8ap-2, 0x1.d4134d14dc93ap-1,
+    0x1.b5d1009e15cc0p-2, 0x1.ced7af43cc773p-1,
+    0x1.cc66e9931c45ep-2, 0x1.c954b213411f5p-1,
     0x1.e2b5d3806f63bp-2, 0x1.c38b2f180bdb1p-1,
+    0x1.f8ba4dbf89abap-2, 0x1.bd7c0ac6f952ap-1,
+    0x1.073879922ffeep-1, 0x1.b728345196e3ep-1,
+    0x1.11eb3541b4b23p-1, 0x1.b090a58150200p-1,
     0x1.1c73b39ae68c8p-1, 0x1.a9b66290ea1a3p-1,
+    0x1.26d054cdd12dfp-1, 0x1.a29a7a0462782p-1,
+    0x1.30ff7fce17035p-1, 0x1.9b3e047f38741p-1,
+    0x1.3affa292050b9p-1, 0x1.93a22499263fbp-1,
     0x1.44cf325091dd6p-1, 0x1.8bc806b151741p-1,
+    0x1.4e6cabbe3e5e9p-1, 0x1.83b0e0bff976ep-1,
+    0x1.57d69348ceca0p-1, 0x1.7b5df226aafafp-1,
+    0x1.610b7551d2cdfp-1, 0x1.72d0837efff96p-1,
     0x1.6a09e667f3bcdp-1, 0x1.6a09e667f3bcdp-1,
+    0x1.72d0837efff96p-1, 0x1.610b7551d2cdfp-1,
+    0x1.7b5df226aafafp-1, 0x1.57d69348ceca0p-1,
+    0x1.83b0e0bff976ep-1, 0x1.4e6cabbe3e5e9p-1,
     0x1.8bc806b151741p-1, 0x1.44cf325091dd6p-1,
+    0x1.93a22499263fbp-1, 0x1.3affa292050b9p-1,
+    0x1.9b3e047f38741p-1, 0x1.30ff7fce17035p-1,
+    0x1.a29a7a0462782p-1, 0x1.26d054cdd12dfp-1,
     0x1.a9b66290ea1a3p-1, 0x1.1c73b39ae68c8p-1,
+    0x1.b090a58150200p-1, 0x1.11eb3541b4b23p-1,
+    0x1.b728345196e3ep-1, 0x1.073879922ffeep-1,
+    0x1.bd7c0ac6f952ap-1, 0x1.f8ba4dbf89abap-2,
     0x1.c38b2f180bdb1p-1, 0x1.e2b5d3806f63bp-2,
+    0x1.c954b213411f5p-1, 0x1.cc66e9931c45ep-2,
+    0x1.ced7af43cc773p-1, 0x1.b5d1009e15cc0p-2,
+    0x1.d4134d14dc93ap-1, 0x1.9ef7943a8ed8ap-2,
     0x1.d906bcf328d46p-1, 0x1.87de2a6aea963p-2,
+    0x1.ddb13b6ccc23cp-1, 0x1.7088530fa459fp-2,
+    0x1.e212104f686e5p-1, 0x1.58f9a75ab1fddp-2,
+    0x1.e6288ec48e112p-1, 0x1.4135c94176601p-2,
     0x1.e9f4156c62ddap-1, 0x1.294062ed59f06p-2,
+    0x1.ed740e7684963p-1, 0x1.111d262b1f677p-2,
+    0x1.f0a7efb9230d7p-1, 0x1.f19f97b215f1bp-3,
+    0x1.f38f3ac64e589p-1, 0x1.c0b826a7e4f63p-3,
     0x1.f6297cff75cb0p-1, 0x1.8f8b83c69a60bp-3,
+    0x1.f8764fa714ba9p-1, 0x1.5e214448b3fc6p-3,
+    0x1.fa7557f08a517p-1, 0x1.2c8106e8e613ap-3,
+    0x1.fc26470e19fd3p-1, 0x1.f564e56a9730ep-4,
     0x1.fd88da3d12526p-1, 0x1.917a6bc29b42cp-4,
+    0x1.fe9cdad01883ap-1, 0x1.2d52092ce19f6p-4,
+    0x1.ff621e3796d7ep-1, 0x1.91f65f10dd814p-5,
+    0x1.ffd886084cd0dp-1, 0x1.92155f7a3667ep-6,
 };
 
 // (c_j, ln(1/c_j)), j = -37..53 at index j + 37, interleaved
@@ -280,23 +521,23 @@ GM_CONST double LOG_T[182] = {
 enum CoefSlot : int {
     kExpC = 0,        // EXP_C[0..12]
     kLog1pC = 13,     // LOG1P_C[0..5]
-    kSinT = 19,       // SINT_C[0..2]
-    kCosT = 22,       // COST_C[0..3]
-    kPio32 = 26,      // PIO32_HI, PIO32_LO
-    kInvPio32 = 28,
-    kPio32Scaled64 = 29,
-    kLn2 = 30,        // LN2_0, LN2_1, LN2_2
-    kLn2Fma = 33,     // LN2_HI, LN2_LO
-    kInvLn2 = 35,
-    kRoundMagic = 36,
-    kCoefCount = 37
+    kSinT = 19,       // SINT_C[0..1]
+    kCosT = 21,       // COST_C[0..2]
+    kPio128 = 24,     // PIO128_HI, PIO128_LO
+    kInvPio128 = 26,
+    kPio128Scaled64 = 27,
+    kLn2 = 28,        // LN2_0, LN2_1, LN2_2
+    kLn2Fma = 31,     // LN2_HI, LN2_LO
+    kInvLn2 = 33,
+    kRoundMagic = 34,
+    kCoefCount = 35
 };
 #define GM_COEF_INIT                                                              \
     {EXP_C[0], EXP_C[1], EXP_C[2], EXP_C[3], EXP_C[4], EXP_C[5], EXP_C[6],        \
      EXP_C[7], EXP_C[8], EXP_C[9], EXP_C[10], EXP_C[11], EXP_C[12],               \
      LOG1P_C[0], LOG1P_C[1], LOG1P_C[2], LOG1P_C[3], LOG1P_C[4], LOG1P_C[5],      \
-     SINT_C[0], SINT_C[1], SINT_C[2], COST_C[0], COST_C[1], COST_C[2], COST_C[3],  \
-     PIO32_HI, PIO32_LO, INV_PIO32, PIO32_SCALED64,                               \
+     SINT_C[0], SINT_C[1], COST_C[0], COST_C[1], COST_C[2],                        \
+     PIO128_HI, PIO128_LO, INV_PIO128, PIO128_SCALED64,                           \
      LN2_0, LN2_1, LN2_2, LN2_HI, LN2_LO, INV_LN2, ROUND_MAGIC}
 
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -362,7 +603,6 @@ GM_HD double u2d(uint64_t u) {
 // ---- exp -------------------------------------------------------------------
 // Arguments whose exp is fixed without arithmetic: NaN, and those beyond the
 // clamp below (+inf above 89, +0 below -104).
-GM_HD bool expf_special(float x) { return !(x >= -104.0f && x <= 89.0f); }
 GM_HD float expf_special_value(float x) {
     return (x != x) ? x : (x > 0.0f ? u2f(0x7f800000u) : 0.0f);
 }
@@ -442,7 +682,7 @@ GM_HD float plogf_r(float b, CoefPtr C) {
 }
 
 // ---- sin / cos ---------------------------------------------------------------
-// x = k pi/32 + t, |t| <= pi/64 (plus rounding), j = k mod 64.
+// x = k pi/128 + t, |t| <= pi/256 (plus rounding), j = k mod 256.
 
 // Finite arguments with |x| >= 2^25 take the Payne-Hanek reduction: with the
 // sign shifted out, the bits of |x| lie in [2^25, inf) as [0x98 << 24, 0xff << 24).
@@ -453,8 +693,8 @@ GM_HD bool sincos_needs_ph(float x) {
 // Payne-Hanek for finite |x| >= 2^25: |x| = M 2^E with E = ex - 150 >= 2.
 // Only bits 2^-i of 2/pi with i >= E-1 matter modulo 4 quarter turns; take
 // 128 of them starting there (W): then |x| * 2/pi = M * W * 2^-126 (mod 4).
-// In P = M * W, bits 127..126 are the quarter turn, so bits 127..122 are k mod
-// 64 for |x| * 32/pi = 16 |x| * 2/pi, and bits 121..0 its fraction.
+// In P = M * W, bits 127..126 are the quarter turn, so bits 127..120 are k mod
+// 256 for |x| * 128/pi = 64 |x| * 2/pi, and bits 119..0 its fraction.
 GM_HD double reduce_ph(float x, int& j, CoefPtr C) {
     const uint32_t u = f2u(x);
     const uint32_t ex = (u >> 23) & 0xffu;
@@ -475,33 +715,33 @@ GM_HD double reduce_ph(float x, int& j, CoefPtr C) {
     const uint64_t lo = a3 + (a2 << 32);
     const uint64_t c0 = lo < a3 ? 1u : 0u;
     const uint64_t mid = (a2 >> 32) + a1 + (a0 << 32) + c0;
-    int k = (int)(mid >> 58);
-    const uint64_t fh = (mid << 6) | (lo >> 58);  // fraction bits 2^-1 .. 2^-64
-    const uint64_t fl = lo << 6;                   // fraction bits 2^-65 .. 2^-122
+    int k = (int)(mid >> 56);
+    const uint64_t fh = (mid << 8) | (lo >> 56);  // fraction bits 2^-1 .. 2^-64
+    const uint64_t fl = lo << 8;                   // fraction bits 2^-65 .. 2^-120
     // A fraction >= 1/2 rounds k up; fh:fl read as a signed 128-bit number is
     // then the fraction minus one.
     if (fh >> 63) k += 1;
     const double f = (double)(int64_t)fh + (double)fl * 0x1p-64;  // fraction * 2^64
-    double t = f * C[kPio32Scaled64];
+    double t = f * C[kPio128Scaled64];
     if (x < 0.0f) {
         t = -t;
         k = -k;
     }
-    j = k & 63;
+    j = k & 255;
     return t;
 }
 
-// sin(j pi/32 + t) = S_j cos t + C_j sin t from the table row (s, c) =
+// sin(j pi/128 + t) = S_j cos t + C_j sin t from the table row (s, c) =
 // (S_j, C_j).  With sin t = t + t^3 ps and cos t - 1 = t^2 pc this is
 // (s + c t) + t^2 (s pc + c t ps): five operations after the two series.
+// For |t| <= pi/256 the series stop at t^5 and t^6: the next terms are below
+// 2^-50 |t| and 2^-70.
 GM_HD double sin_from_row(double t, double s, double c, CoefPtr C) {
     const double t2 = t * t;
-    // sin t = t + t^3 (s3 + t^2 (s5 + t^2 s7))
-    double ps = fma_d(t2, vreg(C[kSinT + 2]), C[kSinT + 1]);
-    ps = fma_d(ps, t2, C[kSinT]);
-    // cos t - 1 = t^2 (c2 + t^2 (c4 + t^2 (c6 + t^2 c8)))
-    double pc = fma_d(t2, vreg(C[kCosT + 3]), C[kCosT + 2]);
-    pc = fma_d(pc, t2, C[kCosT + 1]);
+    // sin t = t + t^3 (s3 + t^2 s5)
+    const double ps = fma_d(t2, vreg(C[kSinT + 1]), C[kSinT]);
+    // cos t - 1 = t^2 (c2 + t^2 (c4 + t^2 c6))
+    double pc = fma_d(t2, vreg(C[kCosT + 2]), C[kCosT + 1]);
     pc = fma_d(pc, t2, C[kCosT]);
     const double ct = c * t;
     return fma_d(t2, fma_d(ct, ps, s * pc), fma_d(c, t, s));
@@ -510,18 +750,18 @@ GM_HD double sin_from_reduced(double t, int j, CoefPtr C) {
     return sin_from_row(t, SINCOS_T[2 * j], SINCOS_T[2 * j + 1], C);
 }
 
-// Cody-Waite for an x that does not need Payne-Hanek: x = k pi/32 + t, and
-// j = k mod 64; the table row is j for sin and j + 16 for cos (cos x =
-// sin(x + pi/2); the table repeats rows 0..15 as 64..79).  k = rint(x 32/pi)
-// is rounded by addition (|k| < 2^28.4), so j is the low word of that sum: no
-// conversion, and for an infinite or NaN x, whose t is NaN, j is still in
-// 0..63.
+// Cody-Waite for an x that does not need Payne-Hanek: x = k pi/128 + t, and
+// j = k mod 256; the table row is j for sin and j + 64 for cos (cos x =
+// sin(x + pi/2); the table repeats rows 0..63 as 256..319).  k =
+// rint(x 128/pi) is rounded by addition (|k| < 2^30.4), so j is the low word
+// of that sum: no conversion, and for an infinite or NaN x, whose t is NaN, j
+// is still in 0..255.
 GM_HD double sincos_reduce(float x, uint32_t& j, CoefPtr C) {
     const double xd = (double)x;
-    const double kb = xd * C[kInvPio32] + C[kRoundMagic];
+    const double kb = xd * C[kInvPio128] + C[kRoundMagic];
     const double k = kb - C[kRoundMagic];
-    j = (uint32_t)d2u(kb) & 63u;
-    return fma_d(-k, C[kPio32 + 1], fma_d(-k, C[kPio32], xd));
+    j = (uint32_t)d2u(kb) & 255u;
+    return fma_d(-k, C[kPio128 + 1], fma_d(-k, C[kPio128], xd));
 }
 
 // The rounded result from the reduced argument and its row: infinities and
@@ -538,7 +778,7 @@ GM_HD float sincos_finish(float x, int want_cos, double t, double s, double c, C
 GM_HD float sincos_cw(float x, int want_cos, CoefPtr C) {
     uint32_t j;
     const double t = sincos_reduce(x, j, C);
-    const uint32_t row = j + 16u * (uint32_t)want_cos;
+    const uint32_t row = j + 64u * (uint32_t)want_cos;
     return sincos_finish(x, want_cos, t, SINCOS_T[2 * row], SINCOS_T[2 * row + 1], C);
 }
 
@@ -556,7 +796,7 @@ GM_HD float sincos_special_value(float x, int want_cos) {
 GM_HD float sincos_ph(float x, int want_cos, CoefPtr C) {
     int j;
     const double t = reduce_ph(x, j, C);
-    return (float)sin_from_reduced(t, (j + 16 * want_cos) & 63, C);
+    return (float)sin_from_reduced(t, j + 64 * want_cos, C);
 }
 
 GM_HD float sinf_r(float x, CoefPtr C) {

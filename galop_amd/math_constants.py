@@ -103,13 +103,19 @@ def taylor(kind, n_terms):
     raise ValueError(kind)
 
 
+# sin/cos reduce by pi/SINCOS_DIV: x = k pi/128 + t, |t| <= pi/256
+SINCOS_DIV = 128
+SINCOS_ROWS = 2 * SINCOS_DIV       # one turn
+SINCOS_QUARTER = SINCOS_DIV // 2   # rows per quarter turn
+
+
 def sin_cos_fixed(r, prec=PREC):
-    """(sin, cos) of r*pi/32 for 0 <= r <= 16, as integers scaled by 2^prec,
+    """(sin, cos) of r*pi/128 for 0 <= r <= 64, as integers scaled by 2^prec,
     by their Taylor series in fixed point (a few units in the last place)."""
     guard = 64
     p = prec + guard
     one = 1 << p
-    theta = r * pi_fixed(p) // 32
+    theta = r * pi_fixed(p) // SINCOS_DIV
     th2 = theta * theta >> p
     out = []
     for start, first in ((theta, 1), (one, 0)):
@@ -128,22 +134,22 @@ def sin_cos_fixed(r, prec=PREC):
 
 
 def sincos_table():
-    """80 pairs (sin(j pi/32), cos(j pi/32)), j = 0..79, rounded to double;
-    the multiples of pi/2 hold exact 0 and +-1.  Rows 64..79 repeat rows
-    0..15, so cos x = sin(x + pi/2), row j + 16, needs no wrap-around."""
+    """320 pairs (sin(j pi/128), cos(j pi/128)), j = 0..319, rounded to
+    double; the multiples of pi/2 hold exact 0 and +-1.  Rows 256..319 repeat
+    rows 0..63, so cos x = sin(x + pi/2), row j + 64, needs no wrap-around."""
     first = []
-    for r in range(17):
+    for r in range(SINCOS_QUARTER + 1):
         sf, cf = sin_cos_fixed(r)
         first.append((float(Fraction(sf, 1 << PREC)), float(Fraction(cf, 1 << PREC))))
     first[0] = (0.0, 1.0)
-    first[16] = (1.0, 0.0)
+    first[SINCOS_QUARTER] = (1.0, 0.0)
     out = []
-    for j in range(64):
-        q, r = divmod(j, 16)
+    for j in range(SINCOS_ROWS):
+        q, r = divmod(j, SINCOS_QUARTER)
         s, c = first[r]
         s, c = [(s, c), (c, -s), (-s, -c), (-c, s)][q]
         out.extend([s + 0.0, c + 0.0])  # no negative zeros
-    return out + out[:32]
+    return out + out[:2 * SINCOS_QUARTER]
 
 
 def round_bits(v, bits):
@@ -198,10 +204,9 @@ def constants():
     pi = pi_fixed()
     ln2 = ln2_fixed()
     c = {
-        "PIO32": split(pi, PREC + 5, 24),
-        "PIO32_FMA": split_fma(pi, PREC + 5),
-        "INV_PIO32": float(Fraction(32 << PREC, pi)),
-        "PIO32_SCALED64": float(Fraction(pi, 1 << (PREC + 5)) / (1 << 64)),
+        "PIO128_FMA": split_fma(pi, PREC + 7),
+        "INV_PIO128": float(Fraction(128 << PREC, pi)),
+        "PIO128_SCALED64": float(Fraction(pi, 1 << (PREC + 7)) / (1 << 64)),
         "LN2": split(ln2, PREC, 32),
         "LN2_FMA": split_fma(ln2, PREC),
         "INV_LN2": float(Fraction(1 << PREC, ln2)),
@@ -209,8 +214,8 @@ def constants():
         "ROUND_MAGIC": float(3 << 51),
         "TWO_OVER_PI_WORDS": two_over_pi_words(12),
         "EXP_POLY": taylor("exp", 13),
-        "SINT_POLY": taylor("sin", 4)[1:],   # -1/3!, 1/5!, -1/7!
-        "COST_POLY": taylor("cos", 5)[1:],   # -1/2!, 1/4!, -1/6!, 1/8!
+        "SINT_POLY": taylor("sin", 3)[1:],   # -1/3!, 1/5!
+        "COST_POLY": taylor("cos", 4)[1:],   # -1/2!, 1/4!, -1/6!
         "SINCOS_T": sincos_table(),
         # log1p(r) = r + r^2 (-1/2 + r/3 - r^2/4 + ... + r^5/7)
         "LOG1P_POLY": [float(Fraction((-1) ** (n + 1), n)) for n in range(2, 8)],

@@ -35,7 +35,7 @@ def _header_values():
         vals[name] = nums if len(nums) > 1 else nums[0]
     words = re.search(r"TWO_OVER_PI_W\[8\] = \{([^}]*)\}", text).group(1)
     vals["TWO_OVER_PI_W"] = [int(w.strip().rstrip("u"), 16) for w in words.split(",")]
-    for name, n in (("SINCOS_T", 160), ("LOG_T", 182)):
+    for name, n in (("SINCOS_T", 640), ("LOG_T", 182)):
         table = re.search(name + r"\[%d\] = \{([^}]*)\}" % n, text).group(1)
         vals[name] = [float.fromhex(t.strip()) for t in table.split(",") if t.strip()]
     return vals
@@ -44,11 +44,11 @@ def _header_values():
 def test_constants_match_derivation():
     c = MC.constants()
     h = _header_values()
-    assert [h["PIO32_HI"], h["PIO32_LO"]] == c["PIO32_FMA"]
+    assert [h["PIO128_HI"], h["PIO128_LO"]] == c["PIO128_FMA"]
     assert [h["LN2_HI"], h["LN2_LO"]] == c["LN2_FMA"]
     assert [h["LN2_0"], h["LN2_1"], h["LN2_2"]] == c["LN2"]
-    assert h["INV_PIO32"] == c["INV_PIO32"]
-    assert h["PIO32_SCALED64"] == c["PIO32_SCALED64"]
+    assert h["INV_PIO128"] == c["INV_PIO128"]
+    assert h["PIO128_SCALED64"] == c["PIO128_SCALED64"]
     assert h["INV_LN2"] == c["INV_LN2"]
     assert h["ROUND_MAGIC"] == c["ROUND_MAGIC"] == 1.5 * 2.0 ** 52
     assert h["EXP_C"] == c["EXP_POLY"]
@@ -70,24 +70,24 @@ def test_constants_match_derivation():
     assert sum(c["LN2"]) == math.log(2)
     # two-part splits: the leading part is the double nearest the value, the
     # second the rest to within half its ulp
-    pi32 = Fraction(MC.pi_fixed(), 1 << (MC.PREC + 5))
+    pi128 = Fraction(MC.pi_fixed(), 1 << (MC.PREC + 7))
     ln2 = Fraction(MC.ln2_fixed(), 1 << MC.PREC)
-    for (hi, lo), v in ((c["PIO32_FMA"], pi32), (c["LN2_FMA"], ln2)):
+    for (hi, lo), v in ((c["PIO128_FMA"], pi128), (c["LN2_FMA"], ln2)):
         assert hi == float(v) and lo == float(v - Fraction(hi))
         assert abs(Fraction(hi) + Fraction(lo) - v) < v * Fraction(1, 2 ** 105)
-    assert c["PIO32_FMA"][0] == math.pi / 32 and c["LN2_FMA"][0] == math.log(2)
+    assert c["PIO128_FMA"][0] == math.pi / 128 and c["LN2_FMA"][0] == math.log(2)
 
 
 def test_fma_reductions_are_exact():
-    """fma(-k, PIO32_HI, x) for the arguments that take Cody-Waite (|x| < 2^25,
-    k = rint(x 32/pi)) and fma(-k, LN2_HI, x) for exp's clamped arguments
+    """fma(-k, PIO128_HI, x) for the arguments that take Cody-Waite (|x| < 2^25,
+    k = rint(x 128/pi)) and fma(-k, LN2_HI, x) for exp's clamped arguments
     (|x| <= 104.5, k = rint(x/ln 2)) are exact: the exact difference is a
     double.  Checked in exact rationals on random arguments of every binade
-    and on the floats nearest multiples of pi/32 and ln 2."""
+    and on the floats nearest multiples of pi/128 and ln 2."""
     from fractions import Fraction
     c = MC.constants()
     rng = np.random.default_rng(9)
-    for (hi, _), inv, lim in ((c["PIO32_FMA"], c["INV_PIO32"], 2.0 ** 25),
+    for (hi, _), inv, lim in ((c["PIO128_FMA"], c["INV_PIO128"], 2.0 ** 25),
                               (c["LN2_FMA"], c["INV_LN2"], 104.5)):
         xs = np.concatenate([
             rng.uniform(-lim, lim, 20000),
@@ -107,25 +107,28 @@ def test_sincos_table_identities():
     """The table is pinned independently of the series that made it:
     exact values at multiples of pi/4 and pi/2 (sqrt(2)/2 from an integer
     square root), sin^2 + cos^2 = 1, the double-angle and addition formulas
-    (each within a few units of 2^-53), and numpy's sin/cos for small j."""
+    (each within a few units of 2^-53), numpy's sin/cos for small j, and the
+    repeated rows 256..319 equal to rows 0..63."""
     from fractions import Fraction
     from math import isqrt
+    n, q4 = MC.SINCOS_ROWS, MC.SINCOS_QUARTER
     t = MC.constants()["SINCOS_T"]
-    S = [Fraction(t[2 * j]) for j in range(64)]
-    C = [Fraction(t[2 * j + 1]) for j in range(64)]
+    assert len(t) == 2 * (n + q4) and t[2 * n:] == t[:2 * q4]
+    S = [Fraction(t[2 * j]) for j in range(n)]
+    C = [Fraction(t[2 * j + 1]) for j in range(n)]
     half_sqrt2 = float(Fraction(isqrt(2 << 240), 1 << 121))  # sqrt(2)/2, correctly rounded
-    assert t[2 * 8] == half_sqrt2 and t[2 * 8 + 1] == half_sqrt2
+    assert t[2 * (q4 // 2)] == half_sqrt2 and t[2 * (q4 // 2) + 1] == half_sqrt2
     for q in range(4):
-        assert (t[2 * 16 * q], t[2 * 16 * q + 1]) == [(0.0, 1.0), (1.0, 0.0), (0.0, -1.0), (-1.0, 0.0)][q]
+        assert (t[2 * q4 * q], t[2 * q4 * q + 1]) == [(0.0, 1.0), (1.0, 0.0), (0.0, -1.0), (-1.0, 0.0)][q]
     eps = Fraction(1, 1 << 51)
-    for j in range(64):
+    for j in range(n):
         assert abs(S[j] ** 2 + C[j] ** 2 - 1) < eps
-        assert abs(S[(2 * j) % 64] - 2 * S[j] * C[j]) < 2 * eps
-        assert abs(S[(j + 1) % 64] - (S[j] * C[1] + C[j] * S[1])) < 2 * eps
-        assert abs(C[(j + 1) % 64] - (C[j] * C[1] - S[j] * S[1])) < 2 * eps
+        assert abs(S[(2 * j) % n] - 2 * S[j] * C[j]) < 2 * eps
+        assert abs(S[(j + 1) % n] - (S[j] * C[1] + C[j] * S[1])) < 2 * eps
+        assert abs(C[(j + 1) % n] - (C[j] * C[1] - S[j] * S[1])) < 2 * eps
     for j in range(9):
-        assert abs(t[2 * j] - np.sin(j * np.pi / 32)) <= 2.3e-16
-        assert abs(t[2 * j + 1] - np.cos(j * np.pi / 32)) <= 2.3e-16
+        assert abs(t[2 * j] - np.sin(j * np.pi / MC.SINCOS_DIV)) <= 2.3e-16
+        assert abs(t[2 * j + 1] - np.cos(j * np.pi / MC.SINCOS_DIV)) <= 2.3e-16
 
 
 def test_log_table_identities():
