@@ -310,6 +310,62 @@ def bench_engine(ctx, population, cases, generations, seeds):
     }
 
 
+def c1_dataset(ctx, seed=0):
+    """The engine configuration's dataset: 1 input x ~ U[-1, 1], 20 cases; y
+    the outputs of a seeded random 20-instruction program over {+, -, *,
+    protected /} on 8 registers (redrawn until finite and not constant)."""
+    from galop_amd import Dataset, Isa, workloads
+    from galop_amd._lib import OPS_REF8
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (20, 1)).astype(np.float32)
+    y, _ = workloads.target_from_program(ctx, Isa(8, OPS_REF8), x, rng, 20)
+    return Dataset(x, y)
+
+
+def bench_c1(ctx, seeds, generations=50):
+    """Whole evolutionary runs on the engine configuration: population 1,000,
+    8 registers, {+, -, *, protected /}, max length 64, tournament 4, the
+    default mutation and crossover, 50 generations, 20 cases; initial lengths
+    U[8, 32] (starting_length_max; the reference starts every program at 8,
+    which is timed beside it).  One warm-up run, then one run per seed; wall
+    time per run, initialisation included."""
+    import torch
+    from galop_amd.engine import VaryCfg
+    from galop_amd.evolve import DeviceEngine
+    ds = c1_dataset(ctx)
+    cur = torch.cuda.current_stream()
+    out = {"workload": "engine configuration: P=1000, 8 registers, REF8, max length 64, "
+                       f"tournament 4, {generations} generations, 20 cases, seeds "
+                       f"{seeds[0]}..{seeds[-1]}"}
+    for name, hi in (("initial_U8_32", 32), ("initial_fixed_8", 0)):
+        cfg = VaryCfg.default(1000, stride=64, tournament_size=4, starting_length=8,
+                              starting_length_max=hi)
+        eng = DeviceEngine(ctx, ds, cfg, seeds[0], generations, stream=cur)
+        eng.evolve_sr()  # warm-up
+        walls, best, ops = [], [], []
+        for seed in seeds:
+            eng.seed = seed
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            hist = eng.evolve_sr()
+            torch.cuda.synchronize()
+            walls.append(time.perf_counter() - t0)
+            best.append(float(hist[-1]["best_fitness"]))
+            ops.append(sum(float(h["mean_length"]) * cfg.population * 20 for h in hist))
+            if seed == seeds[0]:
+                first_len = float(hist[0]["mean_length"])
+        eng.close()
+        w = sorted(walls)
+        med = w[len(w) // 2]
+        out[name] = {"run_s_median": round(med, 5), "run_s": [round(x, 5) for x in walls],
+                     "spread": round((w[-1] - w[0]) / med, 3),
+                     "ms_per_generation": round(med * 1e3 / generations, 4),
+                     "generation0_mean_length": round(first_len, 3),
+                     "raw_gpops": round(float(np.median(ops)) / med / 1e9, 4),
+                     "final_best_fitness": best}
+    return out
+
+
 def bench_casebound(ctx, programs, steps, warmup, threads):
     """The case-bound population at full shape (1,048,576 cases of 5 inputs,
     16 registers, 128-instruction programs), timed on its first `programs`
@@ -390,6 +446,9 @@ def main():
                     help="generations per run under Poisson arrivals, after the all-at-once pass")
     ap.add_argument("--serving-pop-scale", type=float, default=1.0 / 8)
     ap.add_argument("--serving-streams", type=int, default=4)
+    ap.add_argument("--serving-window", type=float, default=None,
+                    help="timed phase: Poisson arrivals over this many seconds "
+                         "instead of a fixed number of generations per run")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="run one device copy of this many bytes first (HBM counter calibration)")
     ap.add_argument("--no-uneven", action="store_true",
@@ -406,6 +465,10 @@ def main():
     ap.add_argument("--engine-generations", type=int, default=100)
     ap.add_argument("--engine-only", action="store_true",
                     help="only the device-resident evolutionary runs")
+    ap.add_argument("--no-c1", action="store_true",
+                    help="skip the engine configuration's whole runs (P=1000, 50 generations)")
+    ap.add_argument("--c1-only", action="store_true",
+                    help="only the engine configuration's whole runs")
     ap.add_argument("--no-host-entry", action="store_true",
                     help="skip timing the host-pointer entry point (pinned staging + H2D + D2H)")
     args = ap.parse_args()
@@ -442,10 +505,18 @@ def main():
                           "data": "the reference's quartic dataset (regenerated)",
                           "engine": res}), flush=True)
         return 0
+    if args.c1_only:
+        res = bench_c1(ctx, [0, 1, 2, 3, 4])
+        print(json.dumps({"metric": "engine configuration: seconds per 50-generation run",
+                          "value": res["initial_U8_32"]["run_s_median"], "unit": "s",
+                          "n_gpus": 1, "higher_is_better": False, "dtype": "f32",
+                          "data": "synthetic (seeded)", "c1": res}), flush=True)
+        return 0
     if args.serving_only:
         from galop_amd.serving_mix import run_mix
         res = run_mix(ctx, n_runs=args.serving_runs, generations=args.serving_generations,
-                      pop_scale=args.serving_pop_scale, n_streams=args.serving_streams, log=log)
+                      pop_scale=args.serving_pop_scale, n_streams=args.serving_streams, log=log,
+                      window_s=args.serving_window)
         print(json.dumps({"metric": "serving mix: aggregate GPop/s, p50/p99 eval latency",
                           "value": round(res["gpops"], 3), "unit": "GPop/s", "n_gpus": 1,
                           "higher_is_better": True, "dtype": "f32",
@@ -555,6 +626,10 @@ def main():
         issue_peak = n_cu * SIMDS_PER_CU * CLOCK_HZ / 4
         issue = c["valu_insts"] / (ms / 1e3)
         line["roofline"]["traffic"] = round(traffic)
+        line["roofline"]["traffic_source"] = (
+            "profile-derived, not measured in this run: rocprofv3 --pmc FETCH_SIZE and "
+            "WRITE_SIZE of this build on this seed-0 population, scaled by a 1 GiB copy "
+            "(profiles/plain_kernel_plain_workload_counters.md)")
         line["roofline"]["traffic_over_algorithmic"] = round(traffic / alg_bytes, 3)
         line["roofline"]["hbm_GBps"] = round(traffic / (ms / 1e3) / 1e9, 1)
         line["roofline"]["valu_issue"] = {
@@ -667,11 +742,17 @@ def main():
                            [70, 71, 72, 73, 74])
         log(f"[bench] engine: {res}")
         line["engine"] = res
+    if not args.no_c1 and args.workload == "plain":
+        log("[bench] engine configuration, whole runs")
+        res = bench_c1(ctx, [0, 1, 2, 3, 4])
+        log(f"[bench] c1: {res}")
+        line["c1"] = res
     if not args.no_serving and args.workload == "plain":
         from galop_amd.serving_mix import run_mix
         log("[bench] serving mix")
         res = run_mix(ctx, n_runs=args.serving_runs, generations=args.serving_generations,
-                      pop_scale=args.serving_pop_scale, n_streams=args.serving_streams, log=log)
+                      pop_scale=args.serving_pop_scale, n_streams=args.serving_streams, log=log,
+                      window_s=args.serving_window)
         log(f"[bench] serving: {res}")
         line["serving"] = res
     print(json.dumps(line), flush=True)

@@ -52,7 +52,7 @@ class galop_vary_cfg(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "population", "stride", "elite_count", "tournament_size", "num_registers",
         "input_protect", "op_set", "effmut", "macro_inserts", "macro_random_pos",
-        "starting_length", "reserved")] + [(n, ctypes.c_float) for n in (
+        "starting_length", "starting_length_max")] + [(n, ctypes.c_float) for n in (
         "crossover_rate", "micro_rate", "micro_per_instr", "effmut_rate", "insert_rate",
         "delete_rate")]
 

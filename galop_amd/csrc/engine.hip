@@ -29,7 +29,7 @@ ge::VaryCfg to_cfg(const galop_vary_cfg& c) {
     v.macro_inserts = c.macro_inserts;
     v.macro_random_pos = c.macro_random_pos;
     v.starting_length = c.starting_length;
-    v.reserved = 0;
+    v.starting_length_max = c.starting_length_max;
     v.crossover_rate = c.crossover_rate;
     v.micro_rate = c.micro_rate;
     v.micro_per_instr = c.micro_per_instr;
@@ -53,7 +53,9 @@ const char* check_cfg(const galop_vary_cfg* c) {
     if (c->macro_inserts < 0) return "macro_inserts < 0";
     if (c->macro_random_pos != 0 && c->macro_random_pos != 1) return "macro_random_pos must be 0/1";
     if (c->starting_length < 0 || c->starting_length > c->stride) return "starting_length outside [0, stride]";
-    if (c->reserved != 0) return "reserved must be 0";
+    if (c->starting_length_max != 0 &&
+        (c->starting_length_max < c->starting_length || c->starting_length_max > c->stride))
+        return "starting_length_max neither 0 nor in [starting_length, stride]";
     return nullptr;
 }
 
@@ -412,8 +414,8 @@ __global__ void init_kernel(ge::VaryCfg c, uint64_t base, uint32_t* __restrict__
                             uint8_t* __restrict__ len, float* __restrict__ fit) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= c.population) return;
-    ge::init_program(i, c, base, instr);
-    len[i] = (uint8_t)c.starting_length;
+    ge::Pcg32 g = ge::init_program(i, c, base, instr);
+    len[i] = ge::initial_length(c, c.starting_length_max > 0 ? g.next() : 0u);
     fit[i] = ge::bits_to_float(0x7fc00000u);
 }
 

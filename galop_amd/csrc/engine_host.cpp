@@ -81,8 +81,8 @@ int32_t gv_vary_planned(const ge::VaryCfg* c, const uint32_t* cur_instr, const u
 
 void gv_init(const ge::VaryCfg* c, uint64_t base, uint32_t* instr, uint8_t* len, float* fit) {
     for (int64_t i = 0; i < c->population; ++i) {
-        ge::init_program(i, *c, base, instr);
-        len[i] = (uint8_t)c->starting_length;
+        ge::Pcg32 g = ge::init_program(i, *c, base, instr);
+        len[i] = ge::initial_length(*c, c->starting_length_max > 0 ? g.next() : 0u);
         fit[i] = ge::bits_to_float(0x7fc00000u);
     }
 }

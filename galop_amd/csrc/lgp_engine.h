@@ -39,7 +39,7 @@ struct VaryCfg {
     int32_t macro_inserts;    // MACRO_INSERTS
     int32_t macro_random_pos; // MACRO_RANDOM_POS
     int32_t starting_length;  // STARTING_PROGRAM_SIZE (initializer)
-    int32_t reserved;
+    int32_t starting_length_max;  // 0, or initial lengths U[starting_length, this]
     float crossover_rate;     // CROSSOVER_RATE
     float micro_rate;         // MICRO_RATE
     float micro_per_instr;    // MICRO_PER_INSTR
@@ -372,12 +372,25 @@ EN_HD void plan_pair(int64_t i, const VaryCfg& c, const uint8_t* cur_len, const 
 }
 
 // Program i of the per-program PCG32 initializer (src/LGPEngine.cpp:52-67):
-// every word of the slot from stream (base, i).
-EN_HD void init_program(int64_t i, const VaryCfg& c, uint64_t base, uint32_t* instr) {
+// every word of the slot from stream (base, i).  Returns the stream, which
+// continues with the program's initial length.
+EN_HD Pcg32 init_program(int64_t i, const VaryCfg& c, uint64_t base, uint32_t* instr) {
     Pcg32 g = pcg32_make(base, (uint64_t)i);
     const uint32_t rmask = (uint32_t)c.num_registers - 1u;
     uint32_t* p = instr + i * c.stride;
     for (int k = 0; k < c.stride; ++k) p[k] = encode_from_random(g.next(), rmask);
+    return g;
+}
+
+// A program's initial length.  The reference starts every program at
+// STARTING_PROGRAM_SIZE (src/LGPEngine.cpp:67,76-79), and so does
+// starting_length_max = 0.  Otherwise the length is uniform on
+// [starting_length, starting_length_max] (an option of this engine, off by
+// default): lo + floor(u * span / 2^32) for the 32-bit draw u.
+EN_HD uint8_t initial_length(const VaryCfg& c, uint32_t u) {
+    if (c.starting_length_max <= 0) return (uint8_t)c.starting_length;
+    const uint64_t span = (uint64_t)(c.starting_length_max - c.starting_length + 1);
+    return (uint8_t)(c.starting_length + (int32_t)(((uint64_t)u * span) >> 32));
 }
 
 // Sort key of the elite selection (src/LGPEngine.cpp:375-408): fitness bits

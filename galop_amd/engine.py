@@ -17,7 +17,7 @@ from ._lib import check
 class VaryCfg:
     FIELDS_I = ("population", "stride", "elite_count", "tournament_size", "num_registers",
                 "input_protect", "op_set", "effmut", "macro_inserts", "macro_random_pos",
-                "starting_length")
+                "starting_length", "starting_length_max")
     FIELDS_F = ("crossover_rate", "micro_rate", "micro_per_instr", "effmut_rate",
                 "insert_rate", "delete_rate")
 
@@ -38,6 +38,7 @@ class VaryCfg:
         kw = dict(population=population, stride=stride, elite_count=elites,
                   tournament_size=3, num_registers=8, input_protect=0, op_set=_lib.OPS_REF8,
                   effmut=0, macro_inserts=1, macro_random_pos=1, starting_length=8,
+                  starting_length_max=0,
                   crossover_rate=1.0, micro_rate=0.8, micro_per_instr=0.0, effmut_rate=1.0,
                   insert_rate=0.8, delete_rate=0.75)
         kw.update(overrides)
@@ -47,8 +48,18 @@ class VaryCfg:
         s = _lib.galop_vary_cfg()
         for k in self.FIELDS_I + self.FIELDS_F:
             setattr(s, k, getattr(self, k))
-        s.reserved = 0
         return s
+
+    def initial_lengths(self, draws):
+        """Initial lengths from 32-bit draws, as the device initializer makes
+        them (include/galop_engine.h): starting_length, or with
+        starting_length_max > 0 uniform on [starting_length,
+        starting_length_max] by lo + floor(u * span / 2^32)."""
+        if self.starting_length_max <= 0:
+            return np.full(len(draws), self.starting_length, np.uint8)
+        span = np.uint64(self.starting_length_max - self.starting_length + 1)
+        u = np.asarray(draws, np.uint64)
+        return (self.starting_length + ((u * span) >> np.uint64(32))).astype(np.uint8)
 
 
 def _p(t):

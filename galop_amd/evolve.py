@@ -146,7 +146,13 @@ class DeviceEngine:
                                           stream=self.stream)
                 self._rng.state = {"bit_generator": "MT19937",
                                    "state": {"key": key, "pos": 624}}
-                self.lengths[b].fill_(cfg.starting_length)
+                if cfg.starting_length_max > 0:
+                    # lengths U[starting_length, starting_length_max] from the
+                    # host generator's next P draws (an option of this engine)
+                    lens = cfg.initial_lengths(self._draw(P))
+                    self.lengths[b].copy_(torch.from_numpy(lens))
+                else:
+                    self.lengths[b].fill_(cfg.starting_length)
 
     # ---- src/LGPEngine.cpp:132-145 ---------------------------------------------
     def evaluate_range(self, start, count):

@@ -11,8 +11,11 @@ Latency is submit -> fitness in the client's buffer.
 
 Protocol: a first pass submits every run's generation at once (this also
 measures the server's capacity, in GP-ops per second); then each run submits
-`generations` more, each after an exponential think time whose mean makes
-the offered load `load` times that capacity (Poisson arrivals).
+more generations, each after an exponential think time whose mean makes the
+offered load `load` times that capacity (Poisson arrivals): either
+`generations` each, or, with `window_s`, as many as arrive within a window of
+that many seconds (an open window: requests arriving in it are served to
+completion, none arrive after it).
 """
 import ctypes
 import threading
@@ -58,7 +61,7 @@ class Run:
 
 
 def run_mix(ctx, n_runs=64, generations=1, pop_scale=1.0, load=0.7, n_streams=4,
-            budget_bytes=200 << 30, log=print, check_alone=4, max_programs=0):
+            budget_bytes=200 << 30, log=print, check_alone=4, max_programs=0, window_s=None):
     srv = Server(ctx, budget_bytes, n_streams)
     t_setup = time.perf_counter()
     runs = [Run(ctx, srv, s, pop_scale, max_programs=max_programs) for s in range(n_runs)]
@@ -83,13 +86,15 @@ def run_mix(ctx, n_runs=64, generations=1, pop_scale=1.0, load=0.7, n_streams=4,
     t0 = time.perf_counter()
     first = [None] * n_runs
     done = [0]
+    lock = threading.Lock()
 
     def first_pass(i):
         first[i] = one(runs[i])
-        done[0] += 1
-        if done[0] % 8 == 0 or done[0] == n_runs:
-            log(f"[serving] first pass: {done[0]}/{n_runs} requests done "
-                f"({time.perf_counter() - t0:.1f}s)")
+        with lock:
+            done[0] += 1
+            if done[0] % 8 == 0 or done[0] == n_runs:
+                log(f"[serving] first pass: {done[0]}/{n_runs} requests done "
+                    f"({time.perf_counter() - t0:.1f}s)")
 
     th = [threading.Thread(target=first_pass, args=(i,)) for i in range(n_runs)]
     for t in th:
@@ -108,15 +113,21 @@ def run_mix(ctx, n_runs=64, generations=1, pop_scale=1.0, load=0.7, n_streams=4,
 
     def client(r):
         rng = np.random.default_rng(50000 + r.seed)
-        for _ in range(generations):
-            time.sleep(rng.exponential(think))
+        g = 0
+        while window_s is not None or g < generations:
+            wait = rng.exponential(think)
+            if window_s is not None and time.perf_counter() + wait > t0 + window_s:
+                return
+            time.sleep(wait)
             res = one(r)
             if res is None:
                 return
+            g += 1
             r.latencies.append(res[0])
             r.device_s.append(res[1])
-            log(f"[serving] run {r.seed}: {r.P} programs x {r.n_cases} cases, latency "
-                f"{res[0] * 1e3:.1f} ms")
+            with lock:
+                log(f"[serving] run {r.seed}: {r.P} programs x {r.n_cases} cases, latency "
+                    f"{res[0] * 1e3:.1f} ms")
 
     t0 = time.perf_counter()
     threads = [threading.Thread(target=client, args=(r,)) for r in runs]
@@ -152,7 +163,9 @@ def run_mix(ctx, n_runs=64, generations=1, pop_scale=1.0, load=0.7, n_streams=4,
     srv.close()
     n_total = [r.n_cases for r in runs]
     return {
-        "runs": n_runs, "generations_timed": generations, "requests": int(lats.size),
+        "runs": n_runs, "requests": int(lats.size),
+        "arrivals": (f"Poisson, open window of {window_s} s" if window_s is not None
+                     else f"{generations} generation(s) per run"),
         "pop_scale": pop_scale, "programs": int(sum(r.P for r in runs)),
         "cases_range": [int(min(n_total)), int(max(n_total))],
         "offered_load": load, "n_streams": n_streams, "budget_bytes": budget_bytes,

@@ -32,7 +32,11 @@ typedef struct galop_vary_cfg {
     int32_t macro_inserts;     /* MACRO_INSERTS >= 0                                */
     int32_t macro_random_pos;  /* MACRO_RANDOM_POS 0/1                              */
     int32_t starting_length;   /* STARTING_PROGRAM_SIZE (initializer), 0..stride    */
-    int32_t reserved;          /* must be 0                                         */
+    int32_t starting_length_max; /* 0: every initial length is starting_length (the *
+                                *  reference); else initial lengths uniform on      *
+                                *  [starting_length, starting_length_max] <= stride *
+                                *  (no counterpart in the reference, which fixes    *
+                                *  them, src/LGPEngine.cpp:67,76-79)                */
     float   crossover_rate;    /* CROSSOVER_RATE                                    */
     float   micro_rate;        /* MICRO_RATE                                        */
     float   micro_per_instr;   /* MICRO_PER_INSTR                                   */
@@ -57,7 +61,9 @@ int galop_vary_device(galop_ctx* ctx, const galop_vary_cfg* cfg,
 
 /* The per-program PCG32 initializer (src/LGPEngine.cpp:52-67, the
  * GALOP_PARALLEL_INIT build): every word of slot i from stream (base, i),
- * length starting_length, fitness NaN. */
+ * length starting_length, fitness NaN.  With starting_length_max > 0 the
+ * length is starting_length + floor(u * (starting_length_max -
+ * starting_length + 1) / 2^32), u the stream's next draw after the words. */
 int galop_init_population_device(galop_ctx* ctx, const galop_vary_cfg* cfg,
                                  uint64_t base, uint32_t* d_instr, uint8_t* d_len,
                                  float* d_fit, void* stream);

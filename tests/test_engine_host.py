@@ -41,7 +41,7 @@ def cfg_from_fixture(f):
     n, S, elite, tour, nr, ip, effmut, inserts, rpos, start = (int(v) for v in f["cfg"])
     c.population, c.stride, c.elite_count, c.tournament_size = n, S, elite, tour
     c.num_registers, c.input_protect, c.op_set, c.effmut = nr, ip, 0, effmut
-    c.macro_inserts, c.macro_random_pos, c.starting_length, c.reserved = inserts, rpos, start, 0
+    c.macro_inserts, c.macro_random_pos, c.starting_length, c.starting_length_max = inserts, rpos, start, 0
     (c.crossover_rate, c.micro_rate, c.micro_per_instr, c.effmut_rate, c.insert_rate,
      c.delete_rate) = (float(v) for v in f["rates"])
     return c
@@ -117,6 +117,82 @@ def test_pcg32_init_matches_reference():
     assert np.isnan(fit).all()
 
 
+class _Pcg32:
+    """O'Neill's pcg32 (XSH RR, 64-bit LCG with multiplier 6364136223846793005),
+    seeded as the reference's PCG32(seed, stream): increment 2*stream+1, one
+    step, add the seed, one step.  Its first draws are pinned below by the
+    reference's initializer vectors."""
+    M = (1 << 64) - 1
+
+    def __init__(self, seed, stream):
+        self.state, self.inc = 0, ((stream << 1) | 1) & self.M
+        self.next()
+        self.state = (self.state + seed) & self.M
+        self.next()
+
+    def next(self):
+        old = self.state
+        self.state = (old * 6364136223846793005 + self.inc) & self.M
+        x = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((x >> rot) | (x << ((32 - rot) & 31))) & 0xFFFFFFFF
+
+
+def test_initial_length_option():
+    """starting_length_max > 0 (an option of this engine; the reference fixes
+    every initial length, src/LGPEngine.cpp:67): the words are those of the
+    reference's initializer, and each length is lo + floor(u * span / 2^32)
+    for the stream's next draw u, so lengths cover [lo, hi] evenly."""
+    L = _host()
+    f = load_fixture("vary_em.npz")
+    c = cfg_from_fixture(f)
+    n, S, base = c.population, c.stride, int(f["init_base"][0])
+    rmask = c.num_registers - 1
+    mask = 7 | rmask << 8 | rmask << 16 | rmask << 24 | 1 << 31
+    # the restated generator against the reference's vectors (first programs)
+    for i in range(3):
+        g = _Pcg32(base, i)
+        assert [g.next() & mask for _ in range(S)] == [int(w) for w in f["init_dense"][i]]
+    lo, hi = 2, min(S, 40)
+    c.starting_length, c.starting_length_max = lo, hi
+    instr = np.zeros((n, S), np.uint32)
+    lens = np.zeros(n, np.uint8)
+    fit = np.zeros(n, np.float32)
+    L.gv_init(ctypes.byref(c), base, instr.ctypes.data, lens.ctypes.data, fit.ctypes.data)
+    assert np.array_equal(instr, f["init_dense"])
+    for i in range(0, n, max(1, n // 40)):
+        g = _Pcg32(base, i)
+        for _ in range(S):
+            g.next()
+        assert lens[i] == lo + ((g.next() * (hi - lo + 1)) >> 32), i
+    assert lens.min() >= lo and lens.max() <= hi
+    # the Python mirror used by the serial initializer gives the same lengths
+    from galop_amd.engine import VaryCfg
+    v = VaryCfg.default(n, stride=S, starting_length=lo, starting_length_max=hi)
+    draws = np.array([2 ** 32 - 1, 0, 2 ** 31, 123456789], np.uint64)
+    assert list(v.initial_lengths(draws)) == [lo + ((int(u) * (hi - lo + 1)) >> 32) for u in draws]
+    assert v.initial_lengths(draws)[0] == hi and v.initial_lengths(draws)[1] == lo
+    # off (0): the reference's fixed length
+    assert (VaryCfg.default(n, stride=S).initial_lengths(draws) == 8).all()
+
+
+def test_initial_lengths_cover_range():
+    """U[8, 32] over 20,000 programs: every value occurs, none outside."""
+    from galop_amd._lib import galop_vary_cfg
+    L = _host()
+    c = galop_vary_cfg()
+    c.population, c.stride, c.elite_count, c.tournament_size = 20000, 64, 0, 4
+    c.num_registers, c.input_protect, c.op_set, c.effmut = 8, 0, 0, 0
+    c.macro_inserts, c.macro_random_pos, c.starting_length, c.starting_length_max = 1, 1, 8, 32
+    instr = np.zeros((20000, 64), np.uint32)
+    lens = np.zeros(20000, np.uint8)
+    fit = np.zeros(20000, np.float32)
+    L.gv_init(ctypes.byref(c), 77, instr.ctypes.data, lens.ctypes.data, fit.ctypes.data)
+    counts = np.bincount(lens, minlength=64)
+    assert counts[:8].sum() == 0 and counts[33:].sum() == 0
+    assert (counts[8:33] > 20000 / 25 * 0.8).all() and (counts[8:33] < 20000 / 25 * 1.2).all()
+
+
 def test_stats_match_reference():
     L = _host()
     f = load_fixture("engine_seed42.npz")
@@ -154,7 +230,7 @@ def test_planned_variation_equals_vary_pair(arm):
     c = galop_vary_cfg()
     c.population, c.stride, c.elite_count, c.tournament_size = 6000, S, 600, T
     c.num_registers, c.input_protect, c.op_set, c.effmut = nr, ip, 0, 0
-    c.macro_inserts, c.macro_random_pos, c.starting_length, c.reserved = ins, rpos, 8, 0
+    c.macro_inserts, c.macro_random_pos, c.starting_length, c.starting_length_max = ins, rpos, 8, 0
     (c.crossover_rate, c.micro_rate, c.micro_per_instr, c.effmut_rate, c.insert_rate,
      c.delete_rate) = rates
     n = c.population

@@ -82,6 +82,35 @@ def test_device_init_matches_reference(gpu_ctx):
     assert torch.isnan(fit).all()
 
 
+def test_device_init_initial_length_option(gpu_ctx):
+    """Initial lengths U[8, 32] (starting_length_max = 32, stride 64): the
+    device initializer equals the host build word for word and length for
+    length (the host build is pinned by tests/test_engine_host.py)."""
+    import torch
+    from galop_amd._lib import galop_vary_cfg
+    from galop_amd.engine import init_population_device
+    from gpu_util import stream
+    L = _host()
+    s = stream()
+    c = galop_vary_cfg()
+    c.population, c.stride, c.elite_count, c.tournament_size = 50000, 64, 0, 4
+    c.num_registers, c.input_protect, c.op_set, c.effmut = 8, 0, 0, 0
+    c.macro_inserts, c.macro_random_pos, c.starting_length, c.starting_length_max = 1, 1, 8, 32
+    n, S = c.population, c.stride
+    h_instr = np.zeros((n, S), np.uint32)
+    h_lens = np.zeros(n, np.uint8)
+    h_fit = np.zeros(n, np.float32)
+    L.gv_init(ctypes.byref(c), 991, h_instr.ctypes.data, h_lens.ctypes.data, h_fit.ctypes.data)
+    instr = torch.zeros(n * S, dtype=torch.int32, device="cuda")
+    lens = torch.zeros(n, dtype=torch.uint8, device="cuda")
+    fit = torch.zeros(n, dtype=torch.float32, device="cuda")
+    init_population_device(gpu_ctx, _cfg_obj(c), 991, instr, lens, fit, stream=s)
+    torch.cuda.synchronize()
+    assert np.array_equal(instr.cpu().numpy().view(np.uint32).reshape(n, S), h_instr)
+    assert np.array_equal(lens.cpu().numpy(), h_lens)
+    assert h_lens.min() == 8 and h_lens.max() == 32
+
+
 def test_device_stats_match_reference(gpu_ctx):
     from galop_amd.engine import generation_stats_device
     from gpu_util import stream
@@ -185,7 +214,7 @@ def test_large_population_two_generations_device_equals_host(gpu_ctx, P, S, E, v
     c = galop_vary_cfg()
     c.population, c.stride, c.elite_count, c.tournament_size = P, S, E, 4
     c.num_registers, c.input_protect, c.op_set, c.effmut = 16, 1, 1, 1
-    c.macro_inserts, c.macro_random_pos, c.starting_length, c.reserved = 2, 1, min(8, S), 0
+    c.macro_inserts, c.macro_random_pos, c.starting_length, c.starting_length_max = 2, 1, min(8, S), 0
     (c.crossover_rate, c.micro_rate, c.micro_per_instr, c.effmut_rate, c.insert_rate,
      c.delete_rate) = (0.9, 0.8, 0.0, 0.7, 0.6, 0.5)
     if variant == "per_instr":
