@@ -35,14 +35,14 @@ SIMDS_PER_CU = 4               # one wave64 VALU instruction per SIMD per 4 cloc
 # programs, seed 0), measured with rocprofv3 --pmc; the commands and raw
 # figures are in profiles/plain_kernel_plain_workload_counters.md.  FETCH_SIZE
 # and WRITE_SIZE are scaled by what each reads for a 1 GiB device copy
-# (1,048,576 KiB copied: FETCH_SIZE 524,307 KiB, WRITE_SIZE 1,048,576 KiB).
+# (1,048,576 KiB copied: FETCH_SIZE 524,308.8 KiB, WRITE_SIZE 1,048,576 KiB).
 # They describe that exact population only.
 PLAIN_WORKLOAD_COUNTERS = {
-    "fetch_bytes": 320867.3 * 1024 * (1048576 / 524307.4),
-    "write_bytes": 25671.9 * 1024 * 1.0,
-    "valu_insts": 25649597039,
-    "salu_insts": 7761334077,
-    "valu_busy_pct": 86.8,
+    "fetch_bytes": 319510.8 * 1024 * (1048576 / 524308.8),
+    "write_bytes": 16384.0 * 1024 * 1.0,
+    "valu_insts": 21757034366,
+    "salu_insts": 7067946336,
+    "valu_busy_pct": 83.9,
 }
 
 
@@ -673,6 +673,13 @@ def main():
                                     "excused": cbn["excused"]},
             "vs_device_math_oracle": {"max_rel_err": cbn["device_math_max_rel_err"],
                                       "within_1e-6": cbn["device_math_within_1e-6"]},
+            "error_bound_every_fitness": {
+                "sse_rel": 1.3e-10 + wl.n_cases * 2.0 ** -53,
+                "fitness_rel": 2.0 ** -22,
+                "note": "a priori, for every fitness of every program: the compensated lane "
+                        "sums, lane tree and block sums against the FP64 case-order oracle on "
+                        "the same per-case outputs (which equal the oracle's with the device's "
+                        "unary functions)"},
         }
     if args.workload == "plain" and args.kernel == "plain" and not args.no_effective:
         # The same launch with the effective-word variant: only the words that

@@ -825,7 +825,9 @@ serve_kernel(const ServeDesc* __restrict__ desc, const int64_t* __restrict__ ord
     const ServeDesc* __restrict__ d = desc + r;
     const DevData ds = d->ds;
     const ConstRegs cr = load_const_regs(ds.consts, lane);
-    const int64_t lo = d->offsets[p], hi = d->offsets[p + 1];
+    // the bounds are wave-uniform: broadcast, so the program's base and
+    // length live in SGPRs
+    const int64_t lo = readlane_i64(d->offsets[p], 0), hi = readlane_i64(d->offsets[p + 1], 0);
     const double sse = eval_program_wave<NR, K, EXT>(d->words + lo, hi - lo, ds, lane, cr, nullptr);
     if (lane == 0) {
         d->fitness[p] = fitness_of_sse(sse, ds);

@@ -217,9 +217,6 @@ __device__ __forceinline__ DestDecode dest_decode(int nr, int iprot) {
     return dd;
 }
 
-#ifndef GALOP_ROW_GROUP
-#define GALOP_ROW_GROUP 4
-#endif
 // Ends a group of table-row cases: the scheduler moves no instruction across
 // it, so one group's loads and arithmetic are not spread into the next.
 #define GALOP_GROUP_FENCE() __builtin_amdgcn_sched_barrier(0)
@@ -233,8 +230,12 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
     const int s2 = (int)((w >> 24) & (uint32_t)(NR - 1));
     const bool kc = (w >> 31) != 0u;
     // sin, cos and log read a table row per case; the rows of this many cases
-    // are loaded before any of them is used (a power of two dividing K)
-    constexpr int kRowGroup = K < GALOP_ROW_GROUP ? K : GALOP_ROW_GROUP;
+    // are loaded before any of them is used (a power of two dividing K).
+    // Four in flight measured 1-2.5% faster than two where the registers
+    // allow; at 8 cases per lane four make the kernel spill 32 bytes per
+    // lane (program bounds and sums, written to HBM for every program), and
+    // two measure the same with none.
+    constexpr int kRowGroup = K >= 8 ? 2 : (K < 4 ? K : 4);
 
 // A constant operand stays one scalar value for all K cases: the compute is
 // written out in both branches, so it is never copied into K vector values.
@@ -320,11 +321,6 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
             if (special) {                                                \
                 _Pragma("unroll") for (int k = 0; k < K; ++k)             \
                     cv[k] = galop_math::sincos_special_value(bv[k], WANT_COS); \
-            } else if (mixed) {                                           \
-                _Pragma("unroll") for (int k = 0; k < K; ++k)             \
-                    cv[k] = galop_math::sincos_needs_ph(bv[k])            \
-                                ? galop_math::sincos_ph(bv[k], WANT_COS, C) \
-                                : galop_math::sincos_cw(bv[k], WANT_COS, C); \
             } else {                                                      \
                 /* the table rows of kRowGroup cases are loaded before */ \
                 /* any of them is used */                                 \
@@ -343,6 +339,18 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                 }                                                         \
             }                                                             \
             _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]); \
+            /* then the cases that need Payne-Hanek, one at a time, in */ \
+            /* the register file (their Cody-Waite result is replaced) */ \
+            if (mixed) {                                                  \
+                _Pragma("unroll") for (int k = 0; k < K; ++k) {           \
+                    if (__any(galop_math::sincos_needs_ph(bv[k]))) {      \
+                        float v = r.get(k, d);                            \
+                        if (galop_math::sincos_needs_ph(bv[k]))           \
+                            v = galop_math::sincos_ph(bv[k], WANT_COS, C); \
+                        r.set(k, d, v);                                   \
+                    }                                                     \
+                }                                                         \
+            }                                                             \
         }                                                                 \
     } while (0)
 
@@ -449,11 +457,15 @@ __device__ __forceinline__ void run_program(const uint32_t* __restrict__ prog,
                                             int64_t len, int lane,
                                             RegFile<NR, K>& r, ConstRegs cr,
                                             DestDecode dd) {
-    uint32_t next = (lane < len) ? prog[lane] : 0u;
+    // the chunk base advances as a wave-uniform pointer and each lane adds
+    // its 32-bit word offset, so no 64-bit per-lane address stays live
+    const uint32_t* __restrict__ p = prog;
+    const uint32_t loff = (uint32_t)lane;
+    uint32_t next = (lane < len) ? p[loff] : 0u;
     for (int64_t base = 0; base < len; base += kWave) {
         const uint32_t chunk = next;
-        const int64_t nb = base + kWave;
-        next = (nb + lane < len) ? prog[nb + lane] : 0u;
+        p += kWave;
+        next = (base + kWave + lane < len) ? p[loff] : 0u;
         const int cnt = (int)((len - base) < kWave ? (len - base) : kWave);
         for (int j = 0; j < cnt; ++j)
             exec_word<NR, K, EXT>(wave_word(chunk, j), r, cr, dd);
@@ -630,6 +642,12 @@ __device__ __forceinline__ void load_cases(const DevData& ds, int64_t first,
     }
 }
 
+// LaneAcc's state, parked in LDS between case tiles.
+struct LaneAccState {
+    float s, c;
+    double big;
+};
+
 // Per-lane compensated accumulator of squared errors.
 struct LaneAcc {
     float s = 0.0f;    // FP32 running sum
@@ -658,6 +676,16 @@ struct LaneAcc {
     }
     __device__ __forceinline__ double total() const {
         return ((double)s + (double)c) + big;
+    }
+    __device__ __forceinline__ void load(const LaneAccState& st) {
+        s = st.s;
+        c = st.c;
+        big = st.big;
+    }
+    __device__ __forceinline__ void save(LaneAccState& st) const {
+        st.s = s;
+        st.c = c;
+        st.big = big;
     }
 };
 
@@ -699,34 +727,54 @@ __device__ __forceinline__ float fitness_of_sse(double sse, const DevData& ds) {
 // block, so a program's SSE is the same bits whether one wave walks all its
 // blocks or many waves share them (split_kernel).  kCaseBlock is a multiple
 // of every tile size 64*K.
-constexpr int64_t kCaseBlock = 16384;
+constexpr int kCaseBlockLog2 = 14;
+constexpr int64_t kCaseBlock = (int64_t)1 << kCaseBlockLog2;
 
-// Partial of the block of cases [c0, min(c0 + kCaseBlock, n_cases)); valid in
-// lane 0.  When case_out is non-null the per-case outputs are written there
+// Number of case blocks of a dataset, as a 32-bit count: loops over blocks
+// and tiles compare 32-bit values, which the scalar unit does (a signed
+// 64-bit comparison is a vector instruction, and would keep the bound in
+// VGPRs for the whole program).
+__device__ __forceinline__ int case_blocks(int64_t n_cases) {
+    return (int)((n_cases + kCaseBlock - 1) >> kCaseBlockLog2);
+}
+
+// Partial of the block of cases [c0, min(c0 + kCaseBlock, n_cases)), for
+// c0 < n_cases; valid in lane 0.  When case_out is non-null the per-case outputs are written there
 // instead of being accumulated.
 // `run(r)` runs the program on one tile's registers.
 template <int NR, int K, typename Run>
 __device__ __forceinline__ double block_sse_run(const DevData& ds, int lane, int64_t c0,
                                                 float* __restrict__ case_out, Run run) {
-    const int64_t tile_cases = (int64_t)kWave * K;
-    const int64_t c1 = (c0 + kCaseBlock) < ds.n_cases ? (c0 + kCaseBlock) : ds.n_cases;
+    constexpr int tile_cases = kWave * K;
+    // cases in this block (c0 < n_cases): kCaseBlock, or fewer in the last
+    const int64_t rem = ds.n_cases - c0;
+    const int nb = (rem >> kCaseBlockLog2) != 0 ? (int)kCaseBlock : (int)rem;
     const int out_reg = ds.out_reg;
-    LaneAcc acc;
-    for (int64_t first = c0; first < c1; first += tile_cases) {
+    // Between tiles each lane's accumulator waits in LDS (every kernel that
+    // runs the interpreter has at most 256 threads per block), so it holds
+    // no VGPRs while the program runs.
+    __shared__ LaneAccState acc_slot[256];
+    LaneAccState& slot = acc_slot[threadIdx.x];
+    for (int f = 0; f < nb; f += tile_cases) {
+        const int64_t first = c0 + f;
         RegFile<NR, K> r;
         load_cases<NR, K>(ds, first, lane, r);
         run(r);
         const float* __restrict__ tgt = ds.targets + first;
+        LaneAcc acc;
+        if (f != 0) acc.load(slot);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            if (first + k * kWave + lane < c1) {
+            if (f + k * kWave + lane < nb) {
                 const float out = r.get(k, out_reg);
                 if (case_out) case_out[first + k * kWave + lane] = out;
                 else acc.add(out, tgt[k * kWave + lane]);
             }
         }
+        if (f + tile_cases >= nb) return wave_sum_f64(acc.total());
+        acc.save(slot);
     }
-    return wave_sum_f64(acc.total());
+    return wave_sum_f64(0.0);  // no tile: c0 >= c1
 }
 
 template <int NR, int K, int EXT>
@@ -777,12 +825,14 @@ __device__ __forceinline__ double eval_program_wave(const uint32_t* __restrict__
         EffProgram e;
         if (len <= kSkipMaxWords)
             mark_effective<NR, EXT>(prog, len, lane, ds.out_reg, ds.input_protect, e);
-        for (int64_t c0 = 0; c0 < ds.n_cases; c0 += kCaseBlock)
-            sse += block_sse_eff<NR, K, EXT>(prog, len, e, ds, lane, cr, dd, c0, case_out);
+        for (int b = 0; b < case_blocks(ds.n_cases); ++b)
+            sse += block_sse_eff<NR, K, EXT>(prog, len, e, ds, lane, cr, dd,
+                                             (int64_t)b << kCaseBlockLog2, case_out);
         return sse;
     }
-    for (int64_t c0 = 0; c0 < ds.n_cases; c0 += kCaseBlock)
-        sse += block_sse<NR, K, EXT>(prog, len, ds, lane, cr, dd, c0, case_out);
+    for (int b = 0; b < case_blocks(ds.n_cases); ++b)
+        sse += block_sse<NR, K, EXT>(prog, len, ds, lane, cr, dd, (int64_t)b << kCaseBlockLog2,
+                                     case_out);
     return sse;
 }
 
