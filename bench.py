@@ -91,11 +91,13 @@ def cpu_baseline(wl, gpu_fit, rep_s, threads, reps=5):
     over = np.nonzero(rel > 1e-6)[0]
     excused = []
     for i in over[:2000]:
-        a = O.attribution(isa, words[offs[i]:offs[i + 1]], x, y, 1e-6)
+        a = O.attribution(isa, words[offs[i]:offs[i + 1]], x, y, 1e-6, gpu_fitness=g[i])
         excused.append({"program": int(i), "rel_err": float(rel[i]), "attributed": a["attributed"],
                         "reason": a["reason"],
                         "float_double_gap_rel": a["float_double_gap_rel"],
-                        "ulp_sensitivity_rel": a["ulp_sensitivity_rel"]})
+                        "ulp_sensitivity_rel": a["ulp_sensitivity_rel"],
+                        "libm_choice_gap_rel": a["libm_choice_gap_rel"],
+                        "gpu_vs_double_libm_rel": a["gpu_vs_double_libm_rel"]})
     # the same prefix with the device's own unary functions: only the
     # summation differs, so every fitness should be within 1e-6
     math_host = os.path.join(REPO, "galop_amd", "libgalop_math_host.so")
@@ -407,7 +409,8 @@ def bench_casebound(ctx, programs, steps, warmup, threads):
     r_fit = rel(g_fit, m_fit)
     r_host = rel(g_fit, h_fit)
     over = np.nonzero(r_host > 1e-6)[0]
-    attributed = sum(O.attributed(isa, words[offs[i]:offs[i + 1]], x, y, 1e-6) for i in over[:64])
+    attributed = sum(O.attributed(isa, words[offs[i]:offs[i + 1]], x, y, 1e-6, gpu_fitness=g_fit[i])
+                     for i in over[:64])
     return {
         "workload": f"first {P} of the 65536 x 128-instruction population; {wl.notes}",
         "programs": P, "cases": N, "ms": round(ms, 2),

@@ -59,6 +59,25 @@ void go_set_unary_math(go_unary_fn sin_fn, go_unary_fn cos_fn, go_unary_fn exp_f
     g_plog = plog_fn ? plog_fn : plog_libm;
 }
 
+/* A second faithful implementation of the same four operations: the C
+ * library's double-precision functions, rounded once to float.  They are
+ * correctly rounded for all but the rarest arguments, and are independent of
+ * the float functions above, which misround about one argument in a hundred
+ * (sin, cos) to one in a few thousand (exp, log).  The attribution probe
+ * compares a program's fitness under the two. */
+static int g_fault;
+static float sin_via_double(float b) { return (float)sin((double)b); }
+static float cos_via_double(float b) { return (float)cos((double)b); }
+static float exp_via_double(float b) { return (float)exp((double)b); }
+static float plog_via_double(float b) { return (b != 0.0f) ? (float)log(fabs((double)b)) : 0.0f; }
+
+void go_use_double_libm(int32_t on) {
+    if (on && g_fault != GO_FAULT_DOUBLE_LIBM_IS_FLOAT)
+        go_set_unary_math(sin_via_double, cos_via_double, exp_via_double, plog_via_double);
+    else
+        go_set_unary_math(NULL, NULL, NULL, NULL);
+}
+
 /* ---- apply_op: src/ISA.h:219-231 (REF8) plus the EXT8 table ------------ */
 float go_apply_op(int op_set, int op, float a, float b) {
     switch (op) {
@@ -142,7 +161,6 @@ double go_program_sse(const go_isa* isa, const uint32_t* prog, int64_t len,
  * A test sets one of these to show that its checks of the device-order sum
  * and of the attribution probes fail when the code is wrong in that way; 0
  * restores the real code. */
-static int g_fault = GO_FAULT_NONE;
 void go_plant_fault(int32_t code) { g_fault = code; }
 
 /* ---- the GPU evaluator's summation order ---------------------------------

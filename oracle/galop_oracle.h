@@ -65,6 +65,12 @@ float go_apply_op(int op_set, int op, float a, float b);
 void go_set_unary_math(float (*sin_fn)(float), float (*cos_fn)(float),
                        float (*exp_fn)(float), float (*plog_fn)(float));
 
+/* on != 0: evaluate with the C library's double-precision sin, cos, exp and
+ * log rounded to float (a second faithful implementation of the unary ops,
+ * for the attribution probe); 0 restores the float functions (the default).
+ * Not thread-safe, like go_set_unary_math. */
+void go_use_double_libm(int32_t on);
+
 /* Per-case outputs r[OUTPUT_REG] of one program over n_cases cases.
  * inputs are case-major AoS: inputs[c*n_inputs + i]. */
 void go_run_program(const go_isa* isa, const uint32_t* prog, int64_t len,
@@ -146,6 +152,7 @@ float go_perturbed_fitness(const go_isa* isa, const uint32_t* prog, int64_t len,
 #define GO_FAULT_PERTURB_DROPPED 3      /* perturbation never applied            */
 #define GO_FAULT_PERTURB_ALL_OPS 4      /* every op perturbed, not the unary ones */
 #define GO_FAULT_DEVSUM_UNCOMPENSATED 5 /* device-order sum: compensation dropped */
+#define GO_FAULT_DOUBLE_LIBM_IS_FLOAT 6 /* go_use_double_libm keeps the float functions */
 void go_plant_fault(int32_t code);
 
 #ifdef __cplusplus

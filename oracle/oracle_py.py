@@ -72,6 +72,8 @@ def lib():
         L.go_set_unary_math.argtypes = [vp, vp, vp, vp]
         L.go_plant_fault.restype = None
         L.go_plant_fault.argtypes = [i32]
+        L.go_use_double_libm.restype = None
+        L.go_use_double_libm.argtypes = [i32]
         L.go_sse_device_order.restype = ctypes.c_double
         L.go_sse_device_order.argtypes = [vp, vp, i64]
         L.go_program_sse_device_order.restype = ctypes.c_double
@@ -215,21 +217,62 @@ def libm_sensitivity(isa, prog, inputs, targets, ulps=1, seeds=8):
     return base, worst
 
 
-def attribution(isa, prog, inputs, targets, tol_rel):
+class double_libm:
+    """Context manager: evaluate with the C library's double-precision sin,
+    cos, exp and log rounded to float (go_use_double_libm), a second faithful
+    implementation of the unary ops beside the float functions."""
+
+    def __enter__(self):
+        lib().go_use_double_libm(1)
+        return self
+
+    def __exit__(self, *a):
+        lib().go_use_double_libm(0)
+
+
+def libm_choice_fitness(isa, prog, inputs, targets):
+    """(fitness with the C library's float functions, fitness with its
+    double functions rounded to float) of one program."""
+    prog = np.ascontiguousarray(prog, np.uint32)
+    offs = np.array([0, prog.size], np.int64)
+    f_float, _ = eval_csr(isa, prog, offs, inputs, targets, threads=1)
+    with double_libm():
+        f_double, _ = eval_csr(isa, prog, offs, inputs, targets, threads=1)
+    return float(f_float[0]), float(f_double[0])
+
+
+def attribution_reason(gap, sens, choice, gpu_rel, tol_rel):
+    """The rule that excuses a program (see attribution), or None."""
+    if gap > tol_rel:
+        return "float_vs_double"
+    if sens > tol_rel:
+        return "libm_1ulp"
+    if gpu_rel is not None and choice > tol_rel and gpu_rel <= tol_rel:
+        return "libm_choice"
+    return None
+
+
+def attribution(isa, prog, inputs, targets, tol_rel, gpu_fitness=None):
     """Why a program's fitness is not defined to tol_rel by its arithmetic, if
-    it is not.  Two rules, checked in this order:
+    it is not.  Three rules, checked in this order:
 
     * "float_vs_double": the reference's chaos rule (tests/cuda_diff.cu:162-181)
       -- the fitness with double registers differs from the fitness with float
       registers by more than tol_rel (division protected in both legs);
     * "libm_1ulp": moving every sin/cos/exp/log result by up to one ulp moves
       the fitness by more than tol_rel (two C libraries that differ by one ulp
-      per operation cannot agree to tol_rel on it).
+      per operation cannot agree to tol_rel on it);
+    * "libm_choice" (only with gpu_fitness): the fitness with the C library's
+      float functions and with its double functions rounded to float -- two
+      faithful implementations of the same ops -- differ by more than
+      tol_rel, and gpu_fitness is within tol_rel of the second.
 
     Returns {"attributed", "reason", "float_double_gap_rel",
-    "ulp_sensitivity_rel"}; the gaps are relative to the float-register
-    fitness.  Fitness is finite by construction (non-finite -> 0,
-    src/Evaluator.cpp:88-92), so every gap is a finite number."""
+    "ulp_sensitivity_rel", "libm_choice_gap_rel", "gpu_vs_double_libm_rel"};
+    the gaps are relative to the float-register fitness (the last to the
+    double-libm fitness; None without gpu_fitness).  Fitness is finite by
+    construction (non-finite -> 0, src/Evaluator.cpp:88-92), so every gap is a
+    finite number."""
     ff, fd = dual_precision_fitness(isa, prog, inputs, targets)
     base, worst = libm_sensitivity(isa, prog, inputs, targets)
     if not (np.isfinite(ff) and np.isfinite(fd) and np.isfinite(base) and np.isfinite(worst)):
@@ -237,18 +280,21 @@ def attribution(isa, prog, inputs, targets, tol_rel):
     scale = max(abs(ff), 1e-30)
     gap = abs(fd - ff) / scale
     sens = worst / scale
-    reason = None
-    if gap > tol_rel:
-        reason = "float_vs_double"
-    elif sens > tol_rel:
-        reason = "libm_1ulp"
+    f_flt, f_dbl = libm_choice_fitness(isa, prog, inputs, targets)
+    choice = abs(f_dbl - f_flt) / max(abs(f_flt), 1e-30)
+    gpu_rel = None
+    if gpu_fitness is not None:
+        gpu_rel = abs(float(gpu_fitness) - f_dbl) / max(abs(f_dbl), 1e-30)
+    reason = attribution_reason(gap, sens, choice, gpu_rel, tol_rel)
     return {"attributed": reason is not None, "reason": reason,
-            "float_double_gap_rel": float(gap), "ulp_sensitivity_rel": float(sens)}
+            "float_double_gap_rel": float(gap), "ulp_sensitivity_rel": float(sens),
+            "libm_choice_gap_rel": float(choice),
+            "gpu_vs_double_libm_rel": None if gpu_rel is None else float(gpu_rel)}
 
 
-def attributed(isa, prog, inputs, targets, tol_rel):
+def attributed(isa, prog, inputs, targets, tol_rel, gpu_fitness=None):
     """True when attribution() finds a reason (see there)."""
-    return attribution(isa, prog, inputs, targets, tol_rel)["attributed"]
+    return attribution(isa, prog, inputs, targets, tol_rel, gpu_fitness)["attributed"]
 
 
 class planted_fault:
@@ -256,7 +302,7 @@ class planted_fault:
     mistakes (oracle/galop_oracle.h) in the attribution probes."""
 
     (NONE, DUAL_UNPROTECTED_DIV, DUAL_LEG_FLOAT, PERTURB_DROPPED, PERTURB_ALL_OPS,
-     DEVSUM_UNCOMPENSATED) = range(6)
+     DEVSUM_UNCOMPENSATED, DOUBLE_LIBM_IS_FLOAT) = range(7)
 
     def __init__(self, code):
         self.code = code

@@ -45,6 +45,6 @@ def test_hip_diff_driver(tag):
     rel = rel_err(packed, ref)
     isa = O.make_isa(8)
     for i in np.nonzero(rel > 1e-6)[0]:
-        assert O.attributed(isa, words[offs[i]:offs[i + 1]], f["inputs"], f["targets"], 1e-6), \
-            f"well-defined program {i} disagrees"
+        assert O.attributed(isa, words[offs[i]:offs[i + 1]], f["inputs"], f["targets"], 1e-6,
+                            gpu_fitness=packed[i]), f"well-defined program {i} disagrees"
     assert (rel > 1e-6).sum() <= lengths.size // 100

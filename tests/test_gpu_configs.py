@@ -55,7 +55,7 @@ def _check_against_oracles(isa, words, offsets, x, y, fit, sse, label):
     over = np.nonzero(r > 1e-6)[0]
     reasons = []
     for i in over:
-        a = O.attribution(oisa, words[offsets[i]:offsets[i + 1]], x, y, 1e-6)
+        a = O.attribution(oisa, words[offsets[i]:offsets[i + 1]], x, y, 1e-6, gpu_fitness=fit[i])
         assert a["attributed"], (label, int(i), float(r[i]), a)
         reasons.append(a["reason"])
     counts = {k: reasons.count(k) for k in sorted(set(reasons))}

@@ -250,7 +250,8 @@ def test_fitness_vs_oracle(gpu_ctx, nr, op_set, ip, N, k, P, lo, hi):
     r = rel_err(fit, ofit)
     assert (r[free] <= REL_TOL).all(), f"max rel {r[free].max()} on transcendental-free programs"
     over = np.nonzero(r > REL_TOL)[0]
-    attributed = sum(O.attributed(oisa, words[offsets[i]:offsets[i + 1]], x, y, REL_TOL)
+    attributed = sum(O.attributed(oisa, words[offsets[i]:offsets[i + 1]], x, y, REL_TOL,
+                                  gpu_fitness=fit[i])
                      for i in over)
     print(f"vs host libm: over 1e-6 {over.size}/{P}, attributed {attributed}, "
           f"max rel {r.max():.2e}")
@@ -298,8 +299,8 @@ def test_engine_population_dense_layouts(gpu_ctx, tag):
     oisa = O.make_isa(8)
     over = np.nonzero(r > REL_TOL)[0]
     for i in over:
-        assert O.attributed(oisa, words[offs[i]:offs[i + 1]], f["inputs"], f["targets"], REL_TOL), \
-            f"well-defined program {i} disagrees"
+        assert O.attributed(oisa, words[offs[i]:offs[i + 1]], f["inputs"], f["targets"], REL_TOL,
+                            gpu_fitness=got["packed"][i]), f"well-defined program {i} disagrees"
     assert over.size <= lengths.size // 100
     assert np.abs(got["packed"] - ref).max() < 1e-4  # the reference's own cuda_diff bar
 
@@ -323,7 +324,8 @@ def test_r2_fitness_engine_populations(gpu_ctx, tag):
     oisa = O.make_isa(8, fitness=O.FITNESS_R2)
     over = np.nonzero(r > REL_TOL)[0]
     for i in over:
-        assert O.attributed(oisa, words[offs[i]:offs[i + 1]], f["inputs"], f["targets"], REL_TOL), \
+        assert O.attributed(oisa, words[offs[i]:offs[i + 1]], f["inputs"], f["targets"], REL_TOL,
+                            gpu_fitness=fit[i]), \
             f"well-defined program {i} disagrees: {fit[i]} vs {ref[i]}"
     assert over.size <= offs.size // 100
     assert (fit[ref == 0] == 0).all() or over.size > 0
@@ -411,3 +413,23 @@ def test_tiling_and_kernel_invariance(gpu_ctx):
         fb, sb = gpu_fitness(dds, words, offsets, kernel=KERNEL_BATCHED)
         assert same_bits(fb, ref[0]).all()
         assert np.array_equal(sb.view(np.uint64), ref[1].view(np.uint64))
+
+
+def test_libm_choice_programs_on_gpu(gpu_ctx):
+    """The two plain-population programs that only the libm-choice rule
+    excuses (libm_choice_plain.npz, written on the CPU): the GPU fitness is
+    bit for bit the oracle's with the C library's double functions rounded
+    to float, more than 1e-6 from the oracle's with its float functions, and
+    the rule excuses them."""
+    from galop_amd import Dataset, DeviceDataset, Isa
+    from galop_amd._lib import OPS_EXT8
+    f = load_fixture("libm_choice_plain.npz")
+    x, y, w, o = f["inputs"], f["targets"], f["words"], f["offsets"]
+    dds = DeviceDataset(gpu_ctx, Isa(8, OPS_EXT8, False, f["constants"]), Dataset(x, y))
+    fit, _ = gpu_fitness(dds, w, o)
+    assert same_bits(fit, f["fitness_double_libm"]).all()
+    assert (rel_err(fit, f["fitness_float_libm"]) > REL_TOL).all()
+    oisa = O.make_isa(8, O.OPS_EXT8, constants=list(f["constants"]))
+    for i in range(fit.size):
+        a = O.attribution(oisa, w[o[i]:o[i + 1]], x, y, REL_TOL, gpu_fitness=fit[i])
+        assert a["attributed"] and a["reason"] == "libm_choice", a

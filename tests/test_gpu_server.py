@@ -231,7 +231,8 @@ def test_serving_mix_runs_against_the_oracle(gpu_ctx):
         hfit, _ = O.eval_csr(oisa, w, o, x, y, threads=0)
         rel = rel_err(r.fitness[:n], hfit)
         for i in np.nonzero(rel > 1e-6)[0]:
-            assert O.attributed(oisa, w[o[i]:o[i + 1]], x, y, 1e-6), (r.seed, int(i))
+            assert O.attributed(oisa, w[o[i]:o[i + 1]], x, y, 1e-6, gpu_fitness=r.fitness[i]), \
+                (r.seed, int(i))
     for r in runs:
         r.pop.close()
     srv.close()

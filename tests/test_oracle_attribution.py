@@ -1,9 +1,13 @@
-"""Pins of the two attribution probes that may excuse a program from the 1e-6
-fitness bar against the host-libm oracle (oracle_py.attribution):
+"""Pins of the three attribution probes that may excuse a program from the
+1e-6 fitness bar against the host-libm oracle (oracle_py.attribution):
 
 * the float-vs-double leg, go_dual_precision_fitness -- the reference's chaos
   rule, tests/cuda_diff.cu:92-181, with protected division in both legs;
-* the 1-ulp libm-sensitivity leg, go_perturbed_fitness.
+* the 1-ulp libm-sensitivity leg, go_perturbed_fitness;
+* the libm-choice leg, oracle_py.libm_choice_fitness -- the fitness with the
+  C library's float functions against its double functions rounded to float,
+  two faithful implementations of the same ops; it excuses a program only
+  when the GPU fitness is within tolerance of the second.
 
 Each check below is one property the probes must have; together they pin the
 probes where a plausible mistake would hide a real mismatch:
@@ -18,11 +22,18 @@ probes where a plausible mistake would hide a real mismatch:
 * division by a zero register is protected in both legs (src/ISA.h:224);
 * a program that loses its input to cancellation in float is flagged by the
   float-vs-double leg only;
-* benign programs are not attributed.
+* benign programs are not attributed;
+* the double-libm functions equal numpy's double ufuncs rounded to float, lie
+  within one ulp of the float functions and differ from them on some
+  arguments;
+* two programs of the plain benchmark population that only the libm-choice
+  leg flags are excused with an agreeing GPU fitness and not with one that
+  agrees with neither library; without transcendentals the leg is silent.
 
 Then every planted mistake -- an unprotected double divide, a double leg run in
-float, a dropped perturbation, perturbation of every op, and a probe that
-always answers True -- must make at least one of the checks fail.
+float, a dropped perturbation, perturbation of every op, a double-libm leg
+that keeps the float functions, and a probe that always answers True -- must
+make at least one of the checks fail.
 """
 import numpy as np
 import pytest
@@ -204,10 +215,85 @@ def check_benign_not_attributed(attribution=O.attribution):
         assert not a["attributed"], (p, a)
 
 
+def _map_unary(op, xs):
+    L = O.lib()
+    return np.array([L.go_apply_op(1, op, 0.0, float(v)) for v in xs], np.float32)
+
+
+def check_double_libm_is_second_implementation(attribution=None):
+    """The libm-choice leg's functions are the C library's double functions
+    rounded to float: equal, bit for bit, to numpy's double ufuncs rounded to
+    float (an independent route); within one ulp of the float functions;
+    and a different implementation from them (they differ on some
+    arguments: glibc's sinf/cosf misround about 1 in 100)."""
+    from conftest import ulp_distance
+    rng = np.random.default_rng(14)
+    xs = np.concatenate([rng.uniform(-10, 10, 6000), rng.uniform(-100, 88, 3000),
+                         rng.uniform(-1e4, 1e4, 3000)]).astype(np.float32)
+    fns = {4: np.sin, 5: np.cos, 6: np.exp,
+           7: lambda v: np.where(v != 0, np.log(np.abs(v)), 0.0)}
+    for op, fn in fns.items():
+        flt = _map_unary(op, xs)
+        with O.double_libm():
+            dbl = _map_unary(op, xs)
+        with np.errstate(all="ignore"):
+            ref = fn(xs.astype(np.float64)).astype(np.float32)
+        assert (dbl.view(np.uint32) == ref.view(np.uint32)).all(), op
+        assert ulp_distance(flt, dbl).max() <= 1, op
+        if op in (4, 5):
+            assert (flt != dbl).mean() > 0.002, op
+
+
+LIBM_CHOICE = None
+
+
+def _libm_choice_fixture():
+    global LIBM_CHOICE
+    if LIBM_CHOICE is None:
+        from conftest import load_fixture
+        LIBM_CHOICE = load_fixture("libm_choice_plain.npz")
+    return LIBM_CHOICE
+
+
+def check_libm_choice_excuses_only_agreeing_gpu(attribution=O.attribution):
+    """Two programs of the plain benchmark population (libm_choice_plain.npz)
+    whose fitness moves by more than 1e-6 between the C library's float
+    functions and its double functions rounded to float, while the
+    float-vs-double and 1-ulp rules stay below 1e-6: excused as
+    "libm_choice" when the GPU fitness is within 1e-6 of the double-libm
+    fitness, not excused when it agrees with neither, nor without it."""
+    f = _libm_choice_fixture()
+    isa = O.make_isa(8, O.OPS_EXT8, constants=list(f["constants"]))
+    x, y, w, o = f["inputs"], f["targets"], f["words"], f["offsets"]
+    for i in range(o.size - 1):
+        p = w[o[i]:o[i + 1]]
+        f_flt, f_dbl = O.libm_choice_fitness(isa, p, x, y)
+        assert np.float32(f_flt) == f["fitness_float_libm"][i]
+        assert np.float32(f_dbl) == f["fitness_double_libm"][i]
+        a = attribution(isa, p, x, y, TOL, gpu_fitness=f_dbl)
+        assert a["float_double_gap_rel"] <= TOL and a["ulp_sensitivity_rel"] <= TOL, a
+        assert a["libm_choice_gap_rel"] > TOL, a
+        assert a["attributed"] and a["reason"] == "libm_choice", a
+        wrong = f_flt * (1 + 5 * TOL)  # a fitness that agrees with neither library
+        assert not attribution(isa, p, x, y, TOL, gpu_fitness=wrong)["attributed"]
+        assert not attribution(isa, p, x, y, TOL)["attributed"]
+
+
+def check_libm_choice_zero_without_transcendentals(attribution=None):
+    rng = np.random.default_rng(15)
+    y = (X1[:, 0] * 2).astype(np.float32)
+    isa = O.make_isa(8, O.OPS_EXT8, constants=POOL)
+    for p in random_programs(rng, 40, 3):
+        f_flt, f_dbl = O.libm_choice_fitness(isa, p, X1, y)
+        assert np.float32(f_flt).tobytes() == np.float32(f_dbl).tobytes(), p
+
+
 CHECKS = [check_zero_ulps_is_base, check_transcendental_free_unmoved,
           check_amplified_sin_flagged, check_exact_integer_programs_agree,
           check_protected_division_both_legs, check_cancellation_flagged,
-          check_benign_not_attributed]
+          check_benign_not_attributed, check_double_libm_is_second_implementation,
+          check_libm_choice_excuses_only_agreeing_gpu,
+          check_libm_choice_zero_without_transcendentals]
 
 
 @pytest.mark.parametrize("check", CHECKS, ids=lambda c: c.__name__)
@@ -220,6 +306,8 @@ def test_attribution_probe(check):
     (O.planted_fault.DUAL_LEG_FLOAT, check_cancellation_flagged),
     (O.planted_fault.PERTURB_DROPPED, check_amplified_sin_flagged),
     (O.planted_fault.PERTURB_ALL_OPS, check_transcendental_free_unmoved),
+    (O.planted_fault.DOUBLE_LIBM_IS_FLOAT, check_double_libm_is_second_implementation),
+    (O.planted_fault.DOUBLE_LIBM_IS_FLOAT, check_libm_choice_excuses_only_agreeing_gpu),
 ])
 def test_planted_probe_mistake_is_caught(fault, caught_by):
     with O.planted_fault(fault):
@@ -236,6 +324,20 @@ def test_always_true_probe_is_caught():
         check_benign_not_attributed(always)
     with pytest.raises(AssertionError):
         check_exact_integer_programs_agree(always)
+
+
+def test_attribution_reason_order_and_thresholds():
+    """The rules in order; libm_choice needs a GPU fitness within tol of the
+    double-libm fitness."""
+    r = O.attribution_reason
+    assert r(2e-6, 0, 0, None, TOL) == "float_vs_double"
+    assert r(0, 2e-6, 5e-6, 0.0, TOL) == "libm_1ulp"
+    assert r(0, 0, 2e-6, 0.0, TOL) == "libm_choice"
+    assert r(0, 0, 2e-6, 1e-6, TOL) == "libm_choice"
+    assert r(0, 0, 2e-6, 1.5e-6, TOL) is None
+    assert r(0, 0, 2e-6, None, TOL) is None
+    assert r(0, 0, 1e-6, 0.0, TOL) is None
+    assert r(1e-6, 1e-6, 1e-6, 0.0, TOL) is None
 
 
 def test_attributed_is_attribution():
