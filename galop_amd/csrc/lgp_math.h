@@ -94,7 +94,7 @@ GM_CONST uint32_t TWO_OVER_PI_W[8] = {0xA2F9836Eu, 0x4E441529u, 0xFC2757D1u, 0xF
 // (sin, cos)(j pi/128), j = 0..319, interleaved; rows 256..319 repeat rows
 // 0..63 so that cos (row j + 64) needs no wrap-around
 GM_CONST double SINCOS_T[640] = {
-    0x0.0p+0, 0x1.0000000000000p+0,
+    -0x0.0p+0, 0x1.0000000000000p+0,
     0x1.92155f7a3667ep-6, 0x1.ffd886084cd0dp-1,
     0x1.91f65f10dd814p-5, 0x1.ff621e3796d7ep-1,
     0x1.2d52092ce19f6p-4, 0x1.fe9cdad01883ap-1,
@@ -350,7 +350,7 @@ GM_CONST double SINCOS_T[640] = {
     -0x1.2d52092ce19f6p-4, 0x1.fe9cdad01883ap-1,
     -0x1.91f65f10dd814p-5, 0x1.ff621e3796d7ep-1,
     -0x1.92155f7a3667ep-6, 0x1.ffd886084cd0dp-1,
-    0x0.0p+0, 0x1.0000000000000p+0,
+    -0x0.0p+0, 0x1.0000000000000p+0,
     0x1.92155f7a3667ep-6, 0x1.ffd886084cd0dp-1,
     0x1.91f65f10dd814p-5, 0x1.ff621e3796d7ep-1,
     0x1.2d52092ce19f6p-4, 0x1.fe9cdad01883ap-1,
@@ -555,7 +555,6 @@ static inline CoefPtr coef_table() { return COEF_TABLE; }
 #endif
 
 GM_HD double fma_d(double a, double b, double c) { return __builtin_fma(a, b, c); }
-GM_HD double rint_d(double a) { return __builtin_rint(a); }
 
 // A coefficient copied into a VGPR pair where it is used.  One VOP3
 // instruction reads at most one SGPR pair, so the first Horner step, whose
@@ -614,11 +613,14 @@ GM_HD float expf_special_value(float x) {
 GM_HD float expf_r(float x, CoefPtr C) {
     const float xc = clampf(x, -104.5f, 89.5f);  // NaN: any value, replaced below
     const double xd = (double)xc;
-    const double k = rint_d(xd * C[kInvLn2]);
+    // k = rint(x / ln 2) by addition: |k| <= 151, and its low word is k as an
+    // int32 (two's complement under the magic's high bits)
+    const double kb = fma_d(xd, C[kInvLn2], C[kRoundMagic]);
+    const double k = kb - C[kRoundMagic];
     const double r = fma_d(-k, C[kLn2Fma + 1], fma_d(-k, C[kLn2Fma], xd));
     double p = fma_d(vreg(C[kExpC + 12]), r, C[kExpC + 11]);
     for (int n = 10; n >= 0; --n) p = fma_d(p, r, C[kExpC + n]);
-    const float v = (float)ldexp(p, (int)k);
+    const float v = (float)ldexp(p, (int32_t)(uint32_t)d2u(kb));
     return (x != x) ? x : v;
 }
 
@@ -641,7 +643,7 @@ GM_HD double log_reduce(float x, int32_t& e, uint32_t& row91, CoefPtr C) {
     const uint32_t dh = hi - kLogSplitHi;
     e = (int32_t)dh >> 20;
     const double y = u2d(((uint64_t)(hi - (dh & 0xfff00000u)) << 32) | (bits & 0xffffffffu));
-    row91 = (uint32_t)d2u(y * 128.0 + C[kRoundMagic]);
+    row91 = (uint32_t)d2u(fma_d(y, 128.0, C[kRoundMagic]));  // y * 128 is exact
     return y;
 }
 GM_HD float log_finish(double y, int32_t e, double cj, double lncj, CoefPtr C) {
@@ -733,7 +735,9 @@ GM_HD double reduce_ph(float x, int& j, CoefPtr C) {
 
 // sin(j pi/128 + t) = S_j cos t + C_j sin t from the table row (s, c) =
 // (S_j, C_j).  With sin t = t + t^3 ps and cos t - 1 = t^2 pc this is
-// (s + c t) + t^2 (s pc + c t ps): five operations after the two series.
+// s + t (c + t (s pc + c t ps)): five operations after the two series.  Off
+// the rows where s = +-0 (j = 0, 128), |s| > sin(pi/128) > 2|t|, so no
+// cancellation; at them the result is t (...).
 // For |t| <= pi/256 the series stop at t^5 and t^6: the next terms are below
 // 2^-50 |t| and 2^-70.
 GM_HD double sin_from_row(double t, double s, double c, CoefPtr C) {
@@ -744,7 +748,9 @@ GM_HD double sin_from_row(double t, double s, double c, CoefPtr C) {
     double pc = fma_d(t2, vreg(C[kCosT + 2]), C[kCosT + 1]);
     pc = fma_d(pc, t2, C[kCosT]);
     const double ct = c * t;
-    return fma_d(t2, fma_d(ct, ps, s * pc), fma_d(c, t, s));
+    // s + t (c + t (s pc + c t ps)): with row 0's s = -0 this is t (...)
+    // exactly, so sin(+-0) keeps the sign of the zero
+    return fma_d(t, fma_d(t, fma_d(ct, ps, s * pc), c), s);
 }
 GM_HD double sin_from_reduced(double t, int j, CoefPtr C) {
     return sin_from_row(t, SINCOS_T[2 * j], SINCOS_T[2 * j + 1], C);
@@ -758,18 +764,19 @@ GM_HD double sin_from_reduced(double t, int j, CoefPtr C) {
 // is still in 0..255.
 GM_HD double sincos_reduce(float x, uint32_t& j, CoefPtr C) {
     const double xd = (double)x;
-    const double kb = xd * C[kInvPio128] + C[kRoundMagic];
+    const double kb = fma_d(xd, C[kInvPio128], C[kRoundMagic]);
     const double k = kb - C[kRoundMagic];
     j = (uint32_t)d2u(kb) & 255u;
     return fma_d(-k, C[kPio128 + 1], fma_d(-k, C[kPio128], xd));
 }
 
 // The rounded result from the reduced argument and its row: infinities and
-// NaN come out NaN, cos(+-0) is 1, sin(+0) is +0, and sin(-0), the one case
-// the arithmetic gets wrong (t = -0 + 0 = +0), keeps its argument.
+// NaN come out NaN, cos(+-0) is 1 and sin(+-0) is +-0 (row 0 holds -0, see
+// sin_from_row), all from the arithmetic.
 GM_HD float sincos_finish(float x, int want_cos, double t, double s, double c, CoefPtr C) {
-    const float r = (float)sin_from_row(t, s, c, C);
-    return (want_cos || x != 0.0f) ? r : x;
+    (void)x;
+    (void)want_cos;
+    return (float)sin_from_row(t, s, c, C);
 }
 
 // sin (want_cos = 0) or cos (1) of an x that does not need Payne-Hanek, by

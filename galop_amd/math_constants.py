@@ -136,7 +136,10 @@ def sin_cos_fixed(r, prec=PREC):
 def sincos_table():
     """320 pairs (sin(j pi/128), cos(j pi/128)), j = 0..319, rounded to
     double; the multiples of pi/2 hold exact 0 and +-1.  Rows 256..319 repeat
-    rows 0..63, so cos x = sin(x + pi/2), row j + 64, needs no wrap-around."""
+    rows 0..63, so cos x = sin(x + pi/2), row j + 64, needs no wrap-around.
+    Row 0 (and its copy, row 256) holds sin 0 as -0: the result is formed as
+    S + t * (...), and -0 added to t * (...) leaves every value unchanged
+    while giving sin(-0) = -0 and sin(+0) = +0 without a select."""
     first = []
     for r in range(SINCOS_QUARTER + 1):
         sf, cf = sin_cos_fixed(r)
@@ -148,7 +151,8 @@ def sincos_table():
         q, r = divmod(j, SINCOS_QUARTER)
         s, c = first[r]
         s, c = [(s, c), (c, -s), (-s, -c), (-c, s)][q]
-        out.extend([s + 0.0, c + 0.0])  # no negative zeros
+        out.extend([s + 0.0, c + 0.0])  # no negative zeros here
+    out[0] = -0.0
     return out + out[:2 * SINCOS_QUARTER]
 
 

@@ -120,6 +120,11 @@ def test_sincos_table_identities():
     assert t[2 * (q4 // 2)] == half_sqrt2 and t[2 * (q4 // 2) + 1] == half_sqrt2
     for q in range(4):
         assert (t[2 * q4 * q], t[2 * q4 * q + 1]) == [(0.0, 1.0), (1.0, 0.0), (0.0, -1.0), (-1.0, 0.0)][q]
+    # sin 0 is held as -0 in row 0 and its copy (sin(+-0) keep their sign
+    # through s + t (...)); every other zero is +0
+    import math
+    zeros = [i for i, v in enumerate(t) if v == 0.0]
+    assert [i for i in zeros if math.copysign(1.0, t[i]) < 0] == [0, 2 * n]
     eps = Fraction(1, 1 << 51)
     for j in range(n):
         assert abs(S[j] ** 2 + C[j] ** 2 - 1) < eps
