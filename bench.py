@@ -207,8 +207,9 @@ def bench_uneven(ctx, steps, warmup, programs=0):
         "speedup_batched_vs_strip_then_plain": round((ms_strip + ms_plain_eff) / ms_batched, 3),
         "fitness_bitwise_equal": same, "tuning": tuned,
         "note": "raw ops: the reference's convention (intron words counted); executed ops: the "
-                "words that run.  Per executed op the batched queue matches the plain kernel on "
-                "the stripped programs; its gain over the raw plain launch is intron removal "
+                "words that run.  On the same executed words the queue issues the plain kernel's "
+                "instructions and gains by keeping the VALU busier (persistent waves, longest "
+                "programs first); its gain over the raw plain launch is mostly intron removal "
                 "done inside the generation",
     }
     plan.close()

@@ -41,6 +41,17 @@ class GalopError(RuntimeError):
         self.code = code
 
 
+STAGE_VECTOR = 0
+STAGE_SCALAR = 1
+
+
+class galop_batch_tuning(ctypes.Structure):
+    """include/galop_hip.h: a batch plan's launch tuning."""
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "cases_per_lane", "waves_per_block", "workers_per_cu", "chunk", "program_staging",
+        "reserved")]
+
+
 class galop_isa(ctypes.Structure):
     _fields_ = [("num_registers", ctypes.c_int32), ("op_set", ctypes.c_int32),
                 ("input_protect", ctypes.c_int32), ("fitness", ctypes.c_int32),
@@ -100,6 +111,10 @@ SIGNATURES = {
     "galop_batch_destroy": (None, [_vp]),
     "galop_batch_eval_device": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _vp, _i32, _vp]),
     "galop_batch_autotune": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, _pi32, _pi32, _pi32, _pi32]),
+    "galop_batch_get_tuning": (ctypes.c_int, [_vp, _vp]),
+    "galop_batch_set_tuning": (ctypes.c_int, [_vp, _vp]),
+    "galop_batch_autotune_cached": (ctypes.c_int, [_vp, _vp, _vp, _i64, _vp, ctypes.c_char_p, _vp,
+                                                   _pi32]),
     "galop_server_create": (ctypes.c_int, [_vp, _i64, _i32, ctypes.POINTER(_vp)]),
     "galop_server_destroy": (None, [_vp]),
     "galop_submit": (ctypes.c_int, [_vp, _vp, _vp, _vp, _i64, _vp, _vp, _pi64]),

@@ -2,15 +2,20 @@
 //
 // A plan owns the bucketing / work-queue scratch for one dataset and a
 // hipGraph of one generation: reset -> length histogram -> bucket starts ->
-// scatter into longest-first order -> [intron removal in that order: counts,
-// scan, compaction, laid out by order position] -> persistent evaluation.
+// scatter into longest-first order -> [intron removal in that order, one
+// pass: each program's effective code to the end of its own raw slot] ->
+// persistent evaluation.
 // The graph is captured on first use and replayed while the buffers it was
 // captured with stay the same; the reset is a kernel, so replays start from
 // zeroed counters.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -179,16 +184,36 @@ int galop_batch_eval_device(galop_batch* b, const uint32_t* d_words, const int64
     return e == hipSuccess ? GALOP_OK : hip_fail(ctx, e, "hipGraphLaunch");
 }
 
-int galop_batch_autotune(galop_batch* b, const uint32_t* d_words, const int64_t* d_offsets,
-                         int64_t n_programs, float* d_fitness, int32_t* cases_per_lane,
-                         int32_t* waves_per_block, int32_t* workers_per_cu, int32_t* chunk) {
-    if (!b) return set_error(nullptr, GALOP_ERR_INVALID, "galop_batch_autotune: NULL plan");
+// Milliseconds of one generation of the first `sample` programs under
+// tuning t (the faster of two runs); the plan's tuning is restored.
+static hipError_t time_tuning(galop_batch* b, const BatchTuning& t, const uint32_t* d_words,
+                              const int64_t* d_offsets, int64_t sample, float* d_fitness,
+                              hipEvent_t a, hipEvent_t z, float* best_ms) {
+    hipStream_t s = b->ctx->stream;
+    const BatchTuning saved = b->tuning;
+    b->tuning = t;
+    hipError_t e = hipSuccess;
+    float best = 1e30f;
+    for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
+        float ms = 0.0f;
+        e = hipEventRecord(a, s);
+        if (e == hipSuccess) e = batch_generation(b, d_words, d_offsets, sample, d_fitness, nullptr, s);
+        if (e == hipSuccess) e = hipEventRecord(z, s);
+        if (e == hipSuccess) e = hipEventSynchronize(z);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, z);
+        if (e == hipSuccess) best = std::min(best, ms);
+    }
+    b->tuning = saved;
+    *best_ms = best;
+    return e;
+}
+
+// The search: cases per lane x waves per block x persistent waves per CU x
+// programs per queue grab with the vector program staging, then the scalar
+// staging at the best of those.
+static int autotune_search(galop_batch* b, const uint32_t* d_words, const int64_t* d_offsets,
+                           int64_t n_programs, float* d_fitness) {
     galop_ctx* ctx = b->ctx;
-    if (n_programs <= 0 || n_programs > b->bufs.max_programs || !d_words || !d_offsets || !d_fitness)
-        return set_error(ctx, GALOP_ERR_INVALID, "galop_batch_autotune: bad population");
-    hipError_t e = hipSetDevice(ctx->device);
-    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
-    hipStream_t s = ctx->stream;
     const DevData& dd = b->ds->dev;
     const int nr = b->ds->isa.num_registers;
     // A prefix sample keeps the search short; it is bucketed like a full call.
@@ -202,7 +227,8 @@ int galop_batch_autotune(galop_batch* b, const uint32_t* d_words, const int64_t*
     const int workers[] = {8, 16, 24, 32};
     const int chunks[] = {1, 4, 16};
     hipEvent_t a, z;
-    if ((e = hipEventCreate(&a)) != hipSuccess) return hip_fail(ctx, e, "hipEventCreate");
+    hipError_t e = hipEventCreate(&a);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipEventCreate");
     if ((e = hipEventCreate(&z)) != hipSuccess) {
         hipEventDestroy(a);
         return hip_fail(ctx, e, "hipEventCreate");
@@ -212,39 +238,179 @@ int galop_batch_autotune(galop_batch* b, const uint32_t* d_words, const int64_t*
     for (int k : ks)
         for (int w : wpbs)
             for (int wk : workers)
-            for (int ch : chunks) {
-                BatchTuning t;
-                t.cases_per_lane = k;
-                t.waves_per_block = w;
-                t.workers_per_cu = wk;
-                t.chunk = ch;
-                float ms = 0.0f;
-                const BatchTuning saved = b->tuning;
-                b->tuning = t;
-                for (int rep = 0; rep < 2 && e == hipSuccess; ++rep) {
-                    e = hipEventRecord(a, s);
-                    if (e == hipSuccess)
-                        e = batch_generation(b, d_words, d_offsets, sample, d_fitness, nullptr, s);
-                    if (e == hipSuccess) e = hipEventRecord(z, s);
-                    if (e == hipSuccess) e = hipEventSynchronize(z);
-                    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, z);
+                for (int ch : chunks) {
+                    BatchTuning t;
+                    t.cases_per_lane = k;
+                    t.waves_per_block = w;
+                    t.workers_per_cu = wk;
+                    t.chunk = ch;
+                    t.program_staging = kStageVector;
+                    float ms = 0.0f;
+                    e = time_tuning(b, t, d_words, d_offsets, sample, d_fitness, a, z, &ms);
+                    if (e != hipSuccess) break;
+                    if (ms < best) {
+                        best = ms;
+                        best_t = t;
+                    }
                 }
-                b->tuning = saved;
-                if (e != hipSuccess) break;
-                if (ms < best) {
-                    best = ms;
-                    best_t = t;
-                }
-            }
+    if (e == hipSuccess) {
+        BatchTuning t = best_t;
+        t.program_staging = kStageScalar;
+        float ms = 0.0f;
+        e = time_tuning(b, t, d_words, d_offsets, sample, d_fitness, a, z, &ms);
+        if (e == hipSuccess && ms < best) {
+            best = ms;
+            best_t = t;
+        }
+    }
     hipEventDestroy(a);
     hipEventDestroy(z);
     if (e != hipSuccess) return hip_fail(ctx, e, "galop_batch_autotune");
     b->tuning = best_t;
     b->drop_graph();
-    if (cases_per_lane) *cases_per_lane = best_t.cases_per_lane;
-    if (waves_per_block) *waves_per_block = best_t.waves_per_block;
-    if (workers_per_cu) *workers_per_cu = best_t.workers_per_cu;
-    if (chunk) *chunk = best_t.chunk;
+    return GALOP_OK;
+}
+
+static void tuning_out(const BatchTuning& t, galop_batch_tuning* out) {
+    out->cases_per_lane = t.cases_per_lane;
+    out->waves_per_block = t.waves_per_block;
+    out->workers_per_cu = t.workers_per_cu;
+    out->chunk = t.chunk;
+    out->program_staging = t.program_staging;
+    out->reserved = 0;
+}
+
+// The workload a tuning is kept for: the device, the dataset's shape, the
+// plan's intron removal, and the population's size and mean length to the
+// nearest power of two.
+static int tuning_key(galop_batch* b, const int64_t* d_offsets, int64_t n, std::string* key) {
+    galop_ctx* ctx = b->ctx;
+    hipDeviceProp_t prop;
+    hipError_t e = hipGetDeviceProperties(&prop, ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipGetDeviceProperties");
+    int64_t ends[2] = {0, 0};
+    e = hipMemcpy(&ends[0], d_offsets, sizeof(int64_t), hipMemcpyDeviceToHost);
+    if (e == hipSuccess) e = hipMemcpy(&ends[1], d_offsets + n, sizeof(int64_t), hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return hip_fail(ctx, e, "galop_batch_autotune_cached: offsets");
+    const double mean = (double)(ends[1] - ends[0]) / (double)n;
+    const galop_isa& isa = b->ds->isa;
+    std::ostringstream k;
+    std::string arch = prop.gcnArchName;
+    arch = arch.substr(0, arch.find(':'));
+    k << arch << "/cu" << ctx->n_cu << "/r" << isa.num_registers << "/ops" << isa.op_set << "/ip"
+      << isa.input_protect << "/fit" << isa.fitness << "/n" << b->ds->dev.n_cases << "/in"
+      << b->ds->dev.n_inputs << "/strip" << (b->strip ? 1 : 0) << "/p2^"
+      << (int)std::floor(std::log2((double)n)) << "/w2^"
+      << (mean >= 1.0 ? (int)std::floor(std::log2(mean)) : 0);
+    *key = k.str();
+    return GALOP_OK;
+}
+
+int galop_batch_autotune(galop_batch* b, const uint32_t* d_words, const int64_t* d_offsets,
+                         int64_t n_programs, float* d_fitness, int32_t* cases_per_lane,
+                         int32_t* waves_per_block, int32_t* workers_per_cu, int32_t* chunk) {
+    if (!b) return set_error(nullptr, GALOP_ERR_INVALID, "galop_batch_autotune: NULL plan");
+    galop_ctx* ctx = b->ctx;
+    if (n_programs <= 0 || n_programs > b->bufs.max_programs || !d_words || !d_offsets || !d_fitness)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_batch_autotune: bad population");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    const int rc = autotune_search(b, d_words, d_offsets, n_programs, d_fitness);
+    if (rc != GALOP_OK) return rc;
+    if (cases_per_lane) *cases_per_lane = b->tuning.cases_per_lane;
+    if (waves_per_block) *waves_per_block = b->tuning.waves_per_block;
+    if (workers_per_cu) *workers_per_cu = b->tuning.workers_per_cu;
+    if (chunk) *chunk = b->tuning.chunk;
+    return GALOP_OK;
+}
+
+int galop_batch_get_tuning(const galop_batch* b, galop_batch_tuning* out) {
+    if (!b || !out) return set_error(b ? b->ctx : nullptr, GALOP_ERR_INVALID,
+                                     "galop_batch_get_tuning: NULL argument");
+    tuning_out(b->tuning, out);
+    return GALOP_OK;
+}
+
+int galop_batch_set_tuning(galop_batch* b, const galop_batch_tuning* t) {
+    if (!b || !t) return set_error(b ? b->ctx : nullptr, GALOP_ERR_INVALID,
+                                   "galop_batch_set_tuning: NULL argument");
+    const int nr = b->ds->isa.num_registers;
+    const int k = t->cases_per_lane;
+    const bool k_ok = k == 1 || k == 2 || k == 4 || (k == 8 && nr == 8);
+    if (!k_ok || t->waves_per_block < 1 || t->waves_per_block > 4 || t->workers_per_cu < 1 ||
+        t->workers_per_cu > 64 || t->chunk < 1 || t->chunk > kWave ||
+        (t->program_staging != GALOP_STAGE_VECTOR && t->program_staging != GALOP_STAGE_SCALAR) ||
+        t->reserved != 0)
+        return set_error(b->ctx, GALOP_ERR_INVALID, "galop_batch_set_tuning: value out of range");
+    BatchTuning bt;
+    bt.cases_per_lane = k;
+    bt.waves_per_block = t->waves_per_block;
+    bt.workers_per_cu = t->workers_per_cu;
+    bt.chunk = t->chunk;
+    bt.program_staging = t->program_staging;
+    b->tuning = bt;
+    b->drop_graph();
+    return GALOP_OK;
+}
+
+int galop_batch_autotune_cached(galop_batch* b, const uint32_t* d_words,
+                                const int64_t* d_offsets, int64_t n_programs, float* d_fitness,
+                                const char* cache_path, galop_batch_tuning* out,
+                                int32_t* cache_hit) {
+    if (!b || !cache_path) return set_error(b ? b->ctx : nullptr, GALOP_ERR_INVALID,
+                                            "galop_batch_autotune_cached: NULL argument");
+    galop_ctx* ctx = b->ctx;
+    if (n_programs <= 0 || n_programs > b->bufs.max_programs || !d_words || !d_offsets || !d_fitness)
+        return set_error(ctx, GALOP_ERR_INVALID, "galop_batch_autotune_cached: bad population");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    std::string key;
+    int rc = tuning_key(b, d_offsets, n_programs, &key);
+    if (rc != GALOP_OK) return rc;
+    // the cache: one "key<TAB>K waves_per_block workers_per_cu chunk staging" line per workload
+    std::vector<std::string> lines;
+    bool hit = false;
+    {
+        std::ifstream in(cache_path);
+        std::string line;
+        while (std::getline(in, line)) {
+            const size_t tab = line.find('\t');
+            if (tab == std::string::npos) continue;
+            if (line.compare(0, tab, key) == 0 && tab == key.size()) {
+                galop_batch_tuning t{};
+                std::istringstream v(line.substr(tab + 1));
+                if ((v >> t.cases_per_lane >> t.waves_per_block >> t.workers_per_cu >> t.chunk >>
+                     t.program_staging) &&
+                    galop_batch_set_tuning(b, &t) == GALOP_OK) {
+                    hit = true;
+                    continue;  // rewritten below only on a miss
+                }
+                continue;  // a bad line for this key is dropped and re-tuned
+            }
+            lines.push_back(line);
+        }
+    }
+    if (!hit) {
+        rc = autotune_search(b, d_words, d_offsets, n_programs, d_fitness);
+        if (rc != GALOP_OK) return rc;
+        std::ostringstream v;
+        v << key << '\t' << b->tuning.cases_per_lane << ' ' << b->tuning.waves_per_block << ' '
+          << b->tuning.workers_per_cu << ' ' << b->tuning.chunk << ' ' << b->tuning.program_staging;
+        lines.push_back(v.str());
+        // write a new file beside the old one, then rename it over the old
+        const std::string tmp = std::string(cache_path) + ".tmp";
+        {
+            std::ofstream o(tmp, std::ios::trunc);
+            for (const std::string& l : lines) o << l << '\n';
+            if (!o) return set_error(ctx, GALOP_ERR_INVALID,
+                                     "galop_batch_autotune_cached: cannot write " + tmp);
+        }
+        if (std::rename(tmp.c_str(), cache_path) != 0)
+            return set_error(ctx, GALOP_ERR_INVALID,
+                             std::string("galop_batch_autotune_cached: cannot replace ") + cache_path);
+    }
+    if (out) tuning_out(b->tuning, out);
+    if (cache_hit) *cache_hit = hit ? 1 : 0;
     return GALOP_OK;
 }
 

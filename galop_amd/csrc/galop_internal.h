@@ -87,6 +87,7 @@ struct BatchTuning {
     int waves_per_block = 0;
     int workers_per_cu = 0;   // persistent waves per CU (grid size)
     int chunk = 0;            // programs handed out per queue grab
+    int program_staging = 0;  // kStageVector or kStageScalar (lgp_device.h)
 };
 
 BatchTuning resolve_batch_tuning(const DevData& ds, int num_registers, BatchTuning t,

@@ -683,7 +683,7 @@ __device__ __forceinline__ int64_t readlane_i64(int64_t v, int j) {
 // by_position the CSR is indexed by order position (the compacted programs of
 // the intron-removing generation), otherwise by program id.  Every wave
 // leaves the loop once the head passes n, so the grid always drains.
-template <int NR, int K, int EXT>
+template <int NR, int K, int EXT, int STAGE>
 __global__ void __launch_bounds__(256)
 batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ offs,
                const int32_t* __restrict__ order, int32_t* __restrict__ queue_head,
@@ -719,7 +719,8 @@ batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ o
             const int64_t p = __builtin_amdgcn_readlane(p_l, j);
             const int64_t lo = readlane_i64(lo_l, j);
             const int64_t hi = readlane_i64(hi_l, j);
-            const double sse = eval_program_wave<NR, K, EXT>(words + lo, hi - lo, ds, lane, cr, nullptr);
+            const double sse =
+                eval_program_wave<NR, K, EXT, 0, STAGE>(words + lo, hi - lo, ds, lane, cr, nullptr);
             if (lane == 0) {
                 fitness[p] = fitness_of_sse(sse, ds);
                 if (sse_out) sse_out[p] = sse;
@@ -896,6 +897,7 @@ BatchTuning resolve_batch_tuning(const DevData& ds, int num_registers, BatchTuni
     if (t.workers_per_cu <= 0) t.workers_per_cu = 32;
     if (t.chunk <= 0) t.chunk = 4;
     if (t.chunk > kWave) t.chunk = kWave;
+    if (t.program_staging != kStageScalar) t.program_staging = kStageVector;
     return t;
 }
 
@@ -911,9 +913,14 @@ static hipError_t launch_batched_t(const DevData& ds, const BatchTuning& t, int 
     // leave most waves without work
     int64_t chunk = n / (waves * 4);
     chunk = chunk < 1 ? 1 : (chunk > t.chunk ? t.chunk : chunk);
-    hipLaunchKernelGGL((batched_kernel<NR, K, EXT>), dim3(grid), dim3(wpb * kWave), 0, stream,
-                       words, offs, s.order, s.queue_head, n, (int)chunk, eff_counts, cap, ds,
-                       fitness, sse);
+    if (t.program_staging == kStageScalar)
+        hipLaunchKernelGGL((batched_kernel<NR, K, EXT, kStageScalar>), dim3(grid), dim3(wpb * kWave),
+                           0, stream, words, offs, s.order, s.queue_head, n, (int)chunk,
+                           eff_counts, cap, ds, fitness, sse);
+    else
+        hipLaunchKernelGGL((batched_kernel<NR, K, EXT, kStageVector>), dim3(grid), dim3(wpb * kWave),
+                           0, stream, words, offs, s.order, s.queue_head, n, (int)chunk,
+                           eff_counts, cap, ds, fitness, sse);
     return hipGetLastError();
 }
 

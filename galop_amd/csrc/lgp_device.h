@@ -450,13 +450,34 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
 #undef GALOP_GROUP_FENCE
 }
 
-// Run `len` words starting at `prog` on the K register vectors.  The next
-// 64-word chunk is loaded while the current one executes.
-template <int NR, int K, int EXT>
+// How a program's words reach the scalar unit.
+//   kStageVector: one coalesced 64-word load per chunk into a VGPR (word l in
+//                 lane l), the next chunk loaded while the current one runs,
+//                 each word broadcast with v_readlane;
+//   kStageScalar: one scalar load per word through the constant cache, the
+//                 next word loaded while the current one runs (no VGPR, no
+//                 readlane; the program must not change during the kernel).
+constexpr int kStageVector = 0;
+constexpr int kStageScalar = 1;
+
+// Run `len` words starting at `prog` on the K register vectors.
+template <int NR, int K, int EXT, int STAGE = kStageVector>
 __device__ __forceinline__ void run_program(const uint32_t* __restrict__ prog,
                                             int64_t len, int lane,
                                             RegFile<NR, K>& r, ConstRegs cr,
                                             DestDecode dd) {
+    if constexpr (STAGE == kStageScalar) {
+        typedef const __attribute__((address_space(4))) uint32_t* ScalarWords;
+        const ScalarWords sp = (ScalarWords)prog;
+        if (len <= 0) return;
+        uint32_t next = sp[0];
+        for (int64_t j = 0; j < len; ++j) {
+            const uint32_t w = next;
+            if (j + 1 < len) next = sp[j + 1];
+            exec_word<NR, K, EXT>(w, r, cr, dd);
+        }
+        return;
+    }
     // the chunk base advances as a wave-uniform pointer and each lane adds
     // its 32-bit word offset, so no 64-bit per-lane address stays live
     const uint32_t* __restrict__ p = prog;
@@ -777,13 +798,13 @@ __device__ __forceinline__ double block_sse_run(const DevData& ds, int lane, int
     return wave_sum_f64(0.0);  // no tile: c0 >= c1
 }
 
-template <int NR, int K, int EXT>
+template <int NR, int K, int EXT, int STAGE = kStageVector>
 __device__ __forceinline__ double block_sse(const uint32_t* __restrict__ prog, int64_t len,
                                             const DevData& ds, int lane, ConstRegs cr,
                                             DestDecode dd, int64_t c0,
                                             float* __restrict__ case_out) {
     return block_sse_run<NR, K>(ds, lane, c0, case_out, [&](RegFile<NR, K>& r) {
-        run_program<NR, K, EXT>(prog, len, lane, r, cr, dd);
+        run_program<NR, K, EXT, STAGE>(prog, len, lane, r, cr, dd);
     });
 }
 
@@ -814,7 +835,7 @@ __device__ __forceinline__ double block_sse_eff(const uint32_t* __restrict__ pro
 // With EFF, a program of at most kSkipMaxWords words runs its effective words
 // only (same bits); longer ones run whole.  Either way there is one instance
 // of the interpreter step in the kernel.
-template <int NR, int K, int EXT, int EFF = 0>
+template <int NR, int K, int EXT, int EFF = 0, int STAGE = kStageVector>
 __device__ __forceinline__ double eval_program_wave(const uint32_t* __restrict__ prog,
                                                     int64_t len, const DevData& ds,
                                                     int lane, ConstRegs cr,
@@ -831,8 +852,8 @@ __device__ __forceinline__ double eval_program_wave(const uint32_t* __restrict__
         return sse;
     }
     for (int b = 0; b < case_blocks(ds.n_cases); ++b)
-        sse += block_sse<NR, K, EXT>(prog, len, ds, lane, cr, dd, (int64_t)b << kCaseBlockLog2,
-                                     case_out);
+        sse += block_sse<NR, K, EXT, STAGE>(prog, len, ds, lane, cr, dd,
+                                            (int64_t)b << kCaseBlockLog2, case_out);
     return sse;
 }
 

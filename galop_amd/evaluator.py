@@ -12,6 +12,7 @@ intron removal, batched plans) that the benchmarks and serving use.  Device
 buffers are torch tensors: torch is the allocator and stream plumbing only.
 """
 import ctypes
+import os
 
 import numpy as np
 
@@ -267,16 +268,41 @@ class BatchPlan:
                                                1 if use_graph else 0, stream),
               self.ds.ctx.handle)
 
-    def autotune(self, d_words, d_offsets, n, d_fitness):
-        k = ctypes.c_int32()
-        w = ctypes.c_int32()
-        c = ctypes.c_int32()
-        ch = ctypes.c_int32()
+    def autotune(self, d_words, d_offsets, n, d_fitness, cache=None):
+        """Search the launch tuning on a prefix of the population and keep it
+        in the plan; with `cache` (a file path), reuse a tuning found earlier
+        for the same device, dataset shape and population shape, or store the
+        one found now.  Returns the tuning as a dict (with "cache_hit" when a
+        cache is given)."""
+        if cache is not None:
+            t = _lib.galop_batch_tuning()
+            hit = ctypes.c_int32()
+            check(self.lib.galop_batch_autotune_cached(
+                self.handle, _dptr(d_words), _dptr(d_offsets), int(n), _dptr(d_fitness),
+                os.fsencode(str(cache)), ctypes.byref(t), ctypes.byref(hit)), self.ds.ctx.handle)
+            out = self.tuning()
+            out["cache_hit"] = bool(hit.value)
+            return out
         check(self.lib.galop_batch_autotune(self.handle, _dptr(d_words), _dptr(d_offsets), int(n),
-                                            _dptr(d_fitness), ctypes.byref(k), ctypes.byref(w),
-                                            ctypes.byref(c), ctypes.byref(ch)), self.ds.ctx.handle)
-        return {"cases_per_lane": k.value, "waves_per_block": w.value, "workers_per_cu": c.value,
-                "chunk": ch.value}
+                                            _dptr(d_fitness), None, None, None, None),
+              self.ds.ctx.handle)
+        return self.tuning()
+
+    def tuning(self):
+        t = _lib.galop_batch_tuning()
+        check(self.lib.galop_batch_get_tuning(self.handle, ctypes.byref(t)), self.ds.ctx.handle)
+        return {"cases_per_lane": t.cases_per_lane, "waves_per_block": t.waves_per_block,
+                "workers_per_cu": t.workers_per_cu, "chunk": t.chunk,
+                "program_staging": "scalar" if t.program_staging == _lib.STAGE_SCALAR
+                else "vector"}
+
+    def set_tuning(self, cases_per_lane, waves_per_block, workers_per_cu, chunk,
+                   program_staging="vector"):
+        t = _lib.galop_batch_tuning(int(cases_per_lane), int(waves_per_block),
+                                    int(workers_per_cu), int(chunk),
+                                    {"vector": _lib.STAGE_VECTOR,
+                                     "scalar": _lib.STAGE_SCALAR}[program_staging], 0)
+        check(self.lib.galop_batch_set_tuning(self.handle, ctypes.byref(t)), self.ds.ctx.handle)
 
     def close(self):
         if self.handle:

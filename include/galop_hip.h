@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define GALOP_ABI_VERSION 4
+#define GALOP_ABI_VERSION 5
 
 /* ---- status codes ---- */
 #define GALOP_OK            0
@@ -228,14 +228,45 @@ int  galop_batch_eval_device(galop_batch* b, const uint32_t* d_words,
                              const int64_t* d_offsets, int64_t n_programs,
                              float* d_fitness, double* d_sse, int32_t use_graph,
                              void* stream);
+/* How the batched kernel's waves read a program's words: one coalesced
+ * 64-word load per chunk into a vector register, each word broadcast to the
+ * scalar unit (VECTOR), or one scalar load per word (SCALAR). */
+#define GALOP_STAGE_VECTOR 0
+#define GALOP_STAGE_SCALAR 1
+typedef struct galop_batch_tuning {
+    int32_t cases_per_lane;   /* 1, 2, 4, or 8 (8 registers only)           */
+    int32_t waves_per_block;  /* 1..4                                        */
+    int32_t workers_per_cu;   /* persistent waves per compute unit, 1..64    */
+    int32_t chunk;            /* programs per work-queue grab, 1..64         */
+    int32_t program_staging;  /* GALOP_STAGE_*                               */
+    int32_t reserved;         /* 0                                           */
+} galop_batch_tuning;
 /* Autotune (cases per lane x waves per block x persistent waves per CU x
- * programs per queue grab) on a prefix of the given population; the winner is
- * kept in the plan.  Writes the chosen values (any output may be NULL). */
+ * programs per queue grab, then the program staging at the best of those)
+ * on a prefix of the given population; the winner is kept in the plan.
+ * Writes the chosen values (any output may be NULL; galop_batch_get_tuning
+ * gives the staging too).  No counterpart in the reference, whose launch
+ * shape is fixed (src/EvaluatorCuda.cu:424-450). */
 int  galop_batch_autotune(galop_batch* b, const uint32_t* d_words,
                           const int64_t* d_offsets, int64_t n_programs,
                           float* d_fitness, int32_t* cases_per_lane,
                           int32_t* waves_per_block, int32_t* workers_per_cu,
                           int32_t* chunk);
+int  galop_batch_get_tuning(const galop_batch* b, galop_batch_tuning* out);
+/* GALOP_ERR_INVALID for a value out of range. */
+int  galop_batch_set_tuning(galop_batch* b, const galop_batch_tuning* t);
+/* galop_batch_autotune with a tuning cache that outlives the process:
+ * cache_path is a text file of "key<TAB>tuning" lines, the key naming the
+ * device (architecture, compute units), the dataset's shape (registers, op
+ * table, input protection, fitness, cases, inputs), the plan's intron
+ * removal and the population's program count and mean length to the
+ * nearest power of two.  A hit sets the cached tuning without a search
+ * (*cache_hit = 1); a miss searches and writes the file anew (a temporary
+ * file renamed over it).  Blocking. */
+int  galop_batch_autotune_cached(galop_batch* b, const uint32_t* d_words,
+                                 const int64_t* d_offsets, int64_t n_programs,
+                                 float* d_fitness, const char* cache_path,
+                                 galop_batch_tuning* out, int32_t* cache_hit);
 
 /* ---- serving: many evolution runs, continuous batching -----------------
  * A server runs up to n_streams rounds at once, one per HIP stream.  When a

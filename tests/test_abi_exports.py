@@ -40,7 +40,7 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_without_device():
     from galop_amd import _lib
-    assert _lib.load().galop_abi_version() == 4
+    assert _lib.load().galop_abi_version() == 5
 
 
 def test_null_arguments_rejected_without_device():

@@ -107,6 +107,73 @@ def test_graph_replays_match_eager(gpu_ctx):
     plan.close()
 
 
+@pytest.mark.parametrize("strip", [False, True])
+def test_program_staging_every_tile(gpu_ctx, strip):
+    """Both program stagings (a 64-word vector load per chunk with a
+    broadcast per word, or one scalar load per word) at every cases-per-lane
+    tile and block shape give bitwise the plain kernel's fitness and SSE,
+    also replayed from the graph; out-of-range tunings are refused."""
+    from galop_amd import BatchPlan, GalopError
+    rng = np.random.default_rng(45)
+    isa, dds, words, offsets = _setup(gpu_ctx, rng, P=4000, lo=0, hi=200)
+    n = offsets.size - 1
+    ref_f, ref_s = gpu_fitness(dds, words, offsets)
+    plan = BatchPlan(dds, n, max_words=words.size, strip_introns=strip)
+    dw, do = dev_words(words), dev_i64(offsets)
+    fit = torch.empty(n, dtype=torch.float32, device="cuda")
+    sse = torch.empty(n, dtype=torch.float64, device="cuda")
+    for staging in ("vector", "scalar"):
+        for k, w in ((1, 1), (2, 2), (4, 4), (8, 1)):
+            plan.set_tuning(k, w, 16, 4, staging)
+            assert plan.tuning()["program_staging"] == staging
+            for use_graph in (False, True):
+                fit.fill_(float("nan"))
+                plan.eval(dw, do, n, fit, sse, use_graph=use_graph, stream=stream())
+                torch.cuda.synchronize()
+                assert same_bits(fit.cpu().numpy(), ref_f).all(), (staging, k, use_graph)
+                assert _same64(sse.cpu().numpy(), ref_s), (staging, k, use_graph)
+    for bad in ((3, 1, 16, 4), (4, 5, 16, 4), (4, 1, 0, 4), (4, 1, 16, 65)):
+        with pytest.raises(GalopError):
+            plan.set_tuning(*bad)
+    plan.close()
+
+
+def test_tuning_cache(gpu_ctx, tmp_path):
+    """galop_batch_autotune_cached: the first call searches and writes one
+    line for the workload; a second plan on the same shapes takes the tuning
+    from the file without searching; another dataset shape adds a line; the
+    results are the plain kernel's bits whatever the tuning."""
+    from galop_amd import BatchPlan
+    rng = np.random.default_rng(46)
+    isa, dds, words, offsets = _setup(gpu_ctx, rng, P=20000, hi=120)
+    n = offsets.size - 1
+    ref_f, _ = gpu_fitness(dds, words, offsets)
+    dw, do = dev_words(words), dev_i64(offsets)
+    fit = torch.empty(n, dtype=torch.float32, device="cuda")
+    cache = tmp_path / "tune.txt"
+    plan = BatchPlan(dds, n)
+    first = plan.autotune(dw, do, n, fit, cache=cache)
+    assert not first.pop("cache_hit")
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1 and "/r8/" in lines[0] and "/n256/" in lines[0], lines
+    plan.close()
+    plan = BatchPlan(dds, n)
+    second = plan.autotune(dw, do, n, fit, cache=cache)
+    assert second.pop("cache_hit") and second == first
+    plan.eval(dw, do, n, fit, use_graph=True, stream=stream())
+    torch.cuda.synchronize()
+    assert same_bits(fit.cpu().numpy(), ref_f).all()
+    plan.close()
+    _, dds2, words2, offsets2 = _setup(gpu_ctx, rng, N=1000, P=3000, hi=120)
+    n2 = offsets2.size - 1
+    plan = BatchPlan(dds2, n2)
+    fit2 = torch.empty(n2, dtype=torch.float32, device="cuda")
+    third = plan.autotune(dev_words(words2), dev_i64(offsets2), n2, fit2, cache=cache)
+    assert not third["cache_hit"]
+    assert len(cache.read_text().splitlines()) == 2
+    plan.close()
+
+
 def test_autotune_keeps_results(gpu_ctx):
     from galop_amd import BatchPlan
     rng = np.random.default_rng(43)
