@@ -296,34 +296,39 @@ __device__ __forceinline__ bool effective_step(uint32_t w, const DecodeCfg& c, u
 }
 
 // Calls f(word) for the words of [lo, hi), last to first, reading the
-// 16-byte aligned blocks that hold them (one load per four words), the next
-// block's load issued before the current block is processed.  A block never
-// crosses a page, so its other words are readable whenever the one in range
-// is; they are not used.
+// 128-byte aligned segments that hold them, one whole segment (eight 16-byte
+// loads) per step.  The lanes of a wave walk 64 programs at scattered
+// addresses; taking a cache line's worth per step fetches each line once,
+// where a 16-byte piece per step held every line across eight steps, more
+// lines than the caches keep, so lines were fetched again.  A segment never
+// crosses a page, so its other words are readable whenever one in range is;
+// they are not used.
 template <typename F>
 __device__ __forceinline__ void walk_backward(const uint32_t* __restrict__ words, int64_t lo,
                                               int64_t hi, F f) {
     if (hi <= lo) return;
+    constexpr int kSeg = 32;  // words per segment
     const uintptr_t last = (uintptr_t)(words + (hi - 1));
-    uintptr_t a = last & ~(uintptr_t)15;
-    int64_t blk = (hi - 1) - (int64_t)((last - a) >> 2);  // word index of the block's first word
-    uint4 v = *reinterpret_cast<const uint4*>(a);
-    int64_t j = hi;
+    uintptr_t a = last & ~(uintptr_t)(4 * kSeg - 1);
+    int64_t seg = (hi - 1) - (int64_t)((last - a) >> 2);  // word index of the segment's first word
+    int top = (int)(hi - seg);                            // words below hi in this segment, 1..32
     for (;;) {
-        const bool more = blk > lo;  // word blk-1 is in range: its block is readable
-        uint4 vn = v;
-        if (more) vn = *reinterpret_cast<const uint4*>(a - 16);
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+        const int bot = seg >= lo ? 0 : (int)(lo - seg);  // words below lo in this segment
+        // global loads (a flat load would also wait on the LDS counter)
+        typedef uint32_t U32x4 __attribute__((ext_vector_type(4)));
+        typedef const __attribute__((address_space(1))) U32x4* GlobalBlocks;
+        U32x4 v[kSeg / 4];
 #pragma unroll
-        for (int t = 3; t >= 0; --t) {
-            const int64_t idx = blk + t;
-            if (idx < j && idx >= lo) f(w[t]);
+        for (int q = 0; q < kSeg / 4; ++q) v[q] = reinterpret_cast<GlobalBlocks>(a)[q];
+#pragma unroll
+        for (int t = kSeg - 1; t >= 0; --t) {
+            const uint32_t w = v[t / 4][t & 3];
+            if (t < top && t >= bot) f(w);
         }
-        if (!more) break;
-        j = blk;
-        blk -= 4;
-        a -= 16;
-        v = vn;
+        if (seg <= lo) break;
+        seg -= kSeg;
+        a -= 4 * kSeg;
+        top = kSeg;
     }
 }
 
@@ -341,11 +346,54 @@ __global__ void effective_count_kernel(DecodeCfg cfg, const uint32_t* __restrict
     counts[i] = cnt;
 }
 
-// The compacted words go out through a 16-byte staging block: blocks that lie
-// wholly inside this program's range [start, end) are written with one
-// 16-byte store, the partial blocks at either end word by word (their other
-// words belong to neighbouring programs).  A caller's buffer that is not
-// 16-byte aligned is written word by word throughout.
+// Writes a program's effective words from the back of its output range
+// [end - count, end): the walk finds them last to first, and the staging block
+// holds the last four written, s_j at position end - cnt + j.  When a word
+// lands at a position divisible by four, the four positions from it up are
+// complete and go out with one 16-byte store (word by word where the block
+// reaches `lim`, the end of what may be written, or the buffer is not 16-byte
+// aligned); after the walk, finish() writes the partial block at the front.
+// Only positions of this program's own effective words below `lim` are ever
+// written.
+struct BackWriter {
+    uint32_t* __restrict__ out;
+    int64_t end, lim;
+    bool vec_ok;
+    int cnt = 0;  // words written so far (a program has fewer than 2^31 words)
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+
+    __device__ BackWriter(uint32_t* o, int64_t e, int64_t l, bool v)
+        : out(o), end(e), lim(l), vec_ok(v) {}
+    // positions pos .. pos + n - 1 from s0 .. s(n-1), none at or past lim
+    __device__ __forceinline__ void store(int64_t pos, int n) {
+        if (vec_ok && n == 4 && pos + 4 <= lim) {
+            *reinterpret_cast<uint4*>(out + pos) = make_uint4(s0, s1, s2, s3);
+            return;
+        }
+        if (n > 0 && pos < lim) out[pos] = s0;
+        if (n > 1 && pos + 1 < lim) out[pos + 1] = s1;
+        if (n > 2 && pos + 2 < lim) out[pos + 2] = s2;
+        if (n > 3 && pos + 3 < lim) out[pos + 3] = s3;
+    }
+    __device__ __forceinline__ void push(uint32_t w) {
+        s3 = s2;
+        s2 = s1;
+        s1 = s0;
+        s0 = w;
+        ++cnt;
+        // s_j for j >= cnt are not this program's: their positions are at or
+        // past end, and store() writes none of them
+        if ((((uint32_t)end - (uint32_t)cnt) & 3u) == 0u) store(end - cnt, 4);
+    }
+    __device__ __forceinline__ void finish() {
+        const int64_t pos = end - cnt;
+        const int r = (int)(pos & 3);
+        if (cnt > 0 && r != 0) store(pos, min(4 - r, cnt));
+    }
+};
+
+// The compacted program i goes to [eff_offsets[i], eff_offsets[i + 1]) of
+// eff_words, written from the back (BackWriter).
 __global__ void effective_scatter_kernel(DecodeCfg cfg, const uint32_t* __restrict__ words,
                                          const int64_t* __restrict__ offsets,
                                          const int32_t* __restrict__ order, int64_t n,
@@ -354,40 +402,14 @@ __global__ void effective_scatter_kernel(DecodeCfg cfg, const uint32_t* __restri
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t p = order ? (int64_t)order[i] : i;
-    const int64_t start = eff_offsets[i];
     const int64_t end = eff_offsets[i + 1];
-    const int64_t lim = end < cap ? end : cap;  // never write past the buffer
-    int64_t pos = end;                          // fill from the back
-    int64_t sblk = -1;                          // staged block (first word index)
-    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    const bool vec_ok = ((uintptr_t)eff_words & 15) == 0;
-    auto flush = [&]() {
-        if (sblk < 0) return;
-        if (vec_ok && sblk >= start && sblk + 4 <= lim) {
-            *reinterpret_cast<uint4*>(eff_words + sblk) = make_uint4(s0, s1, s2, s3);
-        } else {
-            const uint32_t sv[4] = {s0, s1, s2, s3};
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
-                if (sblk + t >= start && sblk + t < lim) eff_words[sblk + t] = sv[t];
-        }
-    };
+    BackWriter out(eff_words, end, end < cap ? end : cap,  // never write past the buffer
+                   ((uintptr_t)eff_words & 15) == 0);
     uint32_t live = 1u << cfg.out_reg;
     walk_backward(words, offsets[p], offsets[p + 1], [&](uint32_t w) {
-        if (!effective_step(w, cfg, live)) return;
-        --pos;
-        const int64_t b = pos & ~(int64_t)3;
-        if (b != sblk) {
-            flush();
-            sblk = b;
-        }
-        const int o = (int)(pos & 3);
-        s0 = o == 0 ? w : s0;
-        s1 = o == 1 ? w : s1;
-        s2 = o == 2 ? w : s2;
-        s3 = o == 3 ? w : s3;
+        if (effective_step(w, cfg, live)) out.push(w);
     });
-    flush();
+    out.finish();
 }
 
 // Intron removal for the batched evaluation in one pass, without a prefix
@@ -395,11 +417,9 @@ __global__ void effective_scatter_kernel(DecodeCfg cfg, const uint32_t* __restri
 // words backwards from the end of p's own raw slot, in a buffer laid out like
 // the raw CSR (positions relative to offsets[0]).  A program has no more
 // effective words than raw ones, so they stay inside its slot: its effective
-// code is slot[end_p - counts[i], end_p).  Writes go through the same 16-byte
-// staging as effective_scatter_kernel; a whole block is stored when it lies
-// inside [the slot's start, min(end_p, cap)), so the words below the
-// effective code that it may cover are this slot's own, and no word at or
-// past `cap` is written.
+// code is slot[end_p - counts[i], end_p).  Writes go through BackWriter, as
+// in effective_scatter_kernel: only the positions of p's effective words below
+// min(end_p, cap) are written.
 __global__ void effective_in_slot_kernel(DecodeCfg cfg, const uint32_t* __restrict__ words,
                                          const int64_t* __restrict__ offsets,
                                          const int32_t* __restrict__ order, int64_t n,
@@ -408,41 +428,14 @@ __global__ void effective_in_slot_kernel(DecodeCfg cfg, const uint32_t* __restri
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const int64_t p = order[i];
-    const int64_t base = offsets[0];
-    const int64_t start = offsets[p] - base;
-    const int64_t end = offsets[p + 1] - base;
-    const int64_t lim = end < cap ? end : cap;
-    int64_t pos = end;
-    int64_t sblk = -1;
-    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    auto flush = [&]() {
-        if (sblk < 0) return;
-        if (sblk >= start && sblk + 4 <= lim) {
-            *reinterpret_cast<uint4*>(slot + sblk) = make_uint4(s0, s1, s2, s3);
-        } else {
-            const uint32_t sv[4] = {s0, s1, s2, s3};
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
-                if (sblk + t >= start && sblk + t >= pos && sblk + t < lim) slot[sblk + t] = sv[t];
-        }
-    };
+    const int64_t end = offsets[p + 1] - offsets[0];
+    BackWriter out(slot, end, end < cap ? end : cap, ((uintptr_t)slot & 15) == 0);
     uint32_t live = 1u << cfg.out_reg;
     walk_backward(words, offsets[p], offsets[p + 1], [&](uint32_t w) {
-        if (!effective_step(w, cfg, live)) return;
-        --pos;
-        const int64_t b = pos & ~(int64_t)3;
-        if (b != sblk) {
-            flush();
-            sblk = b;
-        }
-        const int o = (int)(pos & 3);
-        s0 = o == 0 ? w : s0;
-        s1 = o == 1 ? w : s1;
-        s2 = o == 2 ? w : s2;
-        s3 = o == 3 ? w : s3;
+        if (effective_step(w, cfg, live)) out.push(w);
     });
-    flush();
-    counts[i] = end - pos;
+    out.finish();
+    counts[i] = out.cnt;
 }
 
 // Offsets past the effective-word buffer are cut to its end, so that a

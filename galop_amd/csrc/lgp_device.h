@@ -217,6 +217,45 @@ __device__ __forceinline__ DestDecode dest_decode(int nr, int iprot) {
     return dd;
 }
 
+// ---- table rows by index --------------------------------------------------
+// The sin/cos and log tables hold 16-byte rows (two doubles).  A row is read
+// with a buffer load in index mode: the hardware forms base + index * 16 from
+// the row index the reduction produces, so no per-case address arithmetic is
+// issued (a global load needs the index shifted, and log's row offset by 91
+// and masked).  The descriptor is four SGPRs: the base address, a 16-byte
+// stride, the number of rows (index-mode loads past it return zeros) and the
+// gfx9 data-format word.  Every index used is in range by construction.
+typedef int RowRsrc __attribute__((ext_vector_type(4)));
+typedef uint32_t RowBits __attribute__((ext_vector_type(4)));
+__device__ RowBits galop_row_load(RowRsrc rsrc, int vindex, int voffset, int soffset,
+                                  int aux) __asm("llvm.amdgcn.struct.buffer.load.v4i32");
+
+__device__ __forceinline__ RowRsrc row_rsrc(const double* rows, uint32_t n_rows) {
+    const uint64_t a = (uint64_t)(uintptr_t)rows;
+    RowRsrc r;
+    r.x = (int)(uint32_t)a;
+    r.y = (int)(((uint32_t)(a >> 32) & 0xffffu) | (16u << 16));
+    r.z = (int)n_rows;
+    r.w = 0x00020000;
+    return r;
+}
+// Row `index` of the table described by `rsrc`: (first, second) double.
+__device__ __forceinline__ void load_row(RowRsrc rsrc, uint32_t index, double& first,
+                                         double& second) {
+    const RowBits v = galop_row_load(rsrc, (int)index, 0, 0, 0);
+    first = __builtin_bit_cast(double, ((uint64_t)v.y << 32) | v.x);
+    second = __builtin_bit_cast(double, ((uint64_t)v.w << 32) | v.z);
+}
+// (sin, cos)(j pi/128) for sin (row j) or cos (row j + 64), j in 0..255
+__device__ __forceinline__ RowRsrc sincos_rows(int want_cos) {
+    return row_rsrc(galop_math::SINCOS_T + 128 * want_cos, 256);
+}
+// (c_j, ln(1/c_j)) by log_reduce's row91 in 91..181: the descriptor's base is
+// 91 rows before the table
+__device__ __forceinline__ RowRsrc log_rows() {
+    return row_rsrc(galop_math::LOG_T - 2 * 91, 182);
+}
+
 // Ends a group of table-row cases: the scheduler moves no instruction across
 // it, so one group's loads and arithmetic are not spread into the next.
 #define GALOP_GROUP_FENCE() __builtin_amdgcn_sched_barrier(0)
@@ -324,13 +363,13 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
             } else {                                                      \
                 /* the table rows of kRowGroup cases are loaded before */ \
                 /* any of them is used */                                 \
+                const RowRsrc rows = sincos_rows(WANT_COS);               \
                 _Pragma("unroll") for (int g = 0; g < K; g += kRowGroup) {  \
                     double t[kRowGroup], ts[kRowGroup], tc[kRowGroup];    \
                     _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) { \
                         uint32_t j;                                       \
                         t[i] = galop_math::sincos_reduce(bv[g + i], j, C); \
-                        ts[i] = galop_math::SINCOS_T[2 * j + 128 * (WANT_COS)]; \
-                        tc[i] = galop_math::SINCOS_T[2 * j + 1 + 128 * (WANT_COS)]; \
+                        load_row(rows, j, ts[i], tc[i]);                  \
                     }                                                     \
                     _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) \
                         cv[g + i] = galop_math::sincos_finish(bv[g + i], WANT_COS, t[i], \
@@ -374,6 +413,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                 _Pragma("unroll") for (int k = 0; k < K; ++k)             \
                     cv[k] = galop_math::plogf_special_value(bv[k]);       \
             } else {                                                      \
+                const RowRsrc rows = log_rows();                          \
                 _Pragma("unroll") for (int g = 0; g < K; g += kRowGroup) {  \
                     double y[kRowGroup], tcj[kRowGroup], tln[kRowGroup];  \
                     int32_t e[kRowGroup];                                 \
@@ -381,11 +421,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                         uint32_t row91;                                   \
                         y[i] = galop_math::log_reduce(__builtin_fabsf(bv[g + i]), e[i], \
                                                       row91, C);          \
-                        /* row91 - 91 is in 0..90: the mask only tells */ \
-                        /* the compiler so */                              \
-                        const uint32_t row = (row91 - 91u) & 127u;        \
-                        tcj[i] = galop_math::LOG_T[2 * row];              \
-                        tln[i] = galop_math::LOG_T[2 * row + 1];          \
+                        load_row(rows, row91, tcj[i], tln[i]);            \
                     }                                                     \
                     _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) { \
                         const float b = bv[g + i];                        \
