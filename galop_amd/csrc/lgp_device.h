@@ -275,6 +275,12 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
     // lane (program bounds and sums, written to HBM for every program), and
     // two measure the same with none.
     constexpr int kRowGroup = K >= 8 ? 2 : (K < 4 ? K : 4);
+    // Table rows by index-mode buffer loads (load_row) at 4 or more cases per
+    // lane, where the per-case address arithmetic they save is worth the
+    // scalar instructions that form the descriptor for each word; with fewer
+    // cases per lane the kernels are bound by scalar work (the engine's
+    // 32-case evaluation took 8% longer with them) and use global loads.
+    constexpr bool kIndexRows = K >= 4;
 
 // A constant operand stays one scalar value for all K cases: the compute is
 // written out in both branches, so it is never copied into K vector values.
@@ -363,13 +369,18 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
             } else {                                                      \
                 /* the table rows of kRowGroup cases are loaded before */ \
                 /* any of them is used */                                 \
-                const RowRsrc rows = sincos_rows(WANT_COS);               \
+                [[maybe_unused]] const RowRsrc rows = sincos_rows(WANT_COS); \
                 _Pragma("unroll") for (int g = 0; g < K; g += kRowGroup) {  \
                     double t[kRowGroup], ts[kRowGroup], tc[kRowGroup];    \
                     _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) { \
                         uint32_t j;                                       \
                         t[i] = galop_math::sincos_reduce(bv[g + i], j, C); \
-                        load_row(rows, j, ts[i], tc[i]);                  \
+                        if constexpr (kIndexRows) {                       \
+                            load_row(rows, j, ts[i], tc[i]);              \
+                        } else {                                          \
+                            ts[i] = galop_math::SINCOS_T[2 * j + 128 * (WANT_COS)]; \
+                            tc[i] = galop_math::SINCOS_T[2 * j + 1 + 128 * (WANT_COS)]; \
+                        }                                                 \
                     }                                                     \
                     _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) \
                         cv[g + i] = galop_math::sincos_finish(bv[g + i], WANT_COS, t[i], \
@@ -413,7 +424,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                 _Pragma("unroll") for (int k = 0; k < K; ++k)             \
                     cv[k] = galop_math::plogf_special_value(bv[k]);       \
             } else {                                                      \
-                const RowRsrc rows = log_rows();                          \
+                [[maybe_unused]] const RowRsrc rows = log_rows();         \
                 _Pragma("unroll") for (int g = 0; g < K; g += kRowGroup) {  \
                     double y[kRowGroup], tcj[kRowGroup], tln[kRowGroup];  \
                     int32_t e[kRowGroup];                                 \
@@ -421,7 +432,15 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                         uint32_t row91;                                   \
                         y[i] = galop_math::log_reduce(__builtin_fabsf(bv[g + i]), e[i], \
                                                       row91, C);          \
-                        load_row(rows, row91, tcj[i], tln[i]);            \
+                        if constexpr (kIndexRows) {                       \
+                            load_row(rows, row91, tcj[i], tln[i]);        \
+                        } else {                                          \
+                            /* row91 - 91 is in 0..90: the mask only */   \
+                            /* tells the compiler so */                   \
+                            const uint32_t row = (row91 - 91u) & 127u;    \
+                            tcj[i] = galop_math::LOG_T[2 * row];          \
+                            tln[i] = galop_math::LOG_T[2 * row + 1];      \
+                        }                                                 \
                     }                                                     \
                     _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) { \
                         const float b = bv[g + i];                        \
