@@ -35,14 +35,14 @@ SIMDS_PER_CU = 4               # one wave64 VALU instruction per SIMD per 4 cloc
 # programs, seed 0), measured with rocprofv3 --pmc; the commands and raw
 # figures are in profiles/plain_kernel_plain_workload_counters.md.  FETCH_SIZE
 # and WRITE_SIZE are scaled by what each reads for a 1 GiB device copy
-# (1,048,576 KiB copied: FETCH_SIZE 524,308.5 KiB, WRITE_SIZE 1,048,576 KiB).
+# (1,048,576 KiB copied: FETCH_SIZE 524,308.6 KiB, WRITE_SIZE 1,048,576 KiB).
 # They describe that exact population only.
 PLAIN_WORKLOAD_COUNTERS = {
-    "fetch_bytes": 319511.4 * 1024 * (1048576 / 524308.5),
+    "fetch_bytes": 319481.4 * 1024 * (1048576 / 524308.6),
     "write_bytes": 16384.0 * 1024 * 1.0,
-    "valu_insts": 21046997013,
-    "salu_insts": 7139313212,
-    "valu_busy_pct": 83.0,
+    "valu_insts": 20421542997,
+    "salu_insts": 7011908528,
+    "valu_busy_pct": 82.7,
 }
 
 
