@@ -818,7 +818,7 @@ __device__ __forceinline__ int case_blocks(int64_t n_cases) {
 // c0 < n_cases; valid in lane 0.  When case_out is non-null the per-case outputs are written there
 // instead of being accumulated.
 // `run(r)` runs the program on one tile's registers.
-template <int NR, int K, typename Run>
+template <int NR, int K, bool kTargetsFirst, typename Run>
 __device__ __forceinline__ double block_sse_run(const DevData& ds, int lane, int64_t c0,
                                                 float* __restrict__ case_out, Run run) {
     constexpr int tile_cases = kWave * K;
@@ -836,7 +836,18 @@ __device__ __forceinline__ double block_sse_run(const DevData& ds, int lane, int
         RegFile<NR, K> r;
         load_cases<NR, K>(ds, first, lane, r);
         run(r);
+        // With kTargetsFirst the K targets are loaded together before any is
+        // used (the accumulation branches per case, and loads inside it are
+        // waited for one at a time); every tile lies inside the dataset's
+        // zero-padded n_pad cases, so these loads are in bounds.  The
+        // effective-word kernels keep the loads in the accumulation: their
+        // chunk registers leave no room for K more live values at 5 waves.
         const float* __restrict__ tgt = ds.targets + first;
+        float y[K];
+        if constexpr (kTargetsFirst) {
+#pragma unroll
+            for (int k = 0; k < K; ++k) y[k] = tgt[k * kWave + lane];
+        }
         LaneAcc acc;
         if (f != 0) acc.load(slot);
 #pragma unroll
@@ -844,7 +855,7 @@ __device__ __forceinline__ double block_sse_run(const DevData& ds, int lane, int
             if (f + k * kWave + lane < nb) {
                 const float out = r.get(k, out_reg);
                 if (case_out) case_out[first + k * kWave + lane] = out;
-                else acc.add(out, tgt[k * kWave + lane]);
+                else acc.add(out, kTargetsFirst ? y[k] : tgt[k * kWave + lane]);
             }
         }
         if (f + tile_cases >= nb) return wave_sum_f64(acc.total());
@@ -858,7 +869,10 @@ __device__ __forceinline__ double block_sse(const uint32_t* __restrict__ prog, i
                                             const DevData& ds, int lane, ConstRegs cr,
                                             DestDecode dd, int64_t c0,
                                             float* __restrict__ case_out) {
-    return block_sse_run<NR, K>(ds, lane, c0, case_out, [&](RegFile<NR, K>& r) {
+    // targets loaded first at up to 4 cases per lane (short tiles, where their
+    // waits were a measurable share: uneven workload -0.7%); at 8 the gain
+    // was within the noise
+    return block_sse_run<NR, K, (K <= 4)>(ds, lane, c0, case_out, [&](RegFile<NR, K>& r) {
         run_program<NR, K, EXT, STAGE>(prog, len, lane, r, cr, dd);
     });
 }
@@ -872,7 +886,7 @@ __device__ __forceinline__ double block_sse_eff(const uint32_t* __restrict__ pro
                                                 EffProgram& e, const DevData& ds, int lane,
                                                 ConstRegs cr, DestDecode dd, int64_t c0,
                                                 float* __restrict__ case_out) {
-    return block_sse_run<NR, K>(ds, lane, c0, case_out, [&](RegFile<NR, K>& r) {
+    return block_sse_run<NR, K, false>(ds, lane, c0, case_out, [&](RegFile<NR, K>& r) {
         for (int64_t w0 = 0; w0 < len; w0 += kSkipMaxWords) {
             if (len > kSkipMaxWords) {
                 const int64_t left = len - w0;
