@@ -127,12 +127,12 @@ int galop_ctx_create(int32_t device, galop_ctx** out) {
 
 void galop_ctx_destroy(galop_ctx* ctx) {
     if (!ctx) return;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     ctx->staging.release();
     for (auto& ev : ctx->ev)
-        if (ev) hipEventDestroy(ev);
-    hipStreamDestroy(ctx->stream);
+        if (ev) (void)hipEventDestroy(ev);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -285,7 +285,7 @@ int galop_dataset_create(galop_ctx* ctx, const galop_isa* isa, const float* inpu
 
 void galop_dataset_destroy(galop_dataset* ds) {
     if (!ds) return;
-    hipSetDevice(ds->ctx->device);
+    (void)hipSetDevice(ds->ctx->device);
     ds->inputs.release();
     ds->targets.release();
     ds->consts.release();

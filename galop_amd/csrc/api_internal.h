@@ -33,8 +33,9 @@ struct DeviceBuf {
         if (n <= cap && ptr) return hipSuccess;
         return alloc(n + n / 4);
     }
+    // errors on release have no caller to go to and nothing to undo
     void release() {
-        if (ptr) hipFree(ptr);
+        if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
         cap = 0;
     }
@@ -59,7 +60,7 @@ struct PinnedBuf {
         return hipSuccess;
     }
     void release() {
-        if (ptr) hipHostFree(ptr);
+        if (ptr) (void)hipHostFree(ptr);
         ptr = nullptr;
         cap = 0;
     }

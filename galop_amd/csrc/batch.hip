@@ -43,8 +43,8 @@ struct galop_batch {
     int64_t key_n = -1;
     hipStream_t key_stream = nullptr;
     void drop_graph() {
-        if (exec) hipGraphExecDestroy(exec);
-        if (graph) hipGraphDestroy(graph);
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
         exec = nullptr;
         graph = nullptr;
         key_n = -1;
@@ -104,7 +104,7 @@ int galop_batch_create(galop_ctx* ctx, const galop_dataset* ds, int64_t max_prog
 
 void galop_batch_destroy(galop_batch* b) {
     if (!b) return;
-    hipSetDevice(b->ctx->device);
+    (void)hipSetDevice(b->ctx->device);
     b->drop_graph();
     b->bufs.release();
     b->counts.release();
@@ -163,7 +163,7 @@ int galop_batch_eval_device(galop_batch* b, const uint32_t* d_words, const int64
         hipGraph_t g = nullptr;
         e = hipStreamEndCapture(s, &g);
         if (le != hipSuccess) {
-            if (g) hipGraphDestroy(g);
+            if (g) (void)hipGraphDestroy(g);
             return hip_fail(ctx, le, "launch_batched (capture)");
         }
         if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamEndCapture");
@@ -230,7 +230,7 @@ static int autotune_search(galop_batch* b, const uint32_t* d_words, const int64_
     hipError_t e = hipEventCreate(&a);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipEventCreate");
     if ((e = hipEventCreate(&z)) != hipSuccess) {
-        hipEventDestroy(a);
+        (void)hipEventDestroy(a);
         return hip_fail(ctx, e, "hipEventCreate");
     }
     float best = 1e30f;
@@ -263,8 +263,8 @@ static int autotune_search(galop_batch* b, const uint32_t* d_words, const int64_
             best_t = t;
         }
     }
-    hipEventDestroy(a);
-    hipEventDestroy(z);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(z);
     if (e != hipSuccess) return hip_fail(ctx, e, "galop_batch_autotune");
     b->tuning = best_t;
     b->drop_graph();

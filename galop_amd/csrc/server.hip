@@ -276,7 +276,7 @@ bool galop_server::start_round(Slot& s) {
     if (e == hipSuccess) e = hipEventRecord(s.ev_end, st);
     if (e != hipSuccess) {
         hip_fail(ctx, e, "server: round launch");
-        hipStreamSynchronize(st);  // nothing of the round may still run when it is torn down
+        (void)hipStreamSynchronize(st);  // nothing of the round may still run when it is torn down
         return false;
     }
     return true;
@@ -311,8 +311,19 @@ void galop_server::finish_round(Slot& s, hipError_t status) {
 }
 
 void galop_server::run() {
-    hipSetDevice(ctx->device);
+    const hipError_t dev_err = hipSetDevice(ctx->device);
     std::unique_lock<std::mutex> lk(mu);
+    if (dev_err != hipSuccess) {
+        // no round can run on this thread: every request, queued now or
+        // later, completes with the error
+        hip_fail(ctx, dev_err, "server: hipSetDevice in the scheduler thread");
+        while (true) {
+            for (Request* r : queue) complete(r, GALOP_ERR_HIP);
+            queue.clear();
+            if (stop) return;
+            cv_work.wait_for(lk, std::chrono::milliseconds(20));
+        }
+    }
     while (true) {
         bool progressed = false;
         // completed rounds
@@ -461,18 +472,19 @@ void galop_server_destroy(galop_server* s) {
     s->cv_work.notify_all();
     s->cv_done.notify_all();
     if (s->worker.joinable()) s->worker.join();
-    hipSetDevice(s->ctx->device);
+    // teardown: errors here have no caller to go to and nothing to undo
+    (void)hipSetDevice(s->ctx->device);
     for (auto& sl : s->slots) {
-        if (sl.stream) hipStreamSynchronize(sl.stream);
+        if (sl.stream) (void)hipStreamSynchronize(sl.stream);
         sl.release_device();
         sl.h_desc.release();
         sl.h_words.release();
         sl.h_offsets.release();
         sl.h_fitness.release();
         sl.h_sse.release();
-        if (sl.ev_start) hipEventDestroy(sl.ev_start);
-        if (sl.ev_end) hipEventDestroy(sl.ev_end);
-        if (sl.stream) hipStreamDestroy(sl.stream);
+        if (sl.ev_start) (void)hipEventDestroy(sl.ev_start);
+        if (sl.ev_end) (void)hipEventDestroy(sl.ev_end);
+        if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
     for (auto& kv : s->requests) delete kv.second;
     for (galop_population* p : s->populations) {
@@ -546,7 +558,7 @@ int galop_population_destroy(galop_population* p) {
     s->population_bytes -= p->bytes();
     s->populations.erase(std::find(s->populations.begin(), s->populations.end(), p));
     lk.unlock();
-    hipSetDevice(s->ctx->device);
+    (void)hipSetDevice(s->ctx->device);  // teardown, as in release()
     p->words.release();
     p->offsets.release();
     p->fitness.release();
