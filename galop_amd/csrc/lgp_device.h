@@ -218,23 +218,28 @@ __device__ __forceinline__ DestDecode dest_decode(int nr, int iprot) {
 }
 
 // ---- table rows by index --------------------------------------------------
-// The sin/cos and log tables hold 16-byte rows (two doubles).  A row is read
-// with a buffer load in index mode: the hardware forms base + index * 16 from
-// the row index the reduction produces, so no per-case address arithmetic is
-// issued (a global load needs the index shifted, and log's row offset by 91
-// and masked).  The descriptor is four SGPRs: the base address, a 16-byte
-// stride, the number of rows (index-mode loads past it return zeros) and the
-// gfx9 data-format word.  Every index used is in range by construction.
+// The sin/cos and log tables hold 16-byte rows (two doubles), the exp table
+// 8-byte rows.  A row is read with a buffer load in index mode: the hardware
+// forms base + index * stride from the row index the reduction produces, so
+// no per-case address arithmetic is issued (a global load needs the index
+// shifted, and log's row offset by 91 and masked).  The descriptor is four
+// SGPRs: the base address, the stride, the number of rows (index-mode loads
+// past it return zeros) and the gfx9 data-format word.  Every index used is
+// in range by construction.
 typedef int RowRsrc __attribute__((ext_vector_type(4)));
 typedef uint32_t RowBits __attribute__((ext_vector_type(4)));
+typedef uint32_t RowBits2 __attribute__((ext_vector_type(2)));
 __device__ RowBits galop_row_load(RowRsrc rsrc, int vindex, int voffset, int soffset,
                                   int aux) __asm("llvm.amdgcn.struct.buffer.load.v4i32");
+__device__ RowBits2 galop_row_load2(RowRsrc rsrc, int vindex, int voffset, int soffset,
+                                    int aux) __asm("llvm.amdgcn.struct.buffer.load.v2i32");
 
-__device__ __forceinline__ RowRsrc row_rsrc(const double* rows, uint32_t n_rows) {
+__device__ __forceinline__ RowRsrc row_rsrc(const void* rows, uint32_t n_rows,
+                                            uint32_t stride = 16) {
     const uint64_t a = (uint64_t)(uintptr_t)rows;
     RowRsrc r;
     r.x = (int)(uint32_t)a;
-    r.y = (int)(((uint32_t)(a >> 32) & 0xffffu) | (16u << 16));
+    r.y = (int)(((uint32_t)(a >> 32) & 0xffffu) | (stride << 16));
     r.z = (int)n_rows;
     r.w = 0x00020000;
     return r;
@@ -255,6 +260,15 @@ __device__ __forceinline__ RowRsrc sincos_rows(int want_cos) {
 __device__ __forceinline__ RowRsrc log_rows() {
     return row_rsrc(galop_math::LOG_T - 2 * 91, 182);
 }
+// The bits of row `index` of an 8-byte-row table.
+__device__ __forceinline__ uint64_t load_row8(RowRsrc rsrc, uint32_t index) {
+    const RowBits2 v = galop_row_load2(rsrc, (int)index, 0, 0, 0);
+    return ((uint64_t)v.y << 32) | v.x;
+}
+// exp's 2^(j/256) rows, j in 0..255
+__device__ __forceinline__ RowRsrc exp_rows() {
+    return row_rsrc(galop_math::EXP2_T, 256, 8);
+}
 
 // Ends a group of table-row cases: the scheduler moves no instruction across
 // it, so one group's loads and arithmetic are not spread into the next.
@@ -268,7 +282,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
     const int s1 = (int)((w >> 16) & (uint32_t)(NR - 1));
     const int s2 = (int)((w >> 24) & (uint32_t)(NR - 1));
     const bool kc = (w >> 31) != 0u;
-    // sin, cos and log read a table row per case; the rows of this many cases
+    // sin, cos, exp and log read a table row per case; the rows of this many cases
     // are loaded before any of them is used (a power of two dividing K).
     // Four in flight measured 1-2.5% faster than two where the registers
     // allow; at 8 cases per lane four make the kernel spill 32 bytes per
@@ -314,7 +328,8 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
 // exp (EXT8 opcode 6): a word whose every argument has |x| > 104 (every
 // result +inf or +0, without arithmetic) takes the select-only path; the
 // smallest magnitude decides (min_abs: a NaN argument may or may not take
-// part, and both paths give it NaN).
+// part, and both paths give it NaN).  Otherwise the 2^(j/256) rows of
+// kRowGroup cases are loaded before any of them is used.
 #define GALOP_EXP()                                                       \
     do {                                                                  \
         if (kc) {                                                         \
@@ -328,7 +343,23 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                 _Pragma("unroll") for (int k = 0; k < K; ++k)             \
                     cv[k] = op_exp_special_value(bv[k]);                  \
             } else {                                                      \
-                _Pragma("unroll") for (int k = 0; k < K; ++k) cv[k] = op_exp(bv[k], C); \
+                [[maybe_unused]] const RowRsrc rows = exp_rows();         \
+                _Pragma("unroll") for (int g = 0; g < K; g += kRowGroup) {  \
+                    double rr[kRowGroup];                                 \
+                    uint32_t kw[kRowGroup];                               \
+                    uint64_t tb[kRowGroup];                               \
+                    _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) { \
+                        rr[i] = galop_math::exp_reduce(bv[g + i], kw[i], C); \
+                        if constexpr (kIndexRows)                         \
+                            tb[i] = load_row8(rows, kw[i] & 255u);        \
+                        else                                              \
+                            tb[i] = galop_math::EXP2_T[kw[i] & 255u];     \
+                    }                                                     \
+                    _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) \
+                        cv[g + i] = galop_math::exp_finish(bv[g + i], rr[i], kw[i], \
+                                                           tb[i], C);     \
+                    GALOP_GROUP_FENCE();                                  \
+                }                                                         \
             }                                                             \
             _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]); \
         }                                                                 \

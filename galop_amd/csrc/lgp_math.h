@@ -12,17 +12,18 @@
 //
 // Method (textbook; every constant is derived in galop_amd/math_constants.py
 // from exact integer arithmetic and checked by tests/test_math_constants.py):
-//   exp  x = k ln2 + r, |r| <= ln2/2, ln2 split in three (Cody-Waite);
-//        exp(r) by its Taylor series to r^12; scale by 2^k.
+//   exp  x = k ln2/256 + r, |r| <= ln2/512, with ln2/256 in two parts for
+//        two FMAs and k = rint(x 256/ln2) rounded by adding 1.5 2^52; then
+//        exp(x) = 2^(k >> 8) T_j (1 + q), T_j = 2^(j/256) from a 256-entry
+//        table, j = k mod 256, and q = exp(r) - 1 by its Taylor series to
+//        r^4.  The factor 2^(k >> 8) is an integer addition to the high word
+//        of T_j (the table holds that word less j << 12, so k << 12 is added).
 //   log  x = y 2^e with y in [S, 2S), S = sqrt(1/2) cut to the high word of
 //        its double (found by integer operations on that word); j = rint((y-1) 128) picks
 //        c_j = 128/(128+j) rounded to 24 bits, so r = y c_j - 1 is exact
 //        (|r| < 0.0056); log x = e ln2 + ln(1/c_j) + log1p(r), with ln(1/c_j)
 //        from a 91-entry table and log1p(r) by its series to r^7.  No
-//        division.  (The same table method for exp, 2^(j/64) with a
-//        degree-6 series, measured slower in the interpreter than the
-//        table-free Taylor form: its per-lane table load stalls the wave
-//        behind the prefetch of the next program chunk.)
+//        division.
 //   sin, cos
 //        x = k pi/128 + t, |t| <= pi/256: |x| < 2^25 by Cody-Waite with
 //        pi/128 in two parts for two FMAs and k = rint(x 128/pi) rounded by
@@ -33,8 +34,8 @@
 //        (sin, cos)(j pi/128) from a 256-entry table, j = k mod 256 (cos x is
 //        row j + 64), and Taylor series of sin t to t^5 and cos t - 1 to
 //        t^6.
-// Everything is IEEE double add/mul/fma, integer operations on the bits,
-// ldexp and conversions, so the same source compiled for the host gives the
+// Everything is IEEE double add/mul/fma, integer operations on the bits and
+// conversions, so the same source compiled for the host gives the
 // same bits (NaN payloads aside).
 #pragma once
 #include <stdint.h>
@@ -64,11 +65,13 @@ constexpr double PIO128_SCALED64 = 0x1.921fb54442d18p-70;  // pi/128 * 2^-64
 constexpr double LN2_0 = 0x1.62e42ff000000p-1;
 constexpr double LN2_1 = -0x1.718432a200000p-35;
 constexpr double LN2_2 = 0x1.3c7673007e5edp-69;
-// ln 2 = LN2_HI + LN2_LO for exp's reduction, the same way: |k| <= 151 and
-// x - k LN2_HI is a multiple of 2^-53 below 1/2, so fma(-k, LN2_HI, x) is exact.
-constexpr double LN2_HI = 0x1.62e42fefa39efp-1;
-constexpr double LN2_LO = 0x1.abc9e3b39803fp-56;
-constexpr double INV_LN2 = 0x1.71547652b82fep+0;
+// ln2/256 = LN2O256_HI + LN2O256_LO for exp's reduction: for a float x with
+// |x| <= 104.5 and k = rint(x 256/ln2) (|k| < 2^16), x - k LN2O256_HI is a
+// multiple of 2^-61 below 2^-9 (or x itself when k = 0), so
+// fma(-k, LN2O256_HI, x) is exact.
+constexpr double LN2O256_HI = 0x1.62e42fefa39efp-9;
+constexpr double LN2O256_LO = 0x1.abc9e3b39803fp-64;
+constexpr double INV_LN2_256 = 0x1.71547652b82fep+8;  // 256/ln2
 // Rounding by addition: for |v| < 2^51, fl(v + 1.5 2^52) is 1.5 2^52 plus v
 // rounded to the nearest integer (ties to even), and the low word of that
 // double is the integer modulo 2^32.
@@ -76,12 +79,8 @@ constexpr double ROUND_MAGIC = 0x1.8000000000000p+52;
 // log's split point: the high word of sqrt(1/2) (0x1.6a09e667f3bcdp-1)
 constexpr uint32_t kLogSplitHi = 0x3fe6a09eu;
 
-// 1/n!, n = 0..12
-constexpr double EXP_C[13] = {
-    0x1.0000000000000p+0, 0x1.0000000000000p+0, 0x1.0000000000000p-1, 0x1.5555555555555p-3,
-    0x1.5555555555555p-5, 0x1.1111111111111p-7, 0x1.6c16c16c16c17p-10, 0x1.a01a01a01a01ap-13,
-    0x1.a01a01a01a01ap-16, 0x1.71de3a556c734p-19, 0x1.27e4fb7789f5cp-22, 0x1.ae64567f544e4p-26,
-    0x1.1eed8eff8d898p-29};
+// 1/(n+1)!, n = 0..3  ((exp(r) - 1) / r = sum_n r^n / (n+1)!)
+constexpr double EXPM1_C[4] = {0x1.0000000000000p+0, 0x1.0000000000000p-1, 0x1.5555555555555p-3, 0x1.5555555555555p-5};
 // (-1)^k / (2k+1)!, k = 1..2
 constexpr double SINT_C[2] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7};
 // (-1)^k / (2k)!, k = 1..3
@@ -511,6 +510,76 @@ GM_CONST double LOG_T[182] = {
     0x1.6a13ce0000000p-1, 0x1.62c82c939c7a3p-2,
 };
 
+// 2^(j/256), j = 0..255: the bits of the nearest double with j << 12
+// subtracted from the high word (mod 2^32); exp adds k << 12 to that word,
+// k = 256 e + j, and gets the high word of 2^(j/256) 2^e.
+GM_CONST uint64_t EXP2_T[256] = {
+    0x3FF0000000000000ull, 0x3FEFFB1AFA5ABCBFull, 0x3FEFF63DA9FB3335ull, 0x3FEFF168143B0281ull,
+    0x3FEFEC9A3E778061ull, 0x3FEFE7D42E11BBCCull, 0x3FEFE315E86E7F85ull, 0x3FEFDE5F72F654B1ull,
+    0x3FEFD9B0D3158574ull, 0x3FEFD50A0E3C1F89ull, 0x3FEFD06B29DDF6DEull, 0x3FEFCBD42B72A836ull,
+    0x3FEFC74518759BC8ull, 0x3FEFC2BDF66607E0ull, 0x3FEFBE3ECAC6F383ull, 0x3FEFB9C79B1F3919ull,
+    0x3FEFB5586CF9890Full, 0x3FEFB0F145E46C85ull, 0x3FEFAC922B7247F7ull, 0x3FEFA83B23395DECull,
+    0x3FEFA3EC32D3D1A2ull, 0x3FEF9FA55FDFA9C5ull, 0x3FEF9B66AFFED31Bull, 0x3FEF973028D7233Eull,
+    0x3FEF9301D0125B51ull, 0x3FEF8EDBAB5E2AB6ull, 0x3FEF8ABDC06C31CCull, 0x3FEF86A814F204ABull,
+    0x3FEF829AAEA92DE0ull, 0x3FEF7E95934F312Eull, 0x3FEF7A98C8A58E51ull, 0x3FEF76A45471C3C2ull,
+    0x3FEF72B83C7D517Bull, 0x3FEF6ED48695BBC0ull, 0x3FEF6AF9388C8DEAull, 0x3FEF672658375D2Full,
+    0x3FEF635BEB6FCB75ull, 0x3FEF5F99F8138A1Cull, 0x3FEF5BE084045CD4ull, 0x3FEF582F95281C6Bull,
+    0x3FEF54873168B9AAull, 0x3FEF50E75EB44027ull, 0x3FEF4D5022FCD91Dull, 0x3FEF49C18438CE4Dull,
+    0x3FEF463B88628CD6ull, 0x3FEF42BE3578A819ull, 0x3FEF3F49917DDC96ull, 0x3FEF3BDDA27912D1ull,
+    0x3FEF387A6E756238ull, 0x3FEF351FFB82140Aull, 0x3FEF31CE4FB2A63Full, 0x3FEF2E85711ECE75ull,
+    0x3FEF2B4565E27CDDull, 0x3FEF280E341DDF29ull, 0x3FEF24DFE1F56381ull, 0x3FEF21BA7591BB70ull,
+    0x3FEF1E9DF51FDEE1ull, 0x3FEF1B8A66D10F13ull, 0x3FEF187FD0DAD990ull, 0x3FEF157E39771B2Full,
+    0x3FEF1285A6E4030Bull, 0x3FEF0F961F641589ull, 0x3FEF0CAFA93E2F56ull, 0x3FEF09D24ABD886Bull,
+    0x3FEF06FE0A31B715ull, 0x3FEF0432EDEEB2FDull, 0x3FEF0170FC4CD831ull, 0x3FEEFEB83BA8EA32ull,
+    0x3FEEFC08B26416FFull, 0x3FEEF96266E3FA2Dull, 0x3FEEF6C55F929FF1ull, 0x3FEEF431A2DE883Bull,
+    0x3FEEF1A7373AA9CBull, 0x3FEEEF26231E754Aull, 0x3FEEECAE6D05D866ull, 0x3FEEEA401B7140EFull,
+    0x3FEEE7DB34E59FF7ull, 0x3FEEE57FBFEC6CF4ull, 0x3FEEE32DC313A8E5ull, 0x3FEEE0E544EDE173ull,
+    0x3FEEDEA64C123422ull, 0x3FEEDC70DF1C5175ull, 0x3FEEDA4504AC801Cull, 0x3FEED822C367A024ull,
+    0x3FEED60A21F72E2Aull, 0x3FEED3FB2709468Aull, 0x3FEED1F5D950A897ull, 0x3FEECFFA3F84B9D4ull,
+    0x3FEECE086061892Dull, 0x3FEECC2042A7D232ull, 0x3FEECA41ED1D0057ull, 0x3FEEC86D668B3237ull,
+    0x3FEEC6A2B5C13CD0ull, 0x3FEEC4E1E192AED2ull, 0x3FEEC32AF0D7D3DEull, 0x3FEEC17DEA6DB7D7ull,
+    0x3FEEBFDAD5362A27ull, 0x3FEEBE41B817C114ull, 0x3FEEBCB299FDDD0Dull, 0x3FEEBB2D81D8ABFFull,
+    0x3FEEB9B2769D2CA7ull, 0x3FEEB8417F4531EEull, 0x3FEEB6DAA2CF6642ull, 0x3FEEB57DE83F4EEFull,
+    0x3FEEB42B569D4F82ull, 0x3FEEB2E2F4F6AD27ull, 0x3FEEB1A4CA5D920Full, 0x3FEEB070DDE910D2ull,
+    0x3FEEAF4736B527DAull, 0x3FEEAE27DBE2C4CFull, 0x3FEEAD12D497C7FDull, 0x3FEEAC0827FF07CCull,
+    0x3FEEAB07DD485429ull, 0x3FEEAA11FBA87A03ull, 0x3FEEA9268A5946B7ull, 0x3FEEA84590998B93ull,
+    0x3FEEA76F15AD2148ull, 0x3FEEA6A320DCEB71ull, 0x3FEEA5E1B976DC09ull, 0x3FEEA52AE6CDF6F4ull,
+    0x3FEEA47EB03A5585ull, 0x3FEEA3DD1D1929FDull, 0x3FEEA34634CCC320ull, 0x3FEEA2B9FEBC8FB7ull,
+    0x3FEEA23882552225ull, 0x3FEEA1C1C70833F6ull, 0x3FEEA155D44CA973ull, 0x3FEEA0F4B19E9538ull,
+    0x3FEEA09E667F3BCDull, 0x3FEEA052FA75173Eull, 0x3FEEA012750BDABFull, 0x3FEE9FDCDDD47645ull,
+    0x3FEE9FB23C651A2Full, 0x3FEE9F9298593AE5ull, 0x3FEE9F7DF9519484ull, 0x3FEE9F7466F42E87ull,
+    0x3FEE9F75E8EC5F74ull, 0x3FEE9F8286EAD08Aull, 0x3FEE9F9A48A58174ull, 0x3FEE9FBD35D7CBFDull,
+    0x3FEE9FEB564267C9ull, 0x3FEEA024B1AB6E09ull, 0x3FEEA0694FDE5D3Full, 0x3FEEA0B938AC1CF6ull,
+    0x3FEEA11473EB0187ull, 0x3FEEA17B0976CFDBull, 0x3FEEA1ED0130C132ull, 0x3FEEA26A62FF86F0ull,
+    0x3FEEA2F336CF4E62ull, 0x3FEEA3878491C491ull, 0x3FEEA427543E1A12ull, 0x3FEEA4D2ADD106D9ull,
+    0x3FEEA589994CCE13ull, 0x3FEEA64C1EB941F7ull, 0x3FEEA71A4623C7ADull, 0x3FEEA7F4179F5B21ull,
+    0x3FEEA8D99B4492EDull, 0x3FEEA9CAD931A436ull, 0x3FEEAAC7D98A6699ull, 0x3FEEABD0A478580Full,
+    0x3FEEACE5422AA0DBull, 0x3FEEAE05BAD61778ull, 0x3FEEAF3216B5448Cull, 0x3FEEB06A5E0866D9ull,
+    0x3FEEB1AE99157736ull, 0x3FEEB2FED0282C8Aull, 0x3FEEB45B0B91FFC6ull, 0x3FEEB5C353AA2FE2ull,
+    0x3FEEB737B0CDC5E5ull, 0x3FEEB8B82B5F98E5ull, 0x3FEEBA44CBC8520Full, 0x3FEEBBDD9A7670B3ull,
+    0x3FEEBD829FDE4E50ull, 0x3FEEBF33E47A22A2ull, 0x3FEEC0F170CA07BAull, 0x3FEEC2BB4D53FE0Dull,
+    0x3FEEC49182A3F090ull, 0x3FEEC674194BB8D5ull, 0x3FEEC86319E32323ull, 0x3FEECA5E8D07F29Eull,
+    0x3FEECC667B5DE565ull, 0x3FEECE7AED8EB8BBull, 0x3FEED09BEC4A2D33ull, 0x3FEED2C980460AD8ull,
+    0x3FEED503B23E255Dull, 0x3FEED74A8AF46052ull, 0x3FEED99E1330B358ull, 0x3FEEDBFE53C12E59ull,
+    0x3FEEDE6B5579FDBFull, 0x3FEEE0E521356EBAull, 0x3FEEE36BBFD3F37Aull, 0x3FEEE5FF3A3C2774ull,
+    0x3FEEE89F995AD3ADull, 0x3FEEEB4CE622F2FFull, 0x3FEEEE07298DB666ull, 0x3FEEF0CE6C9A8952ull,
+    0x3FEEF3A2B84F15FBull, 0x3FEEF68415B749B1ull, 0x3FEEF9728DE5593Aull, 0x3FEEFC6E29F1C52Aull,
+    0x3FEEFF76F2FB5E47ull, 0x3FEF028CF22749E4ull, 0x3FEF05B030A1064Aull, 0x3FEF08E0B79A6F1Full,
+    0x3FEF0C1E904BC1D2ull, 0x3FEF0F69C3F3A207ull, 0x3FEF12C25BD71E09ull, 0x3FEF16286141B33Dull,
+    0x3FEF199BDD85529Cull, 0x3FEF1D1CD9FA652Cull, 0x3FEF20AB5FFFD07Aull, 0x3FEF244778FAFB22ull,
+    0x3FEF27F12E57D14Bull, 0x3FEF2BA88988C933ull, 0x3FEF2F6D9406E7B5ull, 0x3FEF33405751C4DBull,
+    0x3FEF3720DCEF9069ull, 0x3FEF3B0F2E6D1675ull, 0x3FEF3F0B555DC3FAull, 0x3FEF43155B5BAB74ull,
+    0x3FEF472D4A07897Cull, 0x3FEF4B532B08C968ull, 0x3FEF4F87080D89F2ull, 0x3FEF53C8EACAA1D6ull,
+    0x3FEF5818DCFBA487ull, 0x3FEF5C76E862E6D3ull, 0x3FEF60E316C98398ull, 0x3FEF655D71FF6075ull,
+    0x3FEF69E603DB3285ull, 0x3FEF6E7CD63A8315ull, 0x3FEF7321F301B460ull, 0x3FEF77D5641C0658ull,
+    0x3FEF7C97337B9B5Full, 0x3FEF81676B197D17ull, 0x3FEF864614F5A129ull, 0x3FEF8B333B16EE12ull,
+    0x3FEF902EE78B3FF6ull, 0x3FEF953924676D76ull, 0x3FEF9A51FBC74C83ull, 0x3FEF9F7977CDB740ull,
+    0x3FEFA4AFA2A490DAull, 0x3FEFA9F4867CCA6Eull, 0x3FEFAF482D8E67F1ull, 0x3FEFB4AAA2188510ull,
+    0x3FEFBA1BEE615A27ull, 0x3FEFBF9C1CB6412Aull, 0x3FEFC52B376BBA97ull, 0x3FEFCAC948DD7274ull,
+    0x3FEFD0765B6E4540ull, 0x3FEFD632798844F8ull, 0x3FEFDBFDAD9CBE14ull, 0x3FEFE1D802243C89ull,
+    0x3FEFE7C1819E90D8ull, 0x3FEFEDBA3692D514ull, 0x3FEFF3C22B8F71F1ull, 0x3FEFF9D96B2A23D9ull,
+};
+
 // Every FP64 constant above in one table; the functions below take it as an
 // argument.  On the device the table sits in constant memory and is reached
 // through coef_table(), a pointer the compiler cannot see through, so each
@@ -519,26 +588,25 @@ GM_CONST double LOG_T[182] = {
 // out of the interpreter loop into SGPR and VGPR pairs (a VOP3 instruction on
 // gfx950 takes no 64-bit literal), about 60 VGPRs that cost occupancy.
 enum CoefSlot : int {
-    kExpC = 0,        // EXP_C[0..12]
-    kLog1pC = 13,     // LOG1P_C[0..5]
-    kSinT = 19,       // SINT_C[0..1]
-    kCosT = 21,       // COST_C[0..2]
-    kPio128 = 24,     // PIO128_HI, PIO128_LO
-    kInvPio128 = 26,
-    kPio128Scaled64 = 27,
-    kLn2 = 28,        // LN2_0, LN2_1, LN2_2
-    kLn2Fma = 31,     // LN2_HI, LN2_LO
-    kInvLn2 = 33,
-    kRoundMagic = 34,
-    kCoefCount = 35
+    kExpC = 0,        // EXPM1_C[0..3]
+    kLog1pC = 4,      // LOG1P_C[0..5]
+    kSinT = 10,       // SINT_C[0..1]
+    kCosT = 12,       // COST_C[0..2]
+    kPio128 = 15,     // PIO128_HI, PIO128_LO
+    kInvPio128 = 17,
+    kPio128Scaled64 = 18,
+    kLn2 = 19,        // LN2_0, LN2_1, LN2_2
+    kLn2o256 = 22,    // LN2O256_HI, LN2O256_LO
+    kInvLn2x256 = 24,
+    kRoundMagic = 25,
+    kCoefCount = 26
 };
 #define GM_COEF_INIT                                                              \
-    {EXP_C[0], EXP_C[1], EXP_C[2], EXP_C[3], EXP_C[4], EXP_C[5], EXP_C[6],        \
-     EXP_C[7], EXP_C[8], EXP_C[9], EXP_C[10], EXP_C[11], EXP_C[12],               \
+    {EXPM1_C[0], EXPM1_C[1], EXPM1_C[2], EXPM1_C[3],                              \
      LOG1P_C[0], LOG1P_C[1], LOG1P_C[2], LOG1P_C[3], LOG1P_C[4], LOG1P_C[5],      \
      SINT_C[0], SINT_C[1], COST_C[0], COST_C[1], COST_C[2],                        \
      PIO128_HI, PIO128_LO, INV_PIO128, PIO128_SCALED64,                           \
-     LN2_0, LN2_1, LN2_2, LN2_HI, LN2_LO, INV_LN2, ROUND_MAGIC}
+     LN2_0, LN2_1, LN2_2, LN2O256_HI, LN2O256_LO, INV_LN2_256, ROUND_MAGIC}
 
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const __attribute__((address_space(4))) double* CoefPtr;
@@ -606,22 +674,44 @@ GM_HD float expf_special_value(float x) {
     return (x != x) ? x : (x > 0.0f ? u2f(0x7f800000u) : 0.0f);
 }
 
-// No branches, so the K cases of an interpreter word can run interleaved.
-// The argument is clamped to [-104.5, 89.5]: exp(89.5) rounds to +inf and
-// exp(-104.5) to +0 in FP32, as every argument beyond does; NaN is patched
-// at the end (the clamp maps it to -104.5).
-GM_HD float expf_r(float x, CoefPtr C) {
+// In two steps, so that an interpreter word can issue the table loads of all
+// its cases before it uses any: exp_reduce splits x and gives k, whose low
+// byte is the table row; exp_finish takes the row's bits.  No branches, so
+// the K cases of a word run interleaved.  The argument is clamped to
+// [-104.5, 89.5]: exp(89.5) rounds to +inf and exp(-104.5) to +0 in FP32, as
+// every argument beyond does; NaN is patched at the end (the clamp maps it
+// to -104.5).
+GM_HD double exp_reduce(float x, uint32_t& kw, CoefPtr C) {
     const float xc = clampf(x, -104.5f, 89.5f);  // NaN: any value, replaced below
     const double xd = (double)xc;
-    // k = rint(x / ln 2) by addition: |k| <= 151, and its low word is k as an
-    // int32 (two's complement under the magic's high bits)
-    const double kb = fma_d(xd, C[kInvLn2], C[kRoundMagic]);
+    // k = rint(x 256/ln2) by addition: |k| <= 38,595, and the low word of the
+    // sum is k as an int32 (two's complement under the magic's high bits)
+    const double kb = fma_d(xd, C[kInvLn2x256], C[kRoundMagic]);
     const double k = kb - C[kRoundMagic];
-    const double r = fma_d(-k, C[kLn2Fma + 1], fma_d(-k, C[kLn2Fma], xd));
-    double p = fma_d(vreg(C[kExpC + 12]), r, C[kExpC + 11]);
-    for (int n = 10; n >= 0; --n) p = fma_d(p, r, C[kExpC + n]);
-    const float v = (float)ldexp(p, (int32_t)(uint32_t)d2u(kb));
+    kw = (uint32_t)d2u(kb);
+    return fma_d(-k, C[kLn2o256 + 1], fma_d(-k, C[kLn2o256], xd));
+}
+// exp(x) = T (1 + q) with T = 2^(k >> 8) 2^(j/256) and q = exp(r) - 1; `row`
+// is EXP2_T[k & 255].  The high word of T is the row's plus k << 12, which
+// adds k >> 8 to its exponent field: T stays a normal double (exponent
+// -151..129), so the scaling is exact, and one rounding to FP32 at the end
+// also gives subnormal, zero and infinite results.
+GM_HD float exp_finish(float x, double r, uint32_t kw, uint64_t row, CoefPtr C) {
+    const double T = u2d(((uint64_t)((uint32_t)(row >> 32) + (kw << 12)) << 32) |
+                         (row & 0xffffffffu));
+    // q = r (1 + r/2 + r^2/6 + r^3/24): the rest is below r^5/120 < 4e-17
+    // for |r| <= ln2/512
+    double p = fma_d(vreg(C[kExpC + 3]), r, C[kExpC + 2]);
+    p = fma_d(p, r, C[kExpC + 1]);
+    p = fma_d(p, r, C[kExpC]);
+    const double q = p * r;
+    const float v = (float)fma_d(T, q, T);
     return (x != x) ? x : v;
+}
+GM_HD float expf_r(float x, CoefPtr C) {
+    uint32_t kw;
+    const double r = exp_reduce(x, kw, C);
+    return exp_finish(x, r, kw, EXP2_T[kw & 255u], C);
 }
 
 // ---- log of a positive, finite, non-zero float --------------------------

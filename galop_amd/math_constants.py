@@ -156,6 +156,43 @@ def sincos_table():
     return out + out[:2 * SINCOS_QUARTER]
 
 
+# exp reduces by ln2/EXP_DIV: x = k ln2/256 + r, |r| <= ln2/512, and
+# exp(x) = 2^(k >> 8) 2^(j/256) exp(r) with j = k mod 256
+EXP_DIV = 256
+
+
+def exp_fixed(v_fixed, prec=PREC):
+    """exp(v) * 2^prec for 0 <= v = v_fixed / 2^prec < 1, by its Taylor series
+    in fixed point (a few units in the last place)."""
+    guard = 64
+    p = prec + guard
+    v = v_fixed << guard
+    total = 0
+    term = 1 << p
+    n = 0
+    while term:
+        total += term
+        n += 1
+        term = (term * v >> p) // n
+    return total >> guard
+
+
+def exp2_table():
+    """The 256 doubles nearest 2^(j/256), j = 0..255, each as its 64-bit
+    pattern with j << 12 subtracted from the high word (mod 2^32).  Adding
+    k << 12 to that high word, k = 256 e + j, gives the high word of
+    2^(j/256) 2^e: the exponent field grows by e and the j << 12 cancels."""
+    import struct
+    ln2 = ln2_fixed()
+    out = []
+    for j in range(EXP_DIV):
+        t = float(Fraction(exp_fixed(j * ln2 // EXP_DIV), 1 << PREC))
+        bits = struct.unpack("<Q", struct.pack("<d", t))[0]
+        hi = ((bits >> 32) - (j << 12)) & 0xFFFFFFFF
+        out.append((hi << 32) | (bits & 0xFFFFFFFF))
+    return out
+
+
 def round_bits(v, bits):
     """The rational v rounded to `bits` significant bits (nearest, ties even)."""
     e = v.numerator.bit_length() - v.denominator.bit_length()
@@ -212,12 +249,14 @@ def constants():
         "INV_PIO128": float(Fraction(128 << PREC, pi)),
         "PIO128_SCALED64": float(Fraction(pi, 1 << (PREC + 7)) / (1 << 64)),
         "LN2": split(ln2, PREC, 32),
-        "LN2_FMA": split_fma(ln2, PREC),
-        "INV_LN2": float(Fraction(1 << PREC, ln2)),
+        "LN2O256_FMA": split_fma(ln2, PREC + 8),
+        "INV_LN2_256": float(Fraction(EXP_DIV << PREC, ln2)),
         # 1.5 2^52: adding it rounds |v| < 2^51 to an integer held in the low word
         "ROUND_MAGIC": float(3 << 51),
         "TWO_OVER_PI_WORDS": two_over_pi_words(12),
-        "EXP_POLY": taylor("exp", 13),
+        # (exp(r) - 1) / r = 1 + r/2! + r^2/3! + r^3/4! (+ r^4/5! ...)
+        "EXPM1_POLY": taylor("exp", 5)[1:],
+        "EXP2_T": exp2_table(),
         "SINT_POLY": taylor("sin", 3)[1:],   # -1/3!, 1/5!
         "COST_POLY": taylor("cos", 4)[1:],   # -1/2!, 1/4!, -1/6!
         "SINCOS_T": sincos_table(),
@@ -234,6 +273,8 @@ def c_literals():
     for k, v in c.items():
         if k == "TWO_OVER_PI_WORDS":
             lines.append(f"{k} = {{" + ", ".join(f"0x{w:08X}u" for w in v) + "}")
+        elif k == "EXP2_T":
+            lines.append(f"{k} = {{" + ", ".join(f"0x{w:016X}ull" for w in v) + "}")
         elif isinstance(v, list):
             lines.append(f"{k} = {{" + ", ".join(f"{x!r}" if False else float.hex(x) for x in v) + "}")
         else:

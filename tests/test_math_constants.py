@@ -38,6 +38,8 @@ def _header_values():
     for name, n in (("SINCOS_T", 640), ("LOG_T", 182)):
         table = re.search(name + r"\[%d\] = \{([^}]*)\}" % n, text).group(1)
         vals[name] = [float.fromhex(t.strip()) for t in table.split(",") if t.strip()]
+    table = re.search(r"EXP2_T\[256\] = \{([^}]*)\}", text).group(1)
+    vals["EXP2_T"] = [int(t.strip().rstrip("ul"), 16) for t in table.split(",") if t.strip()]
     return vals
 
 
@@ -45,13 +47,14 @@ def test_constants_match_derivation():
     c = MC.constants()
     h = _header_values()
     assert [h["PIO128_HI"], h["PIO128_LO"]] == c["PIO128_FMA"]
-    assert [h["LN2_HI"], h["LN2_LO"]] == c["LN2_FMA"]
+    assert [h["LN2O256_HI"], h["LN2O256_LO"]] == c["LN2O256_FMA"]
     assert [h["LN2_0"], h["LN2_1"], h["LN2_2"]] == c["LN2"]
     assert h["INV_PIO128"] == c["INV_PIO128"]
     assert h["PIO128_SCALED64"] == c["PIO128_SCALED64"]
-    assert h["INV_LN2"] == c["INV_LN2"]
+    assert h["INV_LN2_256"] == c["INV_LN2_256"]
     assert h["ROUND_MAGIC"] == c["ROUND_MAGIC"] == 1.5 * 2.0 ** 52
-    assert h["EXP_C"] == c["EXP_POLY"]
+    assert h["EXPM1_C"] == c["EXPM1_POLY"] == [1.0, 0.5, 1 / 6, 1 / 24]
+    assert h["EXP2_T"] == c["EXP2_T"]
     assert h["SINT_C"] == c["SINT_POLY"]
     assert h["COST_C"] == c["COST_POLY"]
     assert h["LOG1P_C"] == c["LOG1P_POLY"]
@@ -72,23 +75,24 @@ def test_constants_match_derivation():
     # second the rest to within half its ulp
     pi128 = Fraction(MC.pi_fixed(), 1 << (MC.PREC + 7))
     ln2 = Fraction(MC.ln2_fixed(), 1 << MC.PREC)
-    for (hi, lo), v in ((c["PIO128_FMA"], pi128), (c["LN2_FMA"], ln2)):
+    for (hi, lo), v in ((c["PIO128_FMA"], pi128), (c["LN2O256_FMA"], ln2 / 256)):
         assert hi == float(v) and lo == float(v - Fraction(hi))
         assert abs(Fraction(hi) + Fraction(lo) - v) < v * Fraction(1, 2 ** 105)
-    assert c["PIO128_FMA"][0] == math.pi / 128 and c["LN2_FMA"][0] == math.log(2)
+    assert c["PIO128_FMA"][0] == math.pi / 128 and c["LN2O256_FMA"][0] * 256 == math.log(2)
+    assert c["INV_LN2_256"] == 256 / math.log(2)
 
 
 def test_fma_reductions_are_exact():
     """fma(-k, PIO128_HI, x) for the arguments that take Cody-Waite (|x| < 2^25,
-    k = rint(x 128/pi)) and fma(-k, LN2_HI, x) for exp's clamped arguments
-    (|x| <= 104.5, k = rint(x/ln 2)) are exact: the exact difference is a
+    k = rint(x 128/pi)) and fma(-k, LN2O256_HI, x) for exp's clamped arguments
+    (|x| <= 104.5, k = rint(x 256/ln 2)) are exact: the exact difference is a
     double.  Checked in exact rationals on random arguments of every binade
-    and on the floats nearest multiples of pi/128 and ln 2."""
+    and on the floats nearest multiples of pi/128 and ln2/256."""
     from fractions import Fraction
     c = MC.constants()
     rng = np.random.default_rng(9)
     for (hi, _), inv, lim in ((c["PIO128_FMA"], c["INV_PIO128"], 2.0 ** 25),
-                              (c["LN2_FMA"], c["INV_LN2"], 104.5)):
+                              (c["LN2O256_FMA"], c["INV_LN2_256"], 104.5)):
         xs = np.concatenate([
             rng.uniform(-lim, lim, 20000),
             np.exp2(rng.uniform(-30, np.log2(lim), 20000)) * rng.choice([-1, 1], 20000),
@@ -180,6 +184,55 @@ def test_log_table_identities():
     assert (prod_lo == 0).all()
 
 
+def _exp2_rows(t):
+    """The doubles of EXP2_T: row j with j << 12 added back to its high word."""
+    import struct
+    out = []
+    for j, w in enumerate(t):
+        hi = ((w >> 32) + (j << 12)) & 0xFFFFFFFF
+        out.append(struct.unpack("<d", struct.pack("<Q", (hi << 32) | (w & 0xFFFFFFFF)))[0])
+    return out
+
+
+def _nearest_to_root(t, j):
+    """True when the double t is the one nearest 2^(j/256): the midpoints to
+    its neighbours, raised to the 256th power in exact rationals, bracket 2^j."""
+    from fractions import Fraction
+    half = Fraction(np.spacing(t)) / 2
+    lo_half = Fraction(t - np.nextafter(t, 0.0)) / 2
+    return (Fraction(t) - lo_half) ** 256 < 2 ** j < (Fraction(t) + half) ** 256
+
+
+def test_exp2_table_identities():
+    """Each row is the double nearest 2^(j/256), decided exactly (the 256th
+    powers of the midpoints around it bracket 2^j), independently of the series
+    that made it; row 0 is 1 and row 128 sqrt(2) correctly rounded (integer
+    square root); and the high-word addition of k << 12 scales row k mod 256
+    by 2^(k >> 8) for k across exp's whole range."""
+    from fractions import Fraction
+    from math import isqrt
+    import struct
+    t = MC.constants()["EXP2_T"]
+    rows = _exp2_rows(t)
+    assert rows[0] == 1.0
+    assert rows[128] == float(Fraction(isqrt(2 << 240), 1 << 120))
+    for j in range(256):
+        assert 1.0 <= rows[j] < 2.0 and _nearest_to_root(rows[j], j), j
+    for k in (-38595, -38400, -257, -256, -255, -1, 0, 1, 255, 256, 33055):
+        w = t[k & 255]
+        hi = ((w >> 32) + (k << 12)) & 0xFFFFFFFF
+        v = struct.unpack("<d", struct.pack("<Q", (hi << 32) | (w & 0xFFFFFFFF)))[0]
+        assert v == rows[k & 255] * 2.0 ** (k >> 8), k
+
+
+def test_planted_wrong_exp2_entry_is_caught():
+    rows = _exp2_rows(MC.constants()["EXP2_T"])
+    j = 77
+    assert _nearest_to_root(rows[j], j)
+    assert not _nearest_to_root(float(np.nextafter(rows[j], 2.0)), j)
+    assert not _nearest_to_root(float(np.nextafter(rows[j], 0.0)), j)
+
+
 def test_planted_wrong_log_entry_is_caught():
     import math
     t = list(MC.constants()["LOG_T"])
@@ -261,3 +314,37 @@ def test_special_cases():
     assert L.gm_plogf(inf) == inf and L.gm_plogf(-inf) == inf and np.isnan(L.gm_plogf(nan))
     assert L.gm_plogf(1.0) == 0.0 and L.gm_plogf(-1.0) == 0.0
     assert L.gm_plogf(1e-45) < -103.0  # subnormal argument
+
+
+def test_host_exp_correctly_rounded_on_a_stride_of_its_range():
+    """The host build's exp equals exp in long double (64-bit significand)
+    rounded to float on every 97th float of [-104.5, 89.5], about 23 million
+    arguments, none of which lies within 2^-58 of a rounding midpoint, so the
+    long-double value decides the rounding.  (The same comparison over every
+    float of the range, 2,240,020,482 arguments, also finds no difference.)"""
+    if np.finfo(np.longdouble).nmant < 63:
+        pytest.skip("long double is not the 64-bit-significand format here")
+    L = _host()
+    top_pos = int(np.float32(89.5).view(np.uint32))
+    top_neg = int(np.float32(-104.5).view(np.uint32))
+    spans = [(0, top_pos), (0x80000000, top_neg)]
+    checked = 0
+    for lo, hi in spans:
+        bits = np.arange(lo, hi + 1, 97, dtype=np.uint64).astype(np.uint32)
+        for part in np.array_split(bits, 8):
+            x = part.view(np.float32)
+            y = np.empty_like(x)
+            L.gm_map(6, x.ctypes.data, y.ctypes.data, x.size)
+            e = np.exp(x.astype(np.longdouble))
+            with np.errstate(over="ignore"):  # results beyond the float range are inf
+                ref = e.astype(np.float32)
+            assert np.array_equal(y.view(np.uint32), ref.view(np.uint32))
+            ok = (ref > 0) & np.isfinite(ref)
+            c = ref[ok].astype(np.longdouble)
+            up = np.nextafter(ref[ok], np.float32(np.inf)).astype(np.longdouble)
+            dn = np.nextafter(ref[ok], np.float32(0)).astype(np.longdouble)
+            ee = e[ok]
+            gap = np.minimum(np.abs(ee - (c + up) / 2), np.abs(ee - (c + dn) / 2)) / ee
+            assert gap.min() > np.longdouble(2.0) ** -58
+            checked += x.size
+    assert checked > 23_000_000
