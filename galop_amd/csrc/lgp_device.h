@@ -763,7 +763,10 @@ struct LaneAcc {
 
     __device__ __forceinline__ void add(float out, float y) {
         const float d = out - y;
-        if (fabsf(d) < 0x1p48f) {
+        const bool small = fabsf(d) < 0x1p48f;
+        // the lanes that need the FP64 side sum, from the compare's own mask
+        const uint64_t huge_lanes = __ballot(!small);
+        if (small) {
             // exact residual of d = fl(out - y)   (TwoSum)
             const float bb = d - out;
             const float dlo = (out - (d - bb)) + (-y - bb);
@@ -775,10 +778,19 @@ struct LaneAcc {
             const float z = t - s;
             c += ((s - (t - z)) + (p - z)) + (plo + 2.0f * d * dlo);
             s = t;
-        } else {
-            // huge or non-finite: exact FP64 difference, as the oracle does
-            const double dd = (double)out - (double)y;
-            big += dd * dd;
+        }
+        // Huge or non-finite errors: the exact FP64 difference, as the oracle
+        // does.  This side sum is entered only by waves in which some lane
+        // needs it (a wave-uniform branch): as a short divergent branch it
+        // would be issued, with no lane active, for every case.  The empty
+        // volatile asm keeps the branch (otherwise its body is computed for
+        // every case and selected).
+        if (huge_lanes != 0ull) {
+            __asm__ volatile("");
+            if (!small) {
+                const double dd = (double)out - (double)y;
+                big += dd * dd;
+            }
         }
     }
     __device__ __forceinline__ double total() const {
@@ -867,27 +879,39 @@ __device__ __forceinline__ double block_sse_run(const DevData& ds, int lane, int
         RegFile<NR, K> r;
         load_cases<NR, K>(ds, first, lane, r);
         run(r);
-        // With kTargetsFirst the K targets are loaded together before any is
-        // used (the accumulation branches per case, and loads inside it are
-        // waited for one at a time); every tile lies inside the dataset's
+        // With kTargetsFirst the K outputs leave the register file first,
+        // which is then dead, and the K targets are loaded together before any
+        // is used (in the accumulation, which branches per case, loads are
+        // waited for one at a time).  Every tile lies inside the dataset's
         // zero-padded n_pad cases, so these loads are in bounds.  The
         // effective-word kernels keep the loads in the accumulation: their
-        // chunk registers leave no room for K more live values at 5 waves.
+        // chunk registers leave no room for K more live values.
         const float* __restrict__ tgt = ds.targets + first;
-        float y[K];
+        float o[K], y[K];
         if constexpr (kTargetsFirst) {
 #pragma unroll
-            for (int k = 0; k < K; ++k) y[k] = tgt[k * kWave + lane];
+            for (int k = 0; k < K; ++k) o[k] = r.get(k, out_reg);
+            __builtin_amdgcn_sched_barrier(0);
+            if (!case_out) {
+#pragma unroll
+                for (int k = 0; k < K; ++k) y[k] = tgt[k * kWave + lane];
+            }
         }
         LaneAcc acc;
         if (f != 0) acc.load(slot);
+        const auto take = [&](int k) {
+            const float out = kTargetsFirst ? o[k] : r.get(k, out_reg);
+            if (case_out) case_out[first + k * kWave + lane] = out;
+            else acc.add(out, kTargetsFirst ? y[k] : tgt[k * kWave + lane]);
+        };
+        // a whole tile inside the block needs no per-case bounds test
+        if (f + tile_cases <= nb) {
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (f + k * kWave + lane < nb) {
-                const float out = r.get(k, out_reg);
-                if (case_out) case_out[first + k * kWave + lane] = out;
-                else acc.add(out, kTargetsFirst ? y[k] : tgt[k * kWave + lane]);
-            }
+            for (int k = 0; k < K; ++k) take(k);
+        } else {
+#pragma unroll
+            for (int k = 0; k < K; ++k)
+                if (f + k * kWave + lane < nb) take(k);
         }
         if (f + tile_cases >= nb) return wave_sum_f64(acc.total());
         acc.save(slot);
@@ -900,10 +924,7 @@ __device__ __forceinline__ double block_sse(const uint32_t* __restrict__ prog, i
                                             const DevData& ds, int lane, ConstRegs cr,
                                             DestDecode dd, int64_t c0,
                                             float* __restrict__ case_out) {
-    // targets loaded first at up to 4 cases per lane (short tiles, where their
-    // waits were a measurable share: uneven workload -0.7%); at 8 the gain
-    // was within the noise
-    return block_sse_run<NR, K, (K <= 4)>(ds, lane, c0, case_out, [&](RegFile<NR, K>& r) {
+    return block_sse_run<NR, K, true>(ds, lane, c0, case_out, [&](RegFile<NR, K>& r) {
         run_program<NR, K, EXT, STAGE>(prog, len, lane, r, cr, dd);
     });
 }
