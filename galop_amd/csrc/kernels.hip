@@ -57,8 +57,8 @@ plain_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ off
     const ConstRegs cr = load_const_regs(ds.consts, lane);
     int64_t lo, hi;
     program_bounds(offsets, dense_len, stride, p, lo, hi);
-    const double sse =
-        eval_program_wave<NR, K, EXT, EFF>(words + lo, hi - lo, ds, lane, cr, nullptr);
+    const double sse = canonical_sse(
+        eval_program_wave<NR, K, EXT, EFF>(words + lo, hi - lo, ds, lane, cr, nullptr));
     if (lane == 0) {
         fitness[p] = fitness_of_sse(sse, ds);
         if (sse_out) sse_out[p] = sse;
@@ -113,6 +113,7 @@ __global__ void split_finish_kernel(const double* __restrict__ partials, int64_t
     if (p >= n) return;
     double sse = 0.0;
     for (int64_t b = 0; b < n_blocks; ++b) sse += partials[p * n_blocks + b];
+    sse = canonical_sse(sse);
     fitness[p] = fitness_of_sse(sse, ds);
     if (sse_out) sse_out[p] = sse;
 }
@@ -712,8 +713,8 @@ batched_kernel(const uint32_t* __restrict__ words, const int64_t* __restrict__ o
             const int64_t p = __builtin_amdgcn_readlane(p_l, j);
             const int64_t lo = readlane_i64(lo_l, j);
             const int64_t hi = readlane_i64(hi_l, j);
-            const double sse =
-                eval_program_wave<NR, K, EXT, 0, STAGE>(words + lo, hi - lo, ds, lane, cr, nullptr);
+            const double sse = canonical_sse(
+                eval_program_wave<NR, K, EXT, 0, STAGE>(words + lo, hi - lo, ds, lane, cr, nullptr));
             if (lane == 0) {
                 fitness[p] = fitness_of_sse(sse, ds);
                 if (sse_out) sse_out[p] = sse;
@@ -822,7 +823,8 @@ serve_kernel(const ServeDesc* __restrict__ desc, const int64_t* __restrict__ ord
     // the bounds are wave-uniform: broadcast, so the program's base and
     // length live in SGPRs
     const int64_t lo = readlane_i64(d->offsets[p], 0), hi = readlane_i64(d->offsets[p + 1], 0);
-    const double sse = eval_program_wave<NR, K, EXT>(d->words + lo, hi - lo, ds, lane, cr, nullptr);
+    const double sse =
+        canonical_sse(eval_program_wave<NR, K, EXT>(d->words + lo, hi - lo, ds, lane, cr, nullptr));
     if (lane == 0) {
         d->fitness[p] = fitness_of_sse(sse, ds);
         if (d->sse) d->sse[p] = sse;

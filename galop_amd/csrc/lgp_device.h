@@ -833,6 +833,14 @@ __device__ __forceinline__ float sse_to_r2_fitness(double sse, double ss_tot) {
     return r2 < 0.0f ? 0.0f : r2;
 }
 
+// A NaN SSE as the one quiet NaN with a zero sign and payload.  Which NaN a
+// sum of two NaNs returns depends on the order of its operands, which the
+// compiler may swap, so without this the sign of a NaN SSE could differ
+// between kernels that add the same values (the fitness is 0 either way).
+__device__ __forceinline__ double canonical_sse(double sse) {
+    return sse != sse ? __builtin_bit_cast(double, 0x7ff8000000000000ull) : sse;
+}
+
 __device__ __forceinline__ float fitness_of_sse(double sse, const DevData& ds) {
     return ds.fitness_kind == 1 ? sse_to_r2_fitness(sse, ds.ss_tot)
                                 : sse_to_fitness(sse, ds.n_cases);

@@ -171,10 +171,11 @@ void go_plant_fault(int32_t code) { g_fault = code; }
  * square by FMA, Neumaier step; |err| >= 2^48 and non-finite errors go to an
  * FP64 side sum); the 64 lane totals (s + c) + big are added in the fixed
  * tree of a wave's shuffle-down reduction; block partials are added in block
- * order.  This is a second summation of the same FP64-rounded squares, not a
- * different function: it agrees with go_program_sse to about 1e-15 relative
- * and is used where the checker must reproduce the device's bits (fixtures
- * of whole evolutionary runs). */
+ * order; a NaN sum is the quiet NaN with zero sign and payload.  This is a
+ * second summation of the same FP64-rounded squares, not a different
+ * function: it agrees with go_program_sse to about 1e-15 relative and is used
+ * where the checker must reproduce the device's bits (fixtures of whole
+ * evolutionary runs). */
 #define GO_LANES 64
 #define GO_CASE_BLOCK 16384
 
@@ -211,6 +212,13 @@ double go_sse_device_order(const float* outs, const float* targets, int64_t n_ca
         for (int off = GO_LANES / 2; off > 0; off >>= 1)
             for (int l = 0; l < off; ++l) v[l] = v[l] + v[l + off];
         sse += v[0];
+    }
+    /* a NaN sum is returned as the quiet NaN with zero sign and payload, as
+     * the device writes it (which NaN a sum of NaNs returns depends on the
+     * order of the operands) */
+    if (sse != sse) {
+        const uint64_t qnan = 0x7ff8000000000000ULL;
+        memcpy(&sse, &qnan, sizeof sse);
     }
     return sse;
 }

@@ -76,6 +76,22 @@ def test_huge_and_nonfinite_errors():
     assert O.sse_device_order(np.zeros(0, np.float32), np.zeros(0, np.float32)) == 0.0
 
 
+def test_nan_sum_is_the_canonical_quiet_nan():
+    """Whatever the signs and payloads of the NaN outputs, and wherever they
+    sit among the lanes and blocks, a NaN SSE is the quiet NaN with zero sign
+    and payload (the device writes the same)."""
+    neg = np.array([0xFFC00001], np.uint32).view(np.float32)[0]
+    pos = np.array([0x7FC12345], np.uint32).view(np.float32)[0]
+    for n, spots in ((100, [(3, neg)]), (100, [(3, pos), (70, neg)]),
+                     (40000, [(5, neg), (16390, pos), (39999, neg)]), (5, [(0, neg)])):
+        o = np.linspace(-1, 1, n).astype(np.float32)
+        y = np.zeros(n, np.float32)
+        for i, v in spots:
+            o[i] = v
+        got = np.float64(O.sse_device_order(o, y))
+        assert got.view(np.uint64) == np.uint64(0x7FF8000000000000), (n, hex(int(got.view(np.uint64))))
+
+
 @pytest.mark.parametrize("name", ["percase_r8.npz", "percase_r16.npz", "percase_r16ip.npz"])
 def test_matches_case_order_oracle_on_fixtures(name):
     f = load_fixture(name)
