@@ -151,7 +151,9 @@ int galop_eval_dense(galop_ctx* ctx, const galop_dataset* ds,
                      int32_t layout);
 /* CSR programs (src/EvaluatorCuda.cu:241-265): program i is
  * words[offsets[i]:offsets[i+1]], int64 offsets, no length limit.
- * sse_out (nullable) receives the FP64 sum of squared errors.  Blocking. */
+ * sse_out (nullable) receives the FP64 sum of squared errors (a NaN sum as
+ * the quiet NaN with zero sign and payload, from every entry point).
+ * Blocking. */
 int galop_eval_csr(galop_ctx* ctx, const galop_dataset* ds,
                    const uint32_t* words, const int64_t* offsets,
                    int64_t n_programs, float* fitness_out, double* sse_out);
