@@ -255,10 +255,11 @@ __device__ __forceinline__ void load_row(RowRsrc rsrc, uint32_t index, double& f
 __device__ __forceinline__ RowRsrc sincos_rows(int want_cos) {
     return row_rsrc(galop_math::SINCOS_T + 128 * want_cos, 256);
 }
-// (c_j, ln(1/c_j)) by log_reduce's row91 in 91..181: the descriptor's base is
-// 91 rows before the table
+// (c_j, ln(1/c_j)) by log_reduce's row in kLogRowFirst..362: the
+// descriptor's base is kLogRowFirst rows before the table
 __device__ __forceinline__ RowRsrc log_rows() {
-    return row_rsrc(galop_math::LOG_T - 2 * 91, 182);
+    return row_rsrc(galop_math::LOG_T - 2 * galop_math::kLogRowFirst,
+                    galop_math::kLogRowFirst + 182);
 }
 // The bits of row `index` of an 8-byte-row table.
 __device__ __forceinline__ uint64_t load_row8(RowRsrc rsrc, uint32_t index) {
@@ -460,15 +461,16 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
                     double y[kRowGroup], tcj[kRowGroup], tln[kRowGroup];  \
                     int32_t e[kRowGroup];                                 \
                     _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) { \
-                        uint32_t row91;                                   \
+                        uint32_t row181;                                  \
                         y[i] = galop_math::log_reduce(__builtin_fabsf(bv[g + i]), e[i], \
-                                                      row91, C);          \
+                                                      row181, C);         \
                         if constexpr (kIndexRows) {                       \
-                            load_row(rows, row91, tcj[i], tln[i]);        \
+                            load_row(rows, row181, tcj[i], tln[i]);       \
                         } else {                                          \
-                            /* row91 - 91 is in 0..90: the mask only */   \
+                            /* row181 - 181 is in 0..181: the mask only */ \
                             /* tells the compiler so */                   \
-                            const uint32_t row = (row91 - 91u) & 127u;    \
+                            const uint32_t row =                          \
+                                (row181 - galop_math::kLogRowFirst) & 255u; \
                             tcj[i] = galop_math::LOG_T[2 * row];          \
                             tln[i] = galop_math::LOG_T[2 * row + 1];      \
                         }                                                 \

@@ -19,11 +19,11 @@
 //        r^4.  The factor 2^(k >> 8) is an integer addition to the high word
 //        of T_j (the table holds that word less j << 12, so k << 12 is added).
 //   log  x = y 2^e with y in [S, 2S), S = sqrt(1/2) cut to the high word of
-//        its double (found by integer operations on that word); j = rint((y-1) 128) picks
-//        c_j = 128/(128+j) rounded to 24 bits, so r = y c_j - 1 is exact
-//        (|r| < 0.0056); log x = e ln2 + ln(1/c_j) + log1p(r), with ln(1/c_j)
-//        from a 91-entry table and log1p(r) by its series to r^7.  No
-//        division.
+//        its double (found by integer operations on that word); j = rint((y-1) 256) picks
+//        c_j = 256/(256+j) rounded to 24 bits, so r = y c_j - 1 is exact
+//        (|r| < 0.0028); log x = e ln2 + ln(1/c_j) + log1p(r), with ln(1/c_j)
+//        from a 182-entry table, log1p(r) by its series to r^6, and e ln2
+//        added by one FMA.  No division.
 //   sin, cos
 //        x = k pi/128 + t, |t| <= pi/256: |x| < 2^25 by Cody-Waite with
 //        pi/128 in two parts for two FMAs and k = rint(x 128/pi) rounded by
@@ -61,10 +61,8 @@ constexpr double PIO128_HI = 0x1.921fb54442d18p-6;
 constexpr double PIO128_LO = 0x1.1a62633145c07p-60;
 constexpr double INV_PIO128 = 0x1.45f306dc9c883p+5;      // 128/pi
 constexpr double PIO128_SCALED64 = 0x1.921fb54442d18p-70;  // pi/128 * 2^-64
-// ln 2 = LN2_0 + LN2_1 + LN2_2; LN2_{0,1} carry 32 bits each.
-constexpr double LN2_0 = 0x1.62e42ff000000p-1;
-constexpr double LN2_1 = -0x1.718432a200000p-35;
-constexpr double LN2_2 = 0x1.3c7673007e5edp-69;
+// ln 2 rounded to a double
+constexpr double LN2_HI = 0x1.62e42fefa39efp-1;
 // ln2/256 = LN2O256_HI + LN2O256_LO for exp's reduction: for a float x with
 // |x| <= 104.5 and k = rint(x 256/ln2) (|k| < 2^16), x - k LN2O256_HI is a
 // multiple of 2^-61 below 2^-9 (or x itself when k = 0), so
@@ -85,8 +83,8 @@ constexpr double EXPM1_C[4] = {0x1.0000000000000p+0, 0x1.0000000000000p-1, 0x1.5
 constexpr double SINT_C[2] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7};
 // (-1)^k / (2k)!, k = 1..3
 constexpr double COST_C[3] = {-0x1.0000000000000p-1, 0x1.5555555555555p-5, -0x1.6c16c16c16c17p-10};
-// (-1)^(n+1)/n, n = 2..7  (log1p r = r + r^2 sum_n (-1)^(n+1)/n r^(n-2))
-constexpr double LOG1P_C[6] = {-0x1.0000000000000p-1, 0x1.5555555555555p-2, -0x1.0000000000000p-2, 0x1.999999999999ap-3, -0x1.5555555555555p-3, 0x1.2492492492492p-3};
+// (-1)^(n+1)/n, n = 2..6  (log1p r = r + r^2 sum_n (-1)^(n+1)/n r^(n-2))
+constexpr double LOG1P_C[5] = {-0x1.0000000000000p-1, 0x1.5555555555555p-2, -0x1.0000000000000p-2, 0x1.999999999999ap-3, -0x1.5555555555555p-3};
 // bits 2^-1 .. 2^-256 of 2/pi (word 0 first)
 GM_CONST uint32_t TWO_OVER_PI_W[8] = {0xA2F9836Eu, 0x4E441529u, 0xFC2757D1u, 0xF534DDC0u,
                                       0xDB629599u, 0x3C439041u, 0xFE5163ABu, 0xDEBBC561u};
@@ -415,98 +413,189 @@ GM_CONST double SINCOS_T[640] = {
     0x1.ffd886084cd0dp-1, 0x1.92155f7a3667ep-6,
 };
 
-// (c_j, ln(1/c_j)), j = -37..53 at index j + 37, interleaved
-GM_CONST double LOG_T[182] = {
+// (c_j, ln(1/c_j)), j = -75..106 at index j + 75, interleaved
+GM_CONST double LOG_T[364] = {
+    0x1.6a13ce0000000p+0, -0x1.6300334baac3cp-2,
     0x1.6816820000000p+0, -0x1.5d5bdfa595f2ap-2,
+    0x1.661ec60000000p+0, -0x1.57bf73648d1f4p-2,
     0x1.642c860000000p+0, -0x1.522ae1b38a3d5p-2,
+    0x1.623fa80000000p+0, -0x1.4c9e0b8172c37p-2,
     0x1.6058160000000p+0, -0x1.4718dc171c41bp-2,
+    0x1.5e75bc0000000p+0, -0x1.419b438d5e8c4p-2,
     0x1.5c98820000000p+0, -0x1.3c2525533317bp-2,
+    0x1.5ac0560000000p+0, -0x1.36b67563e110fp-2,
     0x1.58ed240000000p+0, -0x1.314f20fd35cd3p-2,
+    0x1.571ed40000000p+0, -0x1.2bef087dc9353p-2,
     0x1.5555560000000p+0, -0x1.269623134db8ap-2,
+    0x1.5390940000000p+0, -0x1.21445520eb8cfp-2,
     0x1.51d07e0000000p+0, -0x1.1bf99425a6b8cp-2,
+    0x1.5015020000000p+0, -0x1.16b5ced2cfb6bp-2,
     0x1.4e5e0a0000000p+0, -0x1.1178e6c27e478p-2,
+    0x1.4cab880000000p+0, -0x1.0c42d516162dfp-2,
     0x1.4afd6a0000000p+0, -0x1.071385f4d5862p-2,
+    0x1.49539e0000000p+0, -0x1.01eae4aa6c690p-2,
     0x1.47ae140000000p+0, -0x1.f991c3cb3b370p-3,
+    0x1.460cbc0000000p+0, -0x1.ef5adb2dcffdcp-3,
     0x1.446f860000000p+0, -0x1.e530edde7100ep-3,
+    0x1.42d6620000000p+0, -0x1.db13d8bd4893bp-3,
     0x1.4141420000000p+0, -0x1.d10383e655e65p-3,
+    0x1.3fb0140000000p+0, -0x1.c6ffbc8f00f71p-3,
     0x1.3e22cc0000000p+0, -0x1.bd0874c3bd8abp-3,
+    0x1.3c995a0000000p+0, -0x1.b31d83a5bce39p-3,
     0x1.3b13b20000000p+0, -0x1.a93ed8c8ad9cap-3,
+    0x1.3991c20000000p+0, -0x1.9f6c3b808964cp-3,
     0x1.3813820000000p+0, -0x1.95a5b2ef70165p-3,
+    0x1.3698e00000000p+0, -0x1.8beb03b38fe73p-3,
     0x1.3521d00000000p+0, -0x1.823c18551a3bep-3,
+    0x1.33ae460000000p+0, -0x1.7898da4444c6fp-3,
     0x1.323e340000000p+0, -0x1.6f01247756aaap-3,
+    0x1.30d1900000000p+0, -0x1.6574eb68c133ap-3,
     0x1.2f684c0000000p+0, -0x1.5bf407b543db1p-3,
+    0x1.2e025c0000000p+0, -0x1.527e5e2a1b58dp-3,
     0x1.2c9fb40000000p+0, -0x1.4913d2733b540p-3,
+    0x1.2b404a0000000p+0, -0x1.3fb454c9928adp-3,
     0x1.29e4120000000p+0, -0x1.365fc6c159004p-3,
+    0x1.288b020000000p+0, -0x1.2d16169868118p-3,
     0x1.27350c0000000p+0, -0x1.23d715e49c1f7p-3,
+    0x1.25e2280000000p+0, -0x1.1aa2bea23f6fcp-3,
     0x1.24924a0000000p+0, -0x1.1178ee227e458p-3,
+    0x1.2345680000000p+0, -0x1.08598e99e39fcp-3,
     0x1.21fb780000000p+0, -0x1.fe89129dbd565p-4,
+    0x1.20b4700000000p+0, -0x1.ec738d30a10e3p-4,
     0x1.1f70480000000p+0, -0x1.da727838446a0p-4,
+    0x1.1e2ef40000000p+0, -0x1.c885845bc4b1ap-4,
     0x1.1cf06a0000000p+0, -0x1.b6ac7c9ad5ad1p-4,
+    0x1.1bb4a40000000p+0, -0x1.a4e763cb1bc38p-4,
     0x1.1a7b960000000p+0, -0x1.9335e4d594988p-4,
+    0x1.1945380000000p+0, -0x1.8197e2740e3f0p-4,
     0x1.1811820000000p+0, -0x1.700d3deeac089p-4,
+    0x1.16e0680000000p+0, -0x1.5e959c59791a7p-4,
     0x1.15b1e60000000p+0, -0x1.4d31165207eacp-4,
+    0x1.1485f00000000p+0, -0x1.3bdf4d7d1ee10p-4,
     0x1.135c820000000p+0, -0x1.2aa0580471746p-4,
+    0x1.12358e0000000p+0, -0x1.1973b6346554fp-4,
     0x1.1111120000000p+0, -0x1.08599959e39a5p-4,
+    0x1.0fef020000000p+0, -0x1.eea338406b7b4p-5,
     0x1.0ecf560000000p+0, -0x1.ccb7265ddb24dp-5,
+    0x1.0db20a0000000p+0, -0x1.aaef1ccfb10bap-5,
     0x1.0c97140000000p+0, -0x1.894a8349fb262p-5,
+    0x1.0b7e6e0000000p+0, -0x1.67c937ed4bad1p-5,
     0x1.0a68100000000p+0, -0x1.466ad942de386p-5,
+    0x1.0953f40000000p+0, -0x1.252f4078d1811p-5,
     0x1.0842100000000p+0, -0x1.0415c89e74404p-5,
+    0x1.0732600000000p+0, -0x1.c63d06c14aa2ap-6,
     0x1.0624de0000000p+0, -0x1.8492858c8c979p-6,
+    0x1.0519800000000p+0, -0x1.432ab25980c41p-6,
     0x1.0410420000000p+0, -0x1.0205a38935667p-6,
+    0x1.03091c0000000p+0, -0x1.8244e0388a0dcp-7,
     0x1.0204080000000p+0, -0x1.01014f588de6dp-7,
+    0x1.0101020000000p+0, -0x1.0081539588355p-8,
     0x1.0000000000000p+0, 0x0.0p+0,
+    0x1.fe01fe0000000p-1, 0x1.ff00ac2b10bc0p-9,
     0x1.fc07f00000000p-1, 0x1.fe02b6b106791p-8,
+    0x1.fa11ca0000000p-1, 0x1.7dc49e7810addp-7,
     0x1.f81f820000000p-1, 0x1.fc0a890fc03e4p-7,
+    0x1.f6310a0000000p-1, 0x1.3cea5df46a5c8p-6,
     0x1.f4465a0000000p-1, 0x1.7b91acfd5b11cp-6,
+    0x1.f25f640000000p-1, 0x1.b9fc0afaf91a1p-6,
     0x1.f07c200000000p-1, 0x1.f82990e783380p-6,
+    0x1.ee9c800000000p-1, 0x1.1b0d90923d990p-5,
     0x1.ecc07c0000000p-1, 0x1.39e86e1febd8dp-5,
+    0x1.eae8080000000p-1, 0x1.58a5b57c8e4dcp-5,
     0x1.e9131a0000000p-1, 0x1.77459be32dd23p-5,
+    0x1.e741aa0000000p-1, 0x1.95c836cc8e3f4p-5,
     0x1.e573ac0000000p-1, 0x1.b42de091971d5p-5,
+    0x1.e3a9180000000p-1, 0x1.d276b22db0b5dp-5,
     0x1.e1e1e20000000p-1, 0x1.f0a30a01162a7p-5,
+    0x1.e01e020000000p-1, 0x1.075982498e472p-4,
     0x1.de5d6e0000000p-1, 0x1.1653710a37ae3p-4,
+    0x1.dca01e0000000p-1, 0x1.253f6120a1419p-4,
     0x1.dae6080000000p-1, 0x1.341d7461bd1ddp-4,
+    0x1.d92f220000000p-1, 0x1.42edcd9a646f2p-4,
     0x1.d77b660000000p-1, 0x1.51b06dd061852p-4,
+    0x1.d5cac80000000p-1, 0x1.60658ad3750c4p-4,
     0x1.d41d420000000p-1, 0x1.6f0d272e56b4dp-4,
+    0x1.d272ca0000000p-1, 0x1.7da76907b12cfp-4,
     0x1.d0cb580000000p-1, 0x1.8c3465e319b45p-4,
+    0x1.cf26e60000000p-1, 0x1.9ab42252033afp-4,
     0x1.cd85680000000p-1, 0x1.a926d8a4ad570p-4,
+    0x1.cbe6da0000000p-1, 0x1.b78c7d2b0edb1p-4,
     0x1.ca4b300000000p-1, 0x1.c5e54bf5bc748p-4,
+    0x1.c8b2660000000p-1, 0x1.d4313a96cb361p-4,
     0x1.c71c720000000p-1, 0x1.e27074e2af2e8p-4,
+    0x1.c5894e0000000p-1, 0x1.f0a30391162cap-4,
     0x1.c3f8f00000000p-1, 0x1.fec9141dbeabbp-4,
+    0x1.c26b540000000p-1, 0x1.06714f3ca5972p-3,
     0x1.c0e0700000000p-1, 0x1.0d77e8cd08e5ap-3,
+    0x1.bf583e0000000p-1, 0x1.14785c6e742bep-3,
     0x1.bdd2b80000000p-1, 0x1.1b72b012f67a8p-3,
+    0x1.bc4fd60000000p-1, 0x1.2266f328a5acep-3,
     0x1.bacf920000000p-1, 0x1.29552c41ff52ep-3,
+    0x1.b951e20000000p-1, 0x1.303d74c647fddp-3,
     0x1.b7d6c40000000p-1, 0x1.371fc161e8f75p-3,
+    0x1.b65e2e0000000p-1, 0x1.3dfc2c26cc62bp-3,
     0x1.b4e81c0000000p-1, 0x1.44d2b38cb7d29p-3,
+    0x1.b374840000000p-1, 0x1.4ba37269a55f0p-3,
     0x1.b203640000000p-1, 0x1.526e5e5a1b438p-3,
+    0x1.b094b40000000p-1, 0x1.5933896982097p-3,
     0x1.af286c0000000p-1, 0x1.5ff3060a793d5p-3,
+    0x1.adbe880000000p-1, 0x1.66acd4072ad51p-3,
     0x1.ac57020000000p-1, 0x1.6d60fce19d21fp-3,
+    0x1.aaf1d20000000p-1, 0x1.740f93fc037bap-3,
     0x1.a98ef60000000p-1, 0x1.7ab890410d909p-3,
+    0x1.a82e660000000p-1, 0x1.815c059c357ffp-3,
     0x1.a6d01a0000000p-1, 0x1.87fa08620c915p-3,
+    0x1.a574100000000p-1, 0x1.8e92902886d46p-3,
     0x1.a41a420000000p-1, 0x1.9525a80f456b8p-3,
+    0x1.a2c2a80000000p-1, 0x1.9bb36547dfb89p-3,
     0x1.a16d400000000p-1, 0x1.a23bbffe2b567p-3,
+    0x1.a01a020000000p-1, 0x1.a8becdf082f1cp-3,
     0x1.9ec8ea0000000p-1, 0x1.af3c91880bffep-3,
+    0x1.9d79f20000000p-1, 0x1.b5b51740fb5abp-3,
     0x1.9c2d140000000p-1, 0x1.bc286be2d8cecp-3,
+    0x1.9ae24e0000000p-1, 0x1.c2968890c18cbp-3,
     0x1.99999a0000000p-1, 0x1.c8ff7a79a9a26p-3,
+    0x1.9852f00000000p-1, 0x1.cf6359209c5eep-3,
     0x1.970e500000000p-1, 0x1.d5c21434fbb98p-3,
+    0x1.95cbb00000000p-1, 0x1.dc1bcdcabec8bp-3,
     0x1.948b100000000p-1, 0x1.e27075e2af2e7p-3,
+    0x1.934c680000000p-1, 0x1.e8c0250aa5a60p-3,
     0x1.920fb40000000p-1, 0x1.ef0adfddc5940p-3,
+    0x1.90d4f20000000p-1, 0x1.f550a0ecb7b4bp-3,
     0x1.8f9c180000000p-1, 0x1.fb918bd5e3e44p-3,
+    0x1.8e65280000000p-1, 0x1.00e6c38ad501ep-2,
     0x1.8d30180000000p-1, 0x1.04025b6b4d04ap-2,
+    0x1.8bfce80000000p-1, 0x1.071b860cd590dp-2,
     0x1.8acb900000000p-1, 0x1.0a3250a7390f0p-2,
+    0x1.899c100000000p-1, 0x1.0d46b3d9ab750p-2,
     0x1.886e600000000p-1, 0x1.1058bd1ae4ae2p-2,
+    0x1.87427c0000000p-1, 0x1.13686fa13a8b1p-2,
     0x1.8618620000000p-1, 0x1.1675c97aba611p-2,
+    0x1.84f00c0000000p-1, 0x1.1980d34542370p-2,
     0x1.83c9780000000p-1, 0x1.1c898b36999fdp-2,
+    0x1.82a4a00000000p-1, 0x1.1f8ffa248a2f3p-2,
     0x1.8181820000000p-1, 0x1.22941e6cf7969p-2,
+    0x1.8060180000000p-1, 0x1.2596011df763ap-2,
     0x1.7f40600000000p-1, 0x1.2895a0bde86a4p-2,
+    0x1.7e22560000000p-1, 0x1.2b93013789d31p-2,
     0x1.7d05f40000000p-1, 0x1.2e8e2bee11d31p-2,
+    0x1.7beb3a0000000p-1, 0x1.31871a4144190p-2,
     0x1.7ad2200000000p-1, 0x1.347ddb2987d59p-2,
+    0x1.79baa60000000p-1, 0x1.37726827fd863p-2,
     0x1.78a4c80000000p-1, 0x1.3a64c596945eap-2,
+    0x1.7790820000000p-1, 0x1.3d54f7e81f71cp-2,
     0x1.767dce0000000p-1, 0x1.404309206a7e5p-2,
+    0x1.756cac0000000p-1, 0x1.432ef2f84e814p-2,
     0x1.745d180000000p-1, 0x1.4618ba21c5ecap-2,
+    0x1.734f0c0000000p-1, 0x1.490068ec009d2p-2,
     0x1.7242880000000p-1, 0x1.4be5f937778a1p-2,
+    0x1.7137860000000p-1, 0x1.4ec9758200275p-2,
     0x1.702e060000000p-1, 0x1.51aad7c2df82ep-2,
+    0x1.6f26020000000p-1, 0x1.548a2aa6dd268p-2,
     0x1.6e1f760000000p-1, 0x1.5767736c55a74p-2,
+    0x1.6d1a620000000p-1, 0x1.5a42ac334cfe4p-2,
     0x1.6c16c20000000p-1, 0x1.5d1bda55809d0p-2,
+    0x1.6b14900000000p-1, 0x1.5ff308ea793dbp-2,
     0x1.6a13ce0000000p-1, 0x1.62c82c939c7a3p-2,
 };
 
@@ -589,24 +678,24 @@ GM_CONST uint64_t EXP2_T[256] = {
 // gfx950 takes no 64-bit literal), about 60 VGPRs that cost occupancy.
 enum CoefSlot : int {
     kExpC = 0,        // EXPM1_C[0..3]
-    kLog1pC = 4,      // LOG1P_C[0..5]
-    kSinT = 10,       // SINT_C[0..1]
-    kCosT = 12,       // COST_C[0..2]
-    kPio128 = 15,     // PIO128_HI, PIO128_LO
-    kInvPio128 = 17,
-    kPio128Scaled64 = 18,
-    kLn2 = 19,        // LN2_0, LN2_1, LN2_2
-    kLn2o256 = 22,    // LN2O256_HI, LN2O256_LO
-    kInvLn2x256 = 24,
-    kRoundMagic = 25,
-    kCoefCount = 26
+    kLog1pC = 4,      // LOG1P_C[0..4]
+    kSinT = 9,        // SINT_C[0..1]
+    kCosT = 11,       // COST_C[0..2]
+    kPio128 = 14,     // PIO128_HI, PIO128_LO
+    kInvPio128 = 16,
+    kPio128Scaled64 = 17,
+    kLn2Hi = 18,      // LN2_HI
+    kLn2o256 = 19,    // LN2O256_HI, LN2O256_LO
+    kInvLn2x256 = 21,
+    kRoundMagic = 22,
+    kCoefCount = 23
 };
 #define GM_COEF_INIT                                                              \
     {EXPM1_C[0], EXPM1_C[1], EXPM1_C[2], EXPM1_C[3],                              \
-     LOG1P_C[0], LOG1P_C[1], LOG1P_C[2], LOG1P_C[3], LOG1P_C[4], LOG1P_C[5],      \
+     LOG1P_C[0], LOG1P_C[1], LOG1P_C[2], LOG1P_C[3], LOG1P_C[4],                  \
      SINT_C[0], SINT_C[1], COST_C[0], COST_C[1], COST_C[2],                        \
      PIO128_HI, PIO128_LO, INV_PIO128, PIO128_SCALED64,                           \
-     LN2_0, LN2_1, LN2_2, LN2O256_HI, LN2O256_LO, INV_LN2_256, ROUND_MAGIC}
+     LN2_HI, LN2O256_HI, LN2O256_LO, INV_LN2_256, ROUND_MAGIC}
 
 #if defined(__HIP_DEVICE_COMPILE__)
 typedef const __attribute__((address_space(4))) double* CoefPtr;
@@ -725,32 +814,36 @@ GM_HD float expf_r(float x, CoefPtr C) {
 // d = hi - hi(S), e = d >> 20 and y's high word hi - (d & 0xfff00000) =
 // hi(S) + (d & 0xfffff): y lies in [S, 2S) whatever the bits of x, so the row
 // below is in range for every argument, 0, infinities and NaN included.  The
-// row is rint((y - 1) 128) + 37 in 0..90; returned here as rint(y 128) in
-// 91..181 (y 128 is exact, so rounding it by addition is rint).
-GM_HD double log_reduce(float x, int32_t& e, uint32_t& row91, CoefPtr C) {
+// row is rint((y - 1) 256) + 75 in 0..181; returned here as rint(y 256) in
+// kLogRowFirst..362 (y 256 is exact, so rounding it by addition is rint).
+constexpr uint32_t kLogRowFirst = 181;
+GM_HD double log_reduce(float x, int32_t& e, uint32_t& row181, CoefPtr C) {
     const uint64_t bits = d2u((double)x);
     const uint32_t hi = (uint32_t)(bits >> 32);
     const uint32_t dh = hi - kLogSplitHi;
     e = (int32_t)dh >> 20;
     const double y = u2d(((uint64_t)(hi - (dh & 0xfff00000u)) << 32) | (bits & 0xffffffffu));
-    row91 = (uint32_t)d2u(fma_d(y, 128.0, C[kRoundMagic]));  // y * 128 is exact
+    row181 = (uint32_t)d2u(fma_d(y, 256.0, C[kRoundMagic]));  // y * 256 is exact
     return y;
 }
 GM_HD float log_finish(double y, int32_t e, double cj, double lncj, CoefPtr C) {
     const double r = fma_d(y, cj, -1.0);  // exact: c_j has 24 bits
-    // log1p(r) to r^7: the rest is below |r|^8 / 8 < 1.2e-19 for |r| < 0.0056
-    double p = fma_d(vreg(C[kLog1pC + 5]), r, C[kLog1pC + 4]);
-    for (int n = 3; n >= 0; --n) p = fma_d(p, r, C[kLog1pC + n]);
+    // log1p(r) to r^6: the rest is below |r|^7 / 7, for |r| < 0.0028 less than
+    // 2^-53.8 |r|, and at row j = 0 (c_j = 1, |r| <= 1/512) less than 2^-56.8 |r|
+    double p = fma_d(vreg(C[kLog1pC + 4]), r, C[kLog1pC + 3]);
+    for (int n = 2; n >= 0; --n) p = fma_d(p, r, C[kLog1pC + n]);
     const double q = fma_d(r * r, p, r);  // log1p(r)
-    const double ed = (double)e;
-    // e ln2 + ln(1/c_j) + log1p(r); e * LN2_0 is exact
-    return (float)(fma_d(ed, C[kLn2], lncj) + fma_d(ed, C[kLn2 + 1], q));
+    // e ln2 + (ln(1/c_j) + log1p(r)) with one FMA, which forms e LN2_HI
+    // exactly.  LN2_HI, ln 2 rounded to a double, is within 2^-55.2 of it, and
+    // for e != 0 the result exceeds 0.34 |e|, so that term adds less than
+    // 2^-53.7 relative
+    return (float)fma_d((double)e, C[kLn2Hi], lncj + q);
 }
 GM_HD float logf_pos(float x, CoefPtr C) {
     int32_t e;
-    uint32_t row91;
-    const double y = log_reduce(x, e, row91, C);
-    const uint32_t j = (row91 - 91u) & 127u;  // 0..90; the mask only bounds it
+    uint32_t row181;
+    const double y = log_reduce(x, e, row181, C);
+    const uint32_t j = (row181 - kLogRowFirst) & 255u;  // 0..181; the mask only bounds it
     return log_finish(y, e, LOG_T[2 * j], LOG_T[2 * j + 1], C);
 }
 

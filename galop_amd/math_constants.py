@@ -224,9 +224,10 @@ def ln_fixed(y, prec):
 
 
 # log x = e ln2 + ln(1/c_j) + log1p(y c_j - 1) for y in [sqrt(1/2), sqrt(2)),
-# j = rint((y - 1) * 128) in [LOG_J_LO, LOG_J_HI]; c_j is 128 / (128 + j)
+# j = rint((y - 1) * 256) in [LOG_J_LO, LOG_J_HI]; c_j is 256 / (256 + j)
 # rounded to 24 bits, so y c_j is exact in double for a 24-bit y.
-LOG_J_LO, LOG_J_HI = -37, 53
+LOG_DIV = 256
+LOG_J_LO, LOG_J_HI = -75, 106
 
 
 def log_table():
@@ -234,7 +235,7 @@ def log_table():
     exactly: it has 24 bits)."""
     out = []
     for j in range(LOG_J_LO, LOG_J_HI + 1):
-        c = round_bits(Fraction(128, 128 + j), 24)
+        c = round_bits(Fraction(LOG_DIV, LOG_DIV + j), 24)
         assert Fraction(float(c)) == c
         lj = -ln_fixed(c, PREC) if j else 0
         out.extend([float(c), float(Fraction(lj, 1 << PREC)) + 0.0])
@@ -248,7 +249,8 @@ def constants():
         "PIO128_FMA": split_fma(pi, PREC + 7),
         "INV_PIO128": float(Fraction(128 << PREC, pi)),
         "PIO128_SCALED64": float(Fraction(pi, 1 << (PREC + 7)) / (1 << 64)),
-        "LN2": split(ln2, PREC, 32),
+        # ln 2 rounded to a double, and its rounding error (log's e ln2 term)
+        "LN2_HI": float(Fraction(ln2, 1 << PREC)),
         "LN2O256_FMA": split_fma(ln2, PREC + 8),
         "INV_LN2_256": float(Fraction(EXP_DIV << PREC, ln2)),
         # 1.5 2^52: adding it rounds |v| < 2^51 to an integer held in the low word
@@ -260,8 +262,8 @@ def constants():
         "SINT_POLY": taylor("sin", 3)[1:],   # -1/3!, 1/5!
         "COST_POLY": taylor("cos", 4)[1:],   # -1/2!, 1/4!, -1/6!
         "SINCOS_T": sincos_table(),
-        # log1p(r) = r + r^2 (-1/2 + r/3 - r^2/4 + ... + r^5/7)
-        "LOG1P_POLY": [float(Fraction((-1) ** (n + 1), n)) for n in range(2, 8)],
+        # log1p(r) = r + r^2 (-1/2 + r/3 - r^2/4 + r^3/5 - r^4/6)
+        "LOG1P_POLY": [float(Fraction((-1) ** (n + 1), n)) for n in range(2, 7)],
         "LOG_T": log_table(),
     }
     return c
