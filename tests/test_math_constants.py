@@ -389,3 +389,51 @@ def test_host_log_correctly_rounded_on_a_stride_of_floats():
     L.gm_map(7, x.ctypes.data, y.ctypes.data, x.size)
     assert (ulp_distance(y, np.log(x.astype(np.longdouble)).astype(np.float32)) == 1).all()
     assert checked > 22_000_000
+
+
+# The finite floats whose sin / cos the host build (and so the device) does not
+# round correctly: 1 ulp off each, found by comparing every finite float with
+# sinl / cosl rounded to float.
+SIN_ONE_ULP = [0x46199998, 0xC6199998]
+COS_ONE_ULP = [0x5F18B878, 0xDF18B878, 0x6115CB11, 0xE115CB11]
+
+
+@pytest.mark.parametrize("op,known", [(4, SIN_ONE_ULP), (5, COS_ONE_ULP)])
+def test_host_sincos_correctly_rounded_on_a_stride_of_floats(op, known):
+    """sin (op 4) and cos (op 5) of the host build equal sinl / cosl (64-bit
+    significand, glibc's own Payne-Hanek reduction) rounded to float on every
+    193rd finite float of either sign (22 million arguments) except the listed
+    ones, which are 1 ulp off; none of the arguments lies within 2^-58 of a
+    rounding midpoint.  (Over every finite float, 4,278,190,080 arguments,
+    the listed ones are the only misrounded ones.)"""
+    if np.finfo(np.longdouble).nmant < 63:
+        pytest.skip("long double is not the 64-bit-significand format here")
+    L = _host()
+    fn = np.sin if op == 4 else np.cos
+    kn = np.array(known, np.uint32)
+    bits = np.arange(0, 1 << 32, 193, dtype=np.uint64).astype(np.uint32)
+    bits = bits[(bits & 0x7F800000) != 0x7F800000]
+    bits = np.concatenate([bits, kn])
+    checked = 0
+    for part in np.array_split(bits, 8):
+        x = part.view(np.float32)
+        y = np.empty_like(x)
+        L.gm_map(op, x.ctypes.data, y.ctypes.data, x.size)
+        e = fn(x.astype(np.longdouble))
+        ref = e.astype(np.float32)
+        off = y.view(np.uint32) != ref.view(np.uint32)
+        assert set(part[off].tolist()) <= set(known)
+        assert (ulp_distance(y[off], ref[off]) == 1).all()
+        nz = ref != 0
+        c = ref[nz].astype(np.longdouble)
+        up = np.nextafter(ref[nz], np.float32(np.inf)).astype(np.longdouble)
+        dn = np.nextafter(ref[nz], np.float32(-np.inf)).astype(np.longdouble)
+        ee = e[nz]
+        gap = np.minimum(np.abs(ee - (c + up) / 2), np.abs(ee - (c + dn) / 2)) / np.abs(ee)
+        assert gap.min() > np.longdouble(2.0) ** -58
+        checked += x.size
+    x = kn.view(np.float32)
+    y = np.empty_like(x)
+    L.gm_map(op, x.ctypes.data, y.ctypes.data, x.size)
+    assert (ulp_distance(y, fn(x.astype(np.longdouble)).astype(np.float32)) == 1).all()
+    assert checked > 22_000_000
