@@ -13,8 +13,9 @@ Prints ONE JSON line on stdout; progress goes to stderr.
                   [--kernel plain|batched] [--programs P] [--no-cpu-baseline]
                   [--no-uneven] [--no-serving] [--calibrate-copy BYTES]
 The default run also measures, in the same process: the uneven-length
-population with the plain kernel vs the batched persistent generation, and
-the 64-run serving mix (p50/p99 latency).
+population with the plain kernel vs the batched persistent generation, the
+case-bound population (65,536 programs on 1,048,576 cases) with its FP64
+oracle check on a sample, and the 64-run serving mix (p50/p99 latency).
 """
 import argparse
 import json
@@ -35,14 +36,14 @@ SIMDS_PER_CU = 4               # one wave64 VALU instruction per SIMD per 4 cloc
 # programs, seed 0), measured with rocprofv3 --pmc; the commands and raw
 # figures are in profiles/plain_kernel_plain_workload_counters.md.  FETCH_SIZE
 # and WRITE_SIZE are scaled by what each reads for a 1 GiB device copy
-# (1,048,576 KiB copied: FETCH_SIZE 524,308.6 KiB, WRITE_SIZE 1,048,576 KiB).
+# (1,048,576 KiB copied: FETCH_SIZE 524,308.3 KiB, WRITE_SIZE 1,048,576 KiB).
 # They describe that exact population only.
 PLAIN_WORKLOAD_COUNTERS = {
-    "fetch_bytes": 319481.4 * 1024 * (1048576 / 524308.6),
+    "fetch_bytes": 319466.9 * 1024 * (1048576 / 524308.3),
     "write_bytes": 16384.0 * 1024 * 1.0,
-    "valu_insts": 20421542997,
-    "salu_insts": 7011908528,
-    "valu_busy_pct": 82.7,
+    "valu_insts": 19230657227,
+    "salu_insts": 6897884592,
+    "valu_busy_pct": 80.7,
 }
 
 
@@ -370,11 +371,11 @@ def bench_c1(ctx, seeds, generations=50):
 
 
 def bench_casebound(ctx, programs, check, steps, warmup, threads):
-    """The case-bound population at full shape (1,048,576 cases of 5 inputs,
-    16 registers, 128-instruction programs), timed on its first `programs`
-    programs (4,096 or more run one wave per program, as the whole 65,536 do),
-    with the FP64 oracle's SSE and fitness computed for the first `check` of
-    them: the device-math oracle (only the summation differs) for the SSE error,
+    """The case-bound population (1,048,576 cases of 5 inputs, 16 registers,
+    128-instruction programs), timed on its first `programs` programs (all
+    65,536 by default; 4,096 or more run one wave per program, as the whole
+    population does), with the FP64 oracle's SSE and fitness computed for the
+    first `check` of them: the device-math oracle (only the summation differs) for the SSE error,
     the host-libm oracle for the fitness bar."""
     import torch
     from galop_amd import eval_csr_device, workloads
@@ -414,11 +415,14 @@ def bench_casebound(ctx, programs, check, steps, warmup, threads):
     over = np.nonzero(r_host > 1e-6)[0]
     attributed = sum(O.attributed(isa, words[offs[i]:offs[i + 1]], x, y, 1e-6, gpu_fitness=g_fit[i])
                      for i in over[:64])
+    whole = P >= 65536
     return {
-        "workload": f"first {P} of the 65536 x 128-instruction population; {wl.notes}",
+        "workload": (f"the whole 65536 x 128-instruction population; {wl.notes}" if whole else
+                     f"first {P} of the 65536 x 128-instruction population; {wl.notes}"),
         "programs": P, "cases": N, "ms": round(ms, 2),
         "gpops": round(ops / ms / 1e6, 1),
-        "whole_population_ms_extrapolated": round(ms * 65536 / P, 1),
+        "whole_population_ms": round(ms, 1) if whole else None,
+        "whole_population_ms_extrapolated": None if whole else round(ms * 65536 / P, 1),
         "oracle_checked_programs": C,
         "finite_programs": int(fin.sum()),
         "sse_rel_err_vs_fp64_oracle": {"max": float(r_sse.max()) if r_sse.size else 0.0,
@@ -463,9 +467,9 @@ def main():
                     help="skip the uneven-length batched vs plain comparison")
     ap.add_argument("--uneven-programs", type=int, default=0)
     ap.add_argument("--no-casebound", action="store_true",
-                    help="skip the case-bound sample (1,048,576 cases) with its FP64 oracle check")
-    ap.add_argument("--casebound-programs", type=int, default=4096,
-                    help="case-bound programs timed")
+                    help="skip the case-bound population (1,048,576 cases) with its FP64 oracle check")
+    ap.add_argument("--casebound-programs", type=int, default=65536,
+                    help="case-bound programs timed (the first this many of the 65,536)")
     ap.add_argument("--casebound-check", type=int, default=128,
                     help="case-bound programs checked against the FP64 oracle")
     ap.add_argument("--no-effective", action="store_true",
@@ -751,7 +755,7 @@ def main():
     if not args.no_casebound and args.workload == "plain":
         log("[bench] case-bound sample")
         threads = min(len(os.sched_getaffinity(0)), 16)
-        res = bench_casebound(ctx, args.casebound_programs, args.casebound_check, 3, 1, threads)
+        res = bench_casebound(ctx, args.casebound_programs, args.casebound_check, 2, 1, threads)
         log(f"[bench] casebound: {res}")
         line["casebound"] = res
     if not args.no_engine and args.workload == "plain":
