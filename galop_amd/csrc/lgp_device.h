@@ -329,8 +329,30 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
 // exp (EXT8 opcode 6): a word whose every argument has |x| > 104 (every
 // result +inf or +0, without arithmetic) takes the select-only path; the
 // smallest magnitude decides (min_abs: a NaN argument may or may not take
-// part, and both paths give it NaN).  Otherwise the 2^(j/256) rows of
-// kRowGroup cases are loaded before any of them is used.
+// part, and both paths give it NaN).  A word whose every argument has
+// |x| < 89.5 (by the largest magnitude; a NaN that takes part sends the word
+// to the general path) skips the clamp and the NaN select (expf_inrange:
+// the same bits).  Either way the 2^(j/256) rows of kRowGroup cases are
+// loaded before any of them is used.
+#define GALOP_EXP_GROUPS(REDUCE, FINISH)                                  \
+    do {                                                                  \
+        [[maybe_unused]] const RowRsrc rows = exp_rows();                 \
+        _Pragma("unroll") for (int g = 0; g < K; g += kRowGroup) {        \
+            double rr[kRowGroup];                                         \
+            uint32_t kw[kRowGroup];                                       \
+            uint64_t tb[kRowGroup];                                       \
+            _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) {       \
+                rr[i] = galop_math::REDUCE(bv[g + i], kw[i], C);          \
+                if constexpr (kIndexRows)                                 \
+                    tb[i] = load_row8(rows, kw[i] & 255u);                \
+                else                                                      \
+                    tb[i] = galop_math::EXP2_T[kw[i] & 255u];             \
+            }                                                             \
+            _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i)         \
+                cv[g + i] = FINISH;                                       \
+            GALOP_GROUP_FENCE();                                          \
+        }                                                                 \
+    } while (0)
 #define GALOP_EXP()                                                       \
     do {                                                                  \
         if (kc) {                                                         \
@@ -343,24 +365,12 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
             if (__all(min_abs<K>(bv) > 104.0f)) {                         \
                 _Pragma("unroll") for (int k = 0; k < K; ++k)             \
                     cv[k] = op_exp_special_value(bv[k]);                  \
+            } else if (__all(max_abs<K>(bv) < 89.5f)) {                   \
+                GALOP_EXP_GROUPS(exp_reduce_inrange,                      \
+                                 galop_math::exp_finish_inrange(rr[i], kw[i], tb[i], C)); \
             } else {                                                      \
-                [[maybe_unused]] const RowRsrc rows = exp_rows();         \
-                _Pragma("unroll") for (int g = 0; g < K; g += kRowGroup) {  \
-                    double rr[kRowGroup];                                 \
-                    uint32_t kw[kRowGroup];                               \
-                    uint64_t tb[kRowGroup];                               \
-                    _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) { \
-                        rr[i] = galop_math::exp_reduce(bv[g + i], kw[i], C); \
-                        if constexpr (kIndexRows)                         \
-                            tb[i] = load_row8(rows, kw[i] & 255u);        \
-                        else                                              \
-                            tb[i] = galop_math::EXP2_T[kw[i] & 255u];     \
-                    }                                                     \
-                    _Pragma("unroll") for (int i = 0; i < kRowGroup; ++i) \
-                        cv[g + i] = galop_math::exp_finish(bv[g + i], rr[i], kw[i], \
-                                                           tb[i], C);     \
-                    GALOP_GROUP_FENCE();                                  \
-                }                                                         \
+                GALOP_EXP_GROUPS(exp_reduce,                              \
+                                 galop_math::exp_finish(bv[g + i], rr[i], kw[i], tb[i], C)); \
             }                                                             \
             _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]); \
         }                                                                 \
@@ -533,6 +543,7 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
 #undef GALOP_BIN
 #undef GALOP_DIV
 #undef GALOP_EXP
+#undef GALOP_EXP_GROUPS
 #undef GALOP_SINCOS
 #undef GALOP_PLOG
 #undef GALOP_GROUP_FENCE

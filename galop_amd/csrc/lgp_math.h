@@ -803,6 +803,35 @@ GM_HD float expf_r(float x, CoefPtr C) {
     return exp_finish(x, r, kw, EXP2_T[kw & 255u], C);
 }
 
+// The same for an interpreter word whose arguments all have |x| < 89.5 or are
+// NaN: the clamp changes no such argument, and a NaN needs no select.  Its k
+// is a NaN whose low word is 0 (a float NaN's payload lands in bits 29..51
+// of the double), so its row and scaling are those of k = 0, r is NaN, and
+// a NaN reaches the result through the arithmetic (its sign may differ from
+// the argument's, which no operation of the interpreter tells apart).  For
+// every other argument the bits equal expf_r's.
+GM_HD double exp_reduce_inrange(float x, uint32_t& kw, CoefPtr C) {
+    const double xd = (double)x;
+    const double kb = fma_d(xd, C[kInvLn2x256], C[kRoundMagic]);
+    const double k = kb - C[kRoundMagic];
+    kw = (uint32_t)d2u(kb);
+    return fma_d(-k, C[kLn2o256 + 1], fma_d(-k, C[kLn2o256], xd));
+}
+GM_HD float exp_finish_inrange(double r, uint32_t kw, uint64_t row, CoefPtr C) {
+    const double T = u2d(((uint64_t)((uint32_t)(row >> 32) + (kw << 12)) << 32) |
+                         (row & 0xffffffffu));
+    double p = fma_d(vreg(C[kExpC + 3]), r, C[kExpC + 2]);
+    p = fma_d(p, r, C[kExpC + 1]);
+    p = fma_d(p, r, C[kExpC]);
+    const double q = p * r;
+    return (float)fma_d(T, q, T);
+}
+GM_HD float expf_inrange(float x, CoefPtr C) {
+    uint32_t kw;
+    const double r = exp_reduce_inrange(x, kw, C);
+    return exp_finish_inrange(r, kw, EXP2_T[kw & 255u], C);
+}
+
 // ---- log of a positive, finite, non-zero float --------------------------
 // In two steps, so that an interpreter word can issue the table loads of all
 // its cases before it uses any: log_reduce splits x and picks the table row,

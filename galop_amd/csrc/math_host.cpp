@@ -19,4 +19,9 @@ void gm_map(int op, const float* x, float* y, long n) {
         }
     }
 }
+// exp's form for words whose arguments all have |x| < 89.5 or are NaN
+void gm_map_exp_inrange(const float* x, float* y, long n) {
+    const galop_math::CoefPtr C = galop_math::coef_table();
+    for (long i = 0; i < n; ++i) y[i] = galop_math::expf_inrange(x[i], C);
+}
 }

@@ -347,6 +347,29 @@ def test_host_exp_correctly_rounded_on_a_stride_of_its_range():
     assert checked > 23_000_000
 
 
+def test_host_exp_inrange_form_equals_exp():
+    """The form of exp for interpreter words whose arguments all lie below 89.5
+    in magnitude (no clamp, no NaN select) gives expf_r's bits on every 97th
+    float with |x| < 89.5, zeros and subnormals included (on every such float
+    in a one-off comparison), and a NaN for every NaN."""
+    L = _host()
+    L.gm_map_exp_inrange.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_long]
+    top = int(np.float32(89.5).view(np.uint32))
+    bits = np.concatenate([np.arange(0, top, 97, dtype=np.uint64),
+                           np.arange(0x80000000, 0x80000000 + top, 97, dtype=np.uint64)]).astype(np.uint32)
+    for part in np.array_split(bits, 4):
+        x = part.view(np.float32)
+        want = np.empty_like(x)
+        got = np.empty_like(x)
+        L.gm_map(6, x.ctypes.data, want.ctypes.data, x.size)
+        L.gm_map_exp_inrange(x.ctypes.data, got.ctypes.data, x.size)
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    nans = np.array([0x7FC00000, 0xFFC00000, 0x7F800001, 0x7FC12345, 0xFFFFFFFF], np.uint32).view(np.float32)
+    got = np.empty_like(nans)
+    L.gm_map_exp_inrange(nans.ctypes.data, got.ctypes.data, nans.size)
+    assert np.isnan(got).all()
+
+
 # The positive floats whose log the host build (and so the device) does not
 # round correctly: 1 ulp off each, found by comparing every positive finite
 # float with logl rounded to float.
