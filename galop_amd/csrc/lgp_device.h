@@ -518,8 +518,16 @@ __device__ __forceinline__ void exec_word(uint32_t w, RegFile<NR, K>& r,
         } else {                                                          \
             float bv[K];                                                  \
             _Pragma("unroll") for (int k = 0; k < K; ++k) bv[k] = r.get(k, s2); \
-            _Pragma("unroll") for (int k = 0; k < K; ++k)                 \
-                cv[k] = op_div(av[k], bv[k]);                             \
+            /* every divisor non-zero (a zero always makes the smallest */ \
+            /* magnitude 0 or NaN): the same division without the */     \
+            /* protection select */                                       \
+            if (__all(min_abs<K>(bv) > 0.0f)) {                           \
+                _Pragma("unroll") for (int k = 0; k < K; ++k)             \
+                    cv[k] = av[k] / bv[k];                                \
+            } else {                                                      \
+                _Pragma("unroll") for (int k = 0; k < K; ++k)             \
+                    cv[k] = op_div(av[k], bv[k]);                         \
+            }                                                             \
         }                                                                 \
         _Pragma("unroll") for (int k = 0; k < K; ++k) r.set(k, d, cv[k]);    \
     } while (0)
