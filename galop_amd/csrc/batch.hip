@@ -208,9 +208,15 @@ static hipError_t time_tuning(galop_batch* b, const BatchTuning& t, const uint32
     return e;
 }
 
-// The search: cases per lane x waves per block x persistent waves per CU x
-// programs per queue grab with the vector program staging, then the scalar
-// staging at the best of those.
+// The search: cases per lane x waves per block x persistent waves per CU
+// with the vector program staging, then the scalar staging at the best of
+// those.  The queue grab is not searched: every grab is an atomic on one
+// device-wide counter, so fewer, larger grabs win (on the 4,194,304-program
+// uneven workload 27.0 ms at 32-64 programs per grab, 27.3 at 8, 28.9 at 4,
+// 54 at 1), and a prefix cannot show it, because the launch caps the grab at
+// a quarter of the programs per wave (on a 131,072-program prefix at 8).  The
+// tuning keeps the largest grab, 64, and each launch caps it for its own
+// population.
 static int autotune_search(galop_batch* b, const uint32_t* d_words, const int64_t* d_offsets,
                            int64_t n_programs, float* d_fitness) {
     galop_ctx* ctx = b->ctx;
@@ -225,7 +231,6 @@ static int autotune_search(galop_batch* b, const uint32_t* d_words, const int64_
              ks.end());
     const int wpbs[] = {1, 2, 4};  // kernels are built for at most 256 threads per block
     const int workers[] = {8, 16, 24, 32};
-    const int chunks[] = {1, 4, 16};
     hipEvent_t a, z;
     hipError_t e = hipEventCreate(&a);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipEventCreate");
@@ -237,22 +242,21 @@ static int autotune_search(galop_batch* b, const uint32_t* d_words, const int64_
     BatchTuning best_t = b->tuning;
     for (int k : ks)
         for (int w : wpbs)
-            for (int wk : workers)
-                for (int ch : chunks) {
-                    BatchTuning t;
-                    t.cases_per_lane = k;
-                    t.waves_per_block = w;
-                    t.workers_per_cu = wk;
-                    t.chunk = ch;
-                    t.program_staging = kStageVector;
-                    float ms = 0.0f;
-                    e = time_tuning(b, t, d_words, d_offsets, sample, d_fitness, a, z, &ms);
-                    if (e != hipSuccess) break;
-                    if (ms < best) {
-                        best = ms;
-                        best_t = t;
-                    }
+            for (int wk : workers) {
+                BatchTuning t;
+                t.cases_per_lane = k;
+                t.waves_per_block = w;
+                t.workers_per_cu = wk;
+                t.chunk = kMaxQueueGrab;
+                t.program_staging = kStageVector;
+                float ms = 0.0f;
+                e = time_tuning(b, t, d_words, d_offsets, sample, d_fitness, a, z, &ms);
+                if (e != hipSuccess) break;
+                if (ms < best) {
+                    best = ms;
+                    best_t = t;
                 }
+            }
     if (e == hipSuccess) {
         BatchTuning t = best_t;
         t.program_staging = kStageScalar;

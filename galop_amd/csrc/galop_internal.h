@@ -81,6 +81,9 @@ struct BatchScratch {
     int64_t max_programs;
 };
 constexpr int kBuckets = 64;
+// Most programs a persistent wave takes per queue grab: one per lane, whose
+// id and bounds it loads in one go.
+constexpr int kMaxQueueGrab = 64;
 
 struct BatchTuning {
     int cases_per_lane = 0;

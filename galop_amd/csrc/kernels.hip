@@ -890,8 +890,8 @@ BatchTuning resolve_batch_tuning(const DevData& ds, int num_registers, BatchTuni
     if (num_registers == 16 && t.cases_per_lane > 4) t.cases_per_lane = 4;
     if (t.waves_per_block <= 0 || t.waves_per_block > 4) t.waves_per_block = 4;
     if (t.workers_per_cu <= 0) t.workers_per_cu = 32;
-    if (t.chunk <= 0) t.chunk = 4;
-    if (t.chunk > kWave) t.chunk = kWave;
+    if (t.chunk <= 0) t.chunk = kMaxQueueGrab;
+    if (t.chunk > kMaxQueueGrab) t.chunk = kMaxQueueGrab;
     if (t.program_staging != kStageScalar) t.program_staging = kStageVector;
     return t;
 }

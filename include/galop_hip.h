@@ -239,13 +239,18 @@ typedef struct galop_batch_tuning {
     int32_t cases_per_lane;   /* 1, 2, 4, or 8 (8 registers only)           */
     int32_t waves_per_block;  /* 1..4                                        */
     int32_t workers_per_cu;   /* persistent waves per compute unit, 1..64    */
-    int32_t chunk;            /* programs per work-queue grab, 1..64         */
+    int32_t chunk;            /* programs per work-queue grab, 1..64; each
+                                 launch caps it at a quarter of its programs
+                                 per persistent wave                         */
     int32_t program_staging;  /* GALOP_STAGE_*                               */
     int32_t reserved;         /* 0                                           */
 } galop_batch_tuning;
-/* Autotune (cases per lane x waves per block x persistent waves per CU x
- * programs per queue grab, then the program staging at the best of those)
- * on a prefix of the given population; the winner is kept in the plan.
+/* Autotune (cases per lane x waves per block x persistent waves per CU, then
+ * the program staging at the best of those) on a prefix of the given
+ * population; the winner is kept in the plan.  The programs per queue grab
+ * are not searched: they stay at 64, which each launch caps for its own
+ * population (every grab is an atomic on one device-wide counter, and a
+ * prefix cannot show what fewer grabs save on the whole population).
  * Writes the chosen values (any output may be NULL; galop_batch_get_tuning
  * gives the staging too).  No counterpart in the reference, whose launch
  * shape is fixed (src/EvaluatorCuda.cu:424-450). */
