@@ -36,14 +36,14 @@ SIMDS_PER_CU = 4               # one wave64 VALU instruction per SIMD per 4 cloc
 # programs, seed 0), measured with rocprofv3 --pmc; the commands and raw
 # figures are in profiles/plain_kernel_plain_workload_counters.md.  FETCH_SIZE
 # and WRITE_SIZE are scaled by what each reads for a 1 GiB device copy
-# (1,048,576 KiB copied: FETCH_SIZE 524,308.3 KiB, WRITE_SIZE 1,048,576 KiB).
+# (1,048,576 KiB copied: FETCH_SIZE 524,308.1 KiB, WRITE_SIZE 1,048,576 KiB).
 # They describe that exact population only.
 PLAIN_WORKLOAD_COUNTERS = {
-    "fetch_bytes": 319466.9 * 1024 * (1048576 / 524308.3),
+    "fetch_bytes": 319482.8 * 1024 * (1048576 / 524308.1),
     "write_bytes": 16384.0 * 1024 * 1.0,
-    "valu_insts": 19230657227,
-    "salu_insts": 6897884592,
-    "valu_busy_pct": 80.7,
+    "valu_insts": 18736280694,
+    "salu_insts": 7093271714,
+    "valu_busy_pct": 80.0,
 }
 
 
