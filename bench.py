@@ -489,7 +489,8 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
-              "(populations shard across GPUs without communication; see DESIGN.md)",
+              "(a population shards across GPUs by program ranges, without communication: "
+              "run one process per GPU on its own range)",
               file=sys.stderr)
         return 2
 
