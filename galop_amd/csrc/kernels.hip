@@ -279,18 +279,18 @@ hipError_t launch_const_table(float* consts, hipStream_t stream) {
 // compacted programs are laid out by order position i.
 // ---------------------------------------------------------------------------
 struct DecodeCfg {
-    uint32_t mask;   // NR - 1
+    uint32_t mask;        // NR - 1
+    uint32_t dmask;       // the destination field's mask: NR/2 - 1 under INPUT_PROTECT
+    int dbase;            // added to the destination field: NR/2 under INPUT_PROTECT
+    uint32_t unary_mask;  // bit op set for the unary opcodes of the op table
     int out_reg;
-    int iprot;
-    int ext;
 };
 
 __device__ __forceinline__ bool effective_step(uint32_t w, const DecodeCfg& c, uint32_t& live) {
-    const int d = c.iprot ? c.out_reg + (int)((w >> 8) & (c.mask >> 1)) : (int)((w >> 8) & c.mask);
+    const int d = (int)((w >> 8) & c.dmask) + c.dbase;
     if (!((live >> d) & 1u)) return false;
     live &= ~(1u << d);
-    const int op = (int)(w & 7u);
-    const bool unary = (op == 4 || op == 5) || (c.ext && (op == 6 || op == 7));
+    const bool unary = (c.unary_mask >> (w & 7u)) & 1u;
     if (!unary) live |= 1u << ((w >> 16) & c.mask);
     if (!(w >> 31)) live |= 1u << ((w >> 24) & c.mask);
     return true;
@@ -315,6 +315,10 @@ __device__ __forceinline__ void walk_backward(const uint32_t* __restrict__ words
     int top = (int)(hi - seg);                            // words below hi in this segment, 1..32
     for (;;) {
         const int bot = seg >= lo ? 0 : (int)(lo - seg);  // words below lo in this segment
+        // bit t set for the segment's words t in [bot, top): one 32-bit test
+        // per word (a comparison of t with the bounds was done on 64-bit
+        // word indices)
+        const uint32_t take = (top >= kSeg ? ~0u : (1u << top) - 1u) & ~((1u << bot) - 1u);
         // global loads (a flat load would also wait on the LDS counter)
         typedef uint32_t U32x4 __attribute__((ext_vector_type(4)));
         typedef const __attribute__((address_space(1))) U32x4* GlobalBlocks;
@@ -324,7 +328,7 @@ __device__ __forceinline__ void walk_backward(const uint32_t* __restrict__ words
 #pragma unroll
         for (int t = kSeg - 1; t >= 0; --t) {
             const uint32_t w = v[t / 4][t & 3];
-            if (t < top && t >= bot) f(w);
+            if (take & (1u << t)) f(w);
         }
         if (seg <= lo) break;
         seg -= kSeg;
@@ -357,24 +361,28 @@ __global__ void effective_count_kernel(DecodeCfg cfg, const uint32_t* __restrict
 // Only positions of this program's own effective words below `lim` are ever
 // written.
 struct BackWriter {
-    uint32_t* __restrict__ out;
-    int64_t end, lim;
+    uint32_t* __restrict__ pend;  // out + end: position end - c is pend - c
+    int over;                     // positions end - c with c <= over lie past lim
+    uint32_t ealign;              // end & 3: position end - c is a multiple of 4 iff (c & 3) == ealign
     bool vec_ok;
     int cnt = 0;  // words written so far (a program has fewer than 2^31 words)
     uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
 
     __device__ BackWriter(uint32_t* o, int64_t e, int64_t l, bool v)
-        : out(o), end(e), lim(l), vec_ok(v) {}
-    // positions pos .. pos + n - 1 from s0 .. s(n-1), none at or past lim
-    __device__ __forceinline__ void store(int64_t pos, int n) {
-        if (vec_ok && n == 4 && pos + 4 <= lim) {
-            *reinterpret_cast<uint4*>(out + pos) = make_uint4(s0, s1, s2, s3);
+        : pend(o + e), over((int)(e - l < (int64_t(1) << 30) ? e - l : (int64_t(1) << 30))),
+          ealign((uint32_t)e & 3u), vec_ok(v) {}
+    // positions end - c .. end - c + n - 1 from s0 .. s(n-1), none at or past
+    // lim (position end - c + j is below lim iff c - j > over)
+    __device__ __forceinline__ void store(int c, int n) {
+        uint32_t* p = pend - c;
+        if (vec_ok && n == 4 && c - 4 >= over) {
+            *reinterpret_cast<uint4*>(p) = make_uint4(s0, s1, s2, s3);
             return;
         }
-        if (n > 0 && pos < lim) out[pos] = s0;
-        if (n > 1 && pos + 1 < lim) out[pos + 1] = s1;
-        if (n > 2 && pos + 2 < lim) out[pos + 2] = s2;
-        if (n > 3 && pos + 3 < lim) out[pos + 3] = s3;
+        if (n > 0 && c > over) p[0] = s0;
+        if (n > 1 && c - 1 > over) p[1] = s1;
+        if (n > 2 && c - 2 > over) p[2] = s2;
+        if (n > 3 && c - 3 > over) p[3] = s3;
     }
     __device__ __forceinline__ void push(uint32_t w) {
         s3 = s2;
@@ -384,12 +392,11 @@ struct BackWriter {
         ++cnt;
         // s_j for j >= cnt are not this program's: their positions are at or
         // past end, and store() writes none of them
-        if ((((uint32_t)end - (uint32_t)cnt) & 3u) == 0u) store(end - cnt, 4);
+        if (((uint32_t)cnt & 3u) == ealign) store(cnt, 4);
     }
     __device__ __forceinline__ void finish() {
-        const int64_t pos = end - cnt;
-        const int r = (int)(pos & 3);
-        if (cnt > 0 && r != 0) store(pos, min(4 - r, cnt));
+        const int r = (int)((ealign - (uint32_t)cnt) & 3u);  // (end - cnt) & 3
+        if (cnt > 0 && r != 0) store(cnt, min(4 - r, cnt));
     }
 };
 
@@ -449,9 +456,12 @@ __global__ void clamp_offsets_kernel(int64_t* offsets, int64_t n1, int64_t cap) 
 static DecodeCfg make_cfg(int nr, int op_set, int iprot) {
     DecodeCfg c;
     c.mask = (uint32_t)nr - 1u;
-    c.iprot = iprot;
+    c.dmask = iprot ? (uint32_t)(nr / 2 - 1) : c.mask;
+    c.dbase = iprot ? nr / 2 : 0;
+    // SIN, COS (src/ISA.h:209-212,225-226: the first operand is ignored); exp and log
+    // in the EXT8 table
+    c.unary_mask = 0x30u | (op_set ? 0xc0u : 0u);
     c.out_reg = iprot ? nr / 2 : 0;
-    c.ext = op_set;
     return c;
 }
 
